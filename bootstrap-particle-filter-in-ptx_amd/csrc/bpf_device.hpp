@@ -1,0 +1,176 @@
+// bpf_device.hpp -- per-particle arithmetic of the SV bootstrap filter for
+// gfx950.  Every operation is an IEEE correctly rounded +, -, *, /, sqrt, an
+// explicit fmaf, or integer/bit work, and the file is compiled with
+// -ffp-contract=off, so each result is reproducible bit for bit by any IEEE
+// CPU that performs the same sequence.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bpf {
+
+constexpr int kBlock = 256;          // threads per block of the per-tile kernels
+constexpr int kWave = 64;            // CDNA wavefront width
+constexpr uint64_t kPcgMult = 6364136223846793005ULL;
+constexpr uint64_t kPcgHostInc = 1442695040888963407ULL;
+
+__device__ __host__ inline float bits_f32(uint32_t b) { return __builtin_bit_cast(float, b); }
+__device__ __host__ inline uint32_t f32_bits(float f) { return __builtin_bit_cast(uint32_t, f); }
+
+// PCG32 XSH-RR output of the pre-step state
+// (ptx_assembly/bpf_kernels_full.ptx:146-157).
+__device__ __host__ inline uint32_t pcg_output(uint64_t old) {
+    uint32_t xsh = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xsh >> rot) | (xsh << ((0u - rot) & 31u));
+}
+
+// Uniform of ptx:159-164: (r >> 9) * 2^-23 clamped to [1e-6, 0x3F7FFBCE].
+__device__ __host__ inline float pcg_uniform(uint32_t r) {
+    float u = (float)(r >> 9) * (1.0f / 8388608.0f);
+    u = fmaxf(u, bits_f32(0x358637BDu));
+    u = fminf(u, bits_f32(0x3F7FFBCEu));
+    return u;
+}
+
+// exp(x) after S. L. Moshier's Cephes expf: Cody-Waite reduction by ln 2
+// (C1 = 0.693359375, C2 = -2.12194440e-4), degree-5 polynomial, exponent
+// added in the bits.  x < -86 gives 0 (no subnormal results), x > 88 gives
+// +inf, NaN propagates.
+__device__ inline float expf_cw(float x) {
+    float xc = fminf(fmaxf(x, -86.0f), 88.0f);
+    float z = __builtin_rintf(xc * 1.44269504088896341f);
+    float r = __builtin_fmaf(z, -0.693359375f, xc);
+    r = __builtin_fmaf(z, 2.12194440e-4f, r);
+    float p = 1.9875691500e-4f;
+    p = __builtin_fmaf(p, r, 1.3981999507e-3f);
+    p = __builtin_fmaf(p, r, 8.3334519073e-3f);
+    p = __builtin_fmaf(p, r, 4.1665795894e-2f);
+    p = __builtin_fmaf(p, r, 1.6666665459e-1f);
+    p = __builtin_fmaf(p, r, 5.0000001201e-1f);
+    float rr = r * r;
+    p = __builtin_fmaf(p, rr, r);
+    p = p + 1.0f;
+    int n = (int)z;
+    float res = bits_f32(f32_bits(p) + ((uint32_t)n << 23));
+    res = (x > 88.0f) ? __builtin_inff() : res;
+    res = (x >= -86.0f || x > 88.0f) ? res : ((x != x) ? x : 0.0f);
+    return res;
+}
+
+// log(x) after Moshier's Cephes logf: x = m 2^e with m in [sqrt(1/2),
+// sqrt(2)), degree-8 polynomial in m-1, e*ln2 added in two parts.
+__device__ inline float logf_cw(float x) {
+    uint32_t b = f32_bits(x);
+    int sub = (b >> 23) == 0;                         // zero or subnormal
+    uint32_t bs = sub ? f32_bits(x * 33554432.0f) : b;
+    int e = (int)(bs >> 23) - 126 - (sub ? 25 : 0);
+    float m = bits_f32((bs & 0x007FFFFFu) | 0x3F000000u);
+    int lo = m < 0.707106781186547524f;
+    e -= lo;
+    m = lo ? (m + m - 1.0f) : (m - 1.0f);
+    float z = m * m;
+    float y = 7.0376836292e-2f;
+    y = __builtin_fmaf(y, m, -1.1514610310e-1f);
+    y = __builtin_fmaf(y, m, 1.1676998740e-1f);
+    y = __builtin_fmaf(y, m, -1.2420140846e-1f);
+    y = __builtin_fmaf(y, m, 1.4249322787e-1f);
+    y = __builtin_fmaf(y, m, -1.6668057665e-1f);
+    y = __builtin_fmaf(y, m, 2.0000714765e-1f);
+    y = __builtin_fmaf(y, m, -2.4999993993e-1f);
+    y = __builtin_fmaf(y, m, 3.3333331174e-1f);
+    y = (y * m) * z;
+    float fe = (float)e;
+    y = __builtin_fmaf(fe, -2.12194440e-4f, y);
+    y = __builtin_fmaf(-0.5f, z, y);
+    float r = m + y;
+    r = __builtin_fmaf(fe, 0.693359375f, r);
+    // special values
+    r = (x == __builtin_inff()) ? x : r;
+    r = (x == 0.0f) ? -__builtin_inff() : r;
+    r = (x < 0.0f || x != x) ? __builtin_nanf("") : r;
+    return r;
+}
+
+// Acklam (2003) inverse normal CDF with the FP32 coefficients of ptx:172-203.
+// Central region |u - 1/2| < 0.47575 (0x3EF39581, ptx:169): rational in
+// r = q^2.  Valid for every q; callers select it only for central lanes.
+__device__ inline float icdf_central(float q) {
+    float r = q * q;
+    float num = __builtin_fmaf(r, -39.69683028665376f, 220.9460984245205f);
+    num = __builtin_fmaf(r, num, -275.9285104469687f);
+    num = __builtin_fmaf(r, num, 138.3577518672690f);
+    num = __builtin_fmaf(r, num, -30.66479806614716f);
+    num = __builtin_fmaf(r, num, 2.506628277459239f);
+    num = q * num;
+    float den = __builtin_fmaf(r, -54.47609879822406f, 161.5858368580409f);
+    den = __builtin_fmaf(r, den, -155.6989798598866f);
+    den = __builtin_fmaf(r, den, 66.80131188771972f);
+    den = __builtin_fmaf(r, den, -13.28068155288572f);
+    den = __builtin_fmaf(r, den, 1.0f);
+    return num / den;
+}
+
+__device__ inline bool icdf_is_central(float u) {
+    return __builtin_fabsf(u - 0.5f) < bits_f32(0x3EF39581u);
+}
+
+// Tails (ptx:185-205): t = sqrt(-2 log p), p = min(u, 1-u), negated above 1/2.
+__device__ inline float icdf_tail(float u) {
+    int low = u < 0.5f;
+    float p = low ? u : 1.0f - u;
+    float t = __builtin_sqrtf(-2.0f * logf_cw(p));
+    float num = __builtin_fmaf(t, -7.784894002430293e-03f, -3.223964580411365e-01f);
+    num = __builtin_fmaf(t, num, -2.400758277161838e+00f);
+    num = __builtin_fmaf(t, num, -2.549732539343734e+00f);
+    num = __builtin_fmaf(t, num, 4.374664141464968e+00f);
+    num = __builtin_fmaf(t, num, 2.938163982698783e+00f);
+    float den = __builtin_fmaf(t, 7.784695709041462e-03f, 3.224671290700398e-01f);
+    den = __builtin_fmaf(t, den, 2.445134137142996e+00f);
+    den = __builtin_fmaf(t, den, 3.754408661907416e+00f);
+    den = __builtin_fmaf(t, den, 1.0f);
+    float z = num / den;
+    return low ? z : -z;
+}
+
+__device__ inline float icdf(float u) {
+    return icdf_is_central(u) ? icdf_central(u - 0.5f) : icdf_tail(u);
+}
+
+// SV transition of ptx:406-412: x' = mu + phi (x - mu) + sigma eps.
+__device__ inline float transition(float x, float mu, float phi, float sig, float z) {
+    float d = x - mu;
+    d = phi * d;
+    float xn = mu + d;
+    return __builtin_fmaf(sig, z, xn);
+}
+
+// Observation log-weight of ptx:414-450.  Gaussian:
+//   -0.9189385 - y^2 e^{-x} / 2 - x / 2
+// Student-t (nu_obs > 0): C_obs - (nu+1)/2 log(1 + y^2 e^{-x} / nu) - x / 2.
+struct ObsModel {
+    float ysq;      // y_t^2
+    float nu;       // 0 = Gaussian
+    float c_obs;    // Student-t normalising constant (host FP64 lgamma)
+    float nph;      // (nu + 1) / 2
+};
+
+__device__ inline float log_weight(float x, const ObsModel& o) {
+    float eta2 = o.ysq * expf_cw(-x);
+    if (o.nu > 0.0f) {
+        float ta = eta2 / o.nu + 1.0f;
+        float lw = __builtin_fmaf(-o.nph, logf_cw(ta), o.c_obs);
+        return __builtin_fmaf(-0.5f, x, lw);
+    }
+    float lw = __builtin_fmaf(-0.5f, eta2, bits_f32(0xBF6B3F8Eu));
+    return __builtin_fmaf(-0.5f, x, lw);
+}
+
+// Systematic-resampling target of gpu_bpf/gpu_bpf.cu:300-302 (IEEE
+// division), before the wrap at 1.
+__device__ __host__ inline float target_unwrapped(float u, int i, int n) {
+    return u + (float)i / (float)n;
+}
+
+}  // namespace bpf

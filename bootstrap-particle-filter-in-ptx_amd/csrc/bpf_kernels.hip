@@ -1,0 +1,641 @@
+// bpf_kernels.hip -- gfx950 kernels of one bootstrap-filter tick.
+//
+// A tick is at most three launches on the filter's stream:
+//   k_propagate  (every tick)  read x [and l], write x' [and l'], per-tile
+//                              partials {max, sum e, sum e x, sum e x^2, sum e^2}
+//   k_finalize   (1 block)     global max / sums in FP64, statistics, FP64
+//                              log-likelihood, ESS decision, host-stream
+//                              uniform, FP64 tile prefix for the CDF
+//   k_resample   (every tick)  exits at once unless the finalise pass chose to
+//                              resample; else per tile: FP64 scan of the
+//                              weights, claim of the systematic targets that
+//                              fall in the tile's CDF interval, coalesced
+//                              write of the offspring into the other buffer
+// The reference issues 12-16 launches and two host syncs per tick
+// (gpu_bpf/gpu_bpf_ptx_full.cu:591-745).
+#include <hip/hip_runtime.h>
+
+#include "bpf_types.hpp"
+
+namespace bpf {
+
+// ---------------------------------------------------------------------------
+// Wavefront / block helpers (wave64)
+// ---------------------------------------------------------------------------
+
+__device__ inline float wave_max(float v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+__device__ inline int wave_incl_scan(int v, int lane) {
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        int o = __shfl_up(v, off, kWave);
+        v += (lane >= off) ? o : 0;
+    }
+    return v;
+}
+
+__device__ inline double wave_incl_scan(double v, int lane) {
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        double o = __shfl_up(v, off, kWave);
+        v += (lane >= off) ? o : 0.0;
+    }
+    return v;
+}
+
+__device__ inline double wave_incl_max(double v, int lane) {
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        double o = __shfl_up(v, off, kWave);
+        v = (lane >= off) ? fmax(v, o) : v;
+    }
+    return v;
+}
+
+template <int VEC>
+__device__ inline void load_vec(const float* p, float* v) {
+    if constexpr (VEC == 4) {
+        float4 t = *reinterpret_cast<const float4*>(p);
+        v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+    } else if constexpr (VEC == 2) {
+        float2 t = *reinterpret_cast<const float2*>(p);
+        v[0] = t.x; v[1] = t.y;
+    } else {
+        v[0] = *p;
+    }
+}
+
+template <int VEC>
+__device__ inline void store_vec(float* p, const float* v) {
+    if constexpr (VEC == 4) {
+        *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    } else if constexpr (VEC == 2) {
+        *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
+    } else {
+        *p = v[0];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Initialisation: x_0 = mu + sd * z, z from draw 0 of each particle's stream
+// (ptx_assembly/bpf_kernels_full.ptx:103-218, sd of gpu_bpf_ptx_full.cu:430).
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_init(InitArgs a) {
+    long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.n_pad) return;
+    float x = a.mu;
+    if (i < a.n) {
+        uint64_t s = a.pcg_a + (uint64_t)(2 * i + 1) * a.pcg_g;
+        float z = icdf(pcg_uniform(pcg_output(s)));
+        x = __builtin_fmaf(a.sd, z, a.mu);
+    }
+    a.x[i] = x;
+}
+
+// ---------------------------------------------------------------------------
+// Propagate + weight + tile partials.
+// Tile b holds particles [b*TILE, (b+1)*TILE); lane `tid` owns VEC-wide
+// groups at (v*kBlock + tid)*VEC, so every load and store instruction of a
+// wave covers one contiguous 64*VEC*4-byte run.
+// ICDF tails (|u-1/2| >= 0.47575, 4.85% of draws) would make nearly every
+// wave execute the ~60-instruction tail branch; instead tail uniforms are
+// compacted into an LDS queue and evaluated densely by the whole block.
+// ---------------------------------------------------------------------------
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
+    constexpr int VEC = K < 4 ? K : 4;
+    constexpr int NV = K / VEC;
+    constexpr int TILE = kBlock * K;
+    constexpr int NW = kBlock / kWave;
+    __shared__ float s_q[TILE];
+    __shared__ int s_cnt[NW];
+    __shared__ float s_m[NW];
+    __shared__ double s_d[4][NW];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wid = tid / kWave;
+    const int b = blockIdx.x;
+    const long long base = (long long)b * TILE;
+
+    const int cur = a.ctl->cur;
+    const int carry = a.ctl->carry;
+    const float mprev = a.ctl->m_prev;
+    float* __restrict__ xb = cur ? a.x1 : a.x0;
+
+    float x[K], l[K];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        long long off = base + (long long)(v * kBlock + tid) * VEC;
+        load_vec<VEC>(xb + off, &x[v * VEC]);
+    }
+    if (carry) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            long long off = base + (long long)(v * kBlock + tid) * VEC;
+            load_vec<VEC>(a.lw + off, &l[v * VEC]);
+        }
+    }
+
+    ObsModel obs = a.obs;
+    {
+        float yv = a.y_ptr ? *a.y_ptr : a.y;
+        obs.ysq = yv * yv;
+    }
+
+    if (a.do_prop) {
+        float z[K];
+        bool tail[K];
+        int nt = 0;
+        const uint64_t g2 = a.pcg_g * 2u;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            long long i0 = base + (long long)(v * kBlock + tid) * VEC;
+            uint64_t s = a.pcg_a + (uint64_t)(2 * i0 + 1) * a.pcg_g;
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) {
+                const int k = v * VEC + c;
+                float u = pcg_uniform(pcg_output(s));
+                s += g2;
+                bool cen = icdf_is_central(u);
+                float zc = icdf_central(u - 0.5f);
+                z[k] = cen ? zc : u;    // tail items carry u until the queue pass
+                tail[k] = !cen;
+                nt += cen ? 0 : 1;
+            }
+        }
+        // block-wide exclusive scan of the per-lane tail counts
+        int incl = wave_incl_scan(nt, lane);
+        if (lane == kWave - 1) s_cnt[wid] = incl;
+        __syncthreads();
+        int wbase = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            int c = s_cnt[w];
+            wbase += (w < wid) ? c : 0;
+            total += c;
+        }
+        const int first = wbase + incl - nt;
+        int pos = first;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (tail[k]) s_q[pos] = z[k];
+            pos += tail[k] ? 1 : 0;
+        }
+        __syncthreads();
+        for (int e = tid; e < total; e += kBlock) s_q[e] = icdf_tail(s_q[e]);
+        __syncthreads();
+        pos = first;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (tail[k]) z[k] = s_q[pos];
+            pos += tail[k] ? 1 : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) x[k] = transition(x[k], a.mu, a.phi, a.sig, z[k]);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            long long off = base + (long long)(v * kBlock + tid) * VEC;
+            store_vec<VEC>(xb + off, &x[v * VEC]);
+        }
+    }
+
+    // log-weights, with the carry of gpu_bpf_ptx_full.cu:601-604 restated
+    // relative to the previous maximum
+    float mloc = -__builtin_inff();
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) {
+            const int k = v * VEC + c;
+            long long i = base + (long long)(v * kBlock + tid) * VEC + c;
+            float lw = log_weight(x[k], obs);
+            float lk = carry ? lw + (l[k] - mprev) : lw;
+            l[k] = (i < a.n) ? lk : -__builtin_inff();
+            mloc = fmaxf(mloc, l[k]);
+        }
+    }
+    if (a.store_l) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            long long off = base + (long long)(v * kBlock + tid) * VEC;
+            store_vec<VEC>(a.lw + off, &l[v * VEC]);
+        }
+    }
+
+    mloc = wave_max(mloc);
+    if (lane == 0) s_m[wid] = mloc;
+    __syncthreads();
+    float mb = s_m[0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) mb = fmaxf(mb, s_m[w]);
+
+    double S = 0.0, SX = 0.0, SXX = 0.0, Q = 0.0;
+    const bool dead = (mb == -__builtin_inff());
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) {
+            const int k = v * VEC + c;
+            long long i = base + (long long)(v * kBlock + tid) * VEC + c;
+            float e = dead ? 0.0f : expf_cw(l[k] - mb);
+            double de = (double)e;
+            double dx = (i < a.n) ? (double)x[k] : 0.0;
+            double t = de * dx;
+            S += de;
+            SX += t;
+            SXX = fma(t, dx, SXX);
+            Q = fma(de, de, Q);
+        }
+    }
+    S = wave_sum(S);
+    SX = wave_sum(SX);
+    SXX = wave_sum(SXX);
+    Q = wave_sum(Q);
+    if (lane == 0) {
+        s_d[0][wid] = S;
+        s_d[1][wid] = SX;
+        s_d[2][wid] = SXX;
+        s_d[3][wid] = Q;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double r[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double acc = s_d[q][0];
+#pragma unroll
+            for (int w = 1; w < NW; ++w) acc += s_d[q][w];
+            r[q] = acc;
+        }
+        a.part.m[b] = mb;
+        a.part.s[b] = r[0];
+        a.part.sx[b] = r[1];
+        a.part.sxx[b] = r[2];
+        a.part.sq[b] = r[3];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Finalise (one block of 1024 threads).
+// ---------------------------------------------------------------------------
+
+constexpr int kFinBlock = 1024;
+
+__device__ inline float host_pcg_float(unsigned long long* st) {
+    unsigned long long old = *st;
+    *st = old * kPcgMult + kPcgHostInc;
+    return (float)(pcg_output(old) >> 9) * (1.0f / 8388608.0f);
+}
+
+__global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
+    constexpr int NW = kFinBlock / kWave;
+    __shared__ float s_f[NW];
+    __shared__ double s_d[4][NW];
+    __shared__ double s_scan[NW];
+    __shared__ int s_flag;
+    __shared__ double s_S;
+    __shared__ float s_M;
+
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const int G = a.n_tiles;
+
+    float m = -__builtin_inff();
+    for (int b = tid; b < G; b += kFinBlock) m = fmaxf(m, a.part.m[b]);
+    m = wave_max(m);
+    if (lane == 0) s_f[wid] = m;
+    __syncthreads();
+    if (tid == 0) {
+        float mm = s_f[0];
+        for (int w = 1; w < NW; ++w) mm = fmaxf(mm, s_f[w]);
+        s_M = mm;
+    }
+    __syncthreads();
+    const float M = s_M;
+
+    double S = 0.0, SX = 0.0, SXX = 0.0, Q = 0.0;
+    for (int b = tid; b < G; b += kFinBlock) {
+        float mb = a.part.m[b];
+        if (mb == -__builtin_inff()) continue;
+        double f = (double)expf_cw(mb - M);
+        S += f * a.part.s[b];
+        SX += f * a.part.sx[b];
+        SXX += f * a.part.sxx[b];
+        Q += (f * f) * a.part.sq[b];
+    }
+    S = wave_sum(S);
+    SX = wave_sum(SX);
+    SXX = wave_sum(SXX);
+    Q = wave_sum(Q);
+    if (lane == 0) {
+        s_d[0][wid] = S;
+        s_d[1][wid] = SX;
+        s_d[2][wid] = SXX;
+        s_d[3][wid] = Q;
+    }
+    __syncthreads();
+
+    if (tid == 0) {
+        double t4[4];
+        for (int q = 0; q < 4; ++q) {
+            double acc = s_d[q][0];
+            for (int w = 1; w < NW; ++w) acc += s_d[q][w];
+            t4[q] = acc;
+        }
+        S = t4[0]; SX = t4[1]; SXX = t4[2]; Q = t4[3];
+        Ctl* c = a.ctl;
+        const int carry = c->carry;
+        const double n = (double)a.n;
+        const double Md = (double)M;
+        BpfTickStats r;
+        r.tick = c->tick;
+        const bool ok = isfinite(S) && S > 0.0 && isfinite(Md);
+        if (ok) {
+            r.mean = SX / S;
+            r.var = SXX / S - r.mean * r.mean;
+            r.ess = (S * S) / Q;
+            r.ll_ref = c->off + Md + log(fmax(S / n, (double)bits_f32(0x0D6BF94Eu)));
+            r.ll_incr = carry ? (Md + log(S)) - log(c->s_prev) : Md + log(S / n);
+        } else {
+            const double qn = __builtin_nan("");
+            r.mean = r.var = r.ess = r.ll_ref = r.ll_incr = qn;
+        }
+        c->ll_cum += r.ll_incr;
+        r.ll_cum = c->ll_cum;
+        r.h_mean = (float)r.mean;
+        r.log_lik = (float)r.ll_ref;
+        const int resample = ok && (a.threshold <= 0.0f || (r.ess / n) < (double)a.threshold);
+        r.resampled = resample;
+        if (resample) {
+            float u = host_pcg_float(&c->host_rng) / (float)a.n;
+            // W = first i with u + i/N >= 1 (targets are non-decreasing in i)
+            int lo = 0, hi = a.n;
+            while (lo < hi) {
+                int mid = (int)(((unsigned)lo + (unsigned)hi) >> 1);
+                if (target_unwrapped(u, mid, a.n) >= 1.0f) hi = mid; else lo = mid + 1;
+            }
+            c->u = u;
+            c->wrap_start = lo;
+            c->src = c->cur;
+            c->dst = 1 - c->cur;
+            c->cur = c->dst;
+            c->carry = 0;
+            c->off = 0.0;
+            c->resample_count += 1;
+        } else {
+            c->carry = 1;
+            c->off += Md;
+            c->s_prev = S;
+        }
+        c->m_prev = M;
+        c->resample = resample;
+        c->last = r;
+        a.ring[c->tick % a.ring_cap] = r;
+        c->tick += 1;
+        s_flag = resample;
+        s_S = S;
+    }
+    __syncthreads();
+    if (!s_flag) return;
+
+    // FP64 exclusive prefix of the tile weights v_b = f_b * s_b, normalised
+    // by S, clamped to [0, 1] and made non-decreasing by a prefix maximum.
+    S = s_S;
+    const int C = (G + kFinBlock - 1) / kFinBlock;
+    const int b0 = tid * C;
+    const int b1 = min(b0 + C, G);
+    double loc = 0.0;
+    for (int b = b0; b < b1; ++b) {
+        float mb = a.part.m[b];
+        double f = (mb == -__builtin_inff()) ? 0.0 : (double)expf_cw(mb - M);
+        loc += f * a.part.s[b];
+    }
+    double incl = wave_incl_scan(loc, lane);
+    if (lane == kWave - 1) s_scan[wid] = incl;
+    __syncthreads();
+    double wpre = 0.0;
+    for (int w = 0; w < wid; ++w) wpre += s_scan[w];
+    double B = wpre + (incl - loc);
+    B = (tid == 0) ? 0.0 : B;
+    double hi_v = 0.0;
+    for (int b = b0; b < b1; ++b) {
+        float mb = a.part.m[b];
+        double f = (mb == -__builtin_inff()) ? 0.0 : (double)expf_cw(mb - M);
+        double bt = fmin(fmax(B / S, 0.0), 1.0);
+        a.btil[b] = bt;
+        a.ftil[b] = f / S;
+        hi_v = bt;
+        B += f * a.part.s[b];
+    }
+    __syncthreads();
+    double pm = wave_incl_max(hi_v, lane);
+    double wl = __shfl_up(pm, 1, kWave);
+    wl = (lane == 0) ? 0.0 : wl;
+    if (lane == kWave - 1) s_scan[wid] = pm;
+    __syncthreads();
+    double pre = wl;
+    for (int w = 0; w < wid; ++w) pre = fmax(pre, s_scan[w]);
+    for (int b = b0; b < b1; ++b) {
+        double bt = a.btil[b];
+        if (bt < pre) a.btil[b] = pre;
+    }
+    if (tid == 0) {
+        a.btil[0] = 0.0;
+        a.btil[G] = 1.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Resample.  Tile b owns the CDF interval (Bt[b], Bt[b+1]] (tile 0 also owns
+// 0); its in-tile CDF is c_j = min(Bt[b] + ft[b] * L_j, Bt[b+1]) with L_j the
+// FP64 inclusive scan of e_j = exp(l_j - m_b), made non-decreasing, and its
+// last valid element pinned to Bt[b+1].  Output i (target t_i = u + i/N,
+// wrapped once at 1) is written by the tile whose interval holds t_i, with
+// ancestor = lower_bound(c, t_i): the reference's rule
+// (gpu_bpf/gpu_bpf.cu:297-310, clamp to N-1 as ptx:816-830) on this CDF.
+// ---------------------------------------------------------------------------
+
+__device__ inline int count_le(double v, float u, int lo, int hi, int n, bool wrapped) {
+    while (lo < hi) {
+        int mid = (int)(((unsigned)lo + (unsigned)hi) >> 1);
+        float t = target_unwrapped(u, mid, n);
+        if (wrapped) t -= 1.0f;
+        if ((double)t <= v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_resample(ResArgs a) {
+    const Ctl* ctl = a.ctl;
+    if (!ctl->resample) return;
+    constexpr int VEC = K < 4 ? K : 4;
+    constexpr int NV = K / VEC;
+    constexpr int TILE = kBlock * K;
+    constexpr int NW = kBlock / kWave;
+    constexpr int PAD = TILE + TILE / 16 + 1;
+    static_assert(PAD * sizeof(float) <= TILE * sizeof(double), "e staging must fit in s_c");
+    __shared__ float s_x[TILE];
+    __shared__ double s_c[TILE];
+    __shared__ double s_w[NW];
+    __shared__ int s_rng[4];
+    float* s_e = reinterpret_cast<float*>(s_c);
+
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const int b = blockIdx.x;
+    const int n = a.n;
+    const long long base = (long long)b * TILE;
+    const float* __restrict__ xs = ctl->src ? a.x1 : a.x0;
+    float* __restrict__ xd = ctl->dst ? a.x1 : a.x0;
+    const float u = ctl->u;
+    const int W = ctl->wrap_start;
+    const float mb = a.m[b];
+    const bool dead = (mb == -__builtin_inff());
+
+    ObsModel obs = a.obs;
+    {
+        float yv = a.y_ptr ? *a.y_ptr : a.y;
+        obs.ysq = yv * yv;
+    }
+
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const int loc0 = (v * kBlock + tid) * VEC;
+        float xv[VEC], lv[VEC];
+        load_vec<VEC>(xs + base + loc0, xv);
+        if (a.lw) load_vec<VEC>(a.lw + base + loc0, lv);
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) {
+            const int loc = loc0 + c;
+            float lk = a.lw ? lv[c] : log_weight(xv[c], obs);
+            bool valid = base + loc < n;
+            float e = (dead || !valid) ? 0.0f : expf_cw(lk - mb);
+            s_x[loc] = xv[c];
+            s_e[loc + (loc >> 4)] = e;
+        }
+    }
+    __syncthreads();
+
+    // lane-contiguous FP64 scan: lane tid owns tile positions [tid*K, tid*K+K)
+    float ev[K];
+    double tot = 0.0;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        const int j = tid * K + q;
+        ev[q] = s_e[j + (j >> 4)];
+        tot += (double)ev[q];
+    }
+    double incl = wave_incl_scan(tot, lane);
+    if (lane == kWave - 1) s_w[wid] = incl;
+    __syncthreads();
+    double excl = incl - tot;
+    for (int w = 0; w < wid; ++w) excl += s_w[w];
+    excl = (tid == 0) ? 0.0 : excl;
+
+    const double ft = a.ftil[b], Bb = a.btil[b], Bb1 = a.btil[b + 1];
+    const long long lastll = (b == a.n_tiles - 1) ? ((long long)n - 1 - base) : (long long)(TILE - 1);
+    const int last = (int)lastll;
+    double cv[K];
+    double L = excl;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        L += (double)ev[q];
+        cv[q] = fmin(fma(ft, L, Bb), Bb1);
+    }
+    // make the in-tile CDF non-decreasing across lanes (prefix max)
+    double pm = wave_incl_max(cv[K - 1], lane);
+    double prev = __shfl_up(pm, 1, kWave);
+    prev = (lane == 0) ? Bb : prev;
+    __syncthreads();   // s_w reuse and s_e (aliasing s_c) reads are done
+    if (lane == kWave - 1) s_w[wid] = pm;
+    __syncthreads();
+    for (int w = 0; w < wid; ++w) prev = fmax(prev, s_w[w]);
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        const int j = tid * K + q;
+        double c = fmax(cv[q], prev);
+        c = (j == last) ? Bb1 : c;
+        c = (j > last) ? __builtin_inf() : c;
+        s_c[j] = c;
+    }
+    if (tid < 4) {
+        int r;
+        if (tid == 0)      r = (b == 0) ? 0 : count_le(Bb, u, 0, W, n, false);
+        else if (tid == 1) r = (b == a.n_tiles - 1) ? W : count_le(Bb1, u, 0, W, n, false);
+        else if (tid == 2) r = (b == 0) ? W : count_le(Bb, u, W, n, n, true);
+        else               r = (b == a.n_tiles - 1) ? n : count_le(Bb1, u, W, n, n, true);
+        s_rng[tid] = r;
+    }
+    __syncthreads();
+
+    const int ilo = s_rng[0], ihi = s_rng[1], jlo = s_rng[2], jhi = s_rng[3];
+    for (int i = ilo + tid; i < ihi; i += kBlock) {
+        double t = (double)target_unwrapped(u, i, n);
+        int lo = 0, hi = last;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (s_c[mid] < t) lo = mid + 1; else hi = mid;
+        }
+        xd[i] = s_x[lo];
+    }
+    for (int i = jlo + tid; i < jhi; i += kBlock) {
+        double t = (double)(target_unwrapped(u, i, n) - 1.0f);
+        int lo = 0, hi = last;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (s_c[mid] < t) lo = mid + 1; else hi = mid;
+        }
+        xd[i] = s_x[lo];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+
+hipError_t launch_init(const InitArgs& a, hipStream_t st) {
+    int grid = (a.n_pad + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_init, dim3(grid), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+// ev (nullable): four events recorded before propagate, before finalise,
+// before resample and after resample.
+template <int K>
+static hipError_t launch_tick_k(const PropArgs& p, const FinArgs& f, const ResArgs& r,
+                                hipStream_t st, hipEvent_t* ev) {
+    if (ev) (void)hipEventRecord(ev[0], st);
+    hipLaunchKernelGGL(k_propagate<K>, dim3(f.n_tiles), dim3(kBlock), 0, st, p);
+    if (ev) (void)hipEventRecord(ev[1], st);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, st, f);
+    if (ev) (void)hipEventRecord(ev[2], st);
+    hipLaunchKernelGGL(k_resample<K>, dim3(f.n_tiles), dim3(kBlock), 0, st, r);
+    if (ev) (void)hipEventRecord(ev[3], st);
+    return hipGetLastError();
+}
+
+hipError_t launch_tick(int K, const PropArgs& p, const FinArgs& f, const ResArgs& r,
+                       hipStream_t st, hipEvent_t* ev) {
+    switch (K) {
+        case 1: return launch_tick_k<1>(p, f, r, st, ev);
+        case 2: return launch_tick_k<2>(p, f, r, st, ev);
+        case 4: return launch_tick_k<4>(p, f, r, st, ev);
+        case 8: return launch_tick_k<8>(p, f, r, st, ev);
+        case 16: return launch_tick_k<16>(p, f, r, st, ev);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace bpf

@@ -1,0 +1,92 @@
+// bpf_types.hpp -- data shared by the host runtime and the gfx950 kernels.
+#pragma once
+
+#include <stdint.h>
+
+#include "../../include/gpu_bpf.h"
+#include "bpf_device.hpp"
+
+namespace bpf {
+
+// Device-resident control block.  Written only by the single-block finalise
+// kernel of each tick and read by the next kernels on the same stream, so
+// the tick needs no host round trip (the reference syncs twice per tick,
+// gpu_bpf/gpu_bpf_ptx_full.cu:677-685,766).
+struct Ctl {
+    int cur;             // x buffer holding the current particles
+    int carry;           // 1: the propagate pass adds (l_prev - m_prev)
+    int resample;        // decision of the latest tick
+    int src, dst;        // resample reads x[src], writes x[dst]
+    int wrap_start;      // first i whose unwrapped target u + i/N is >= 1
+    int resample_count;
+    int tick;            // ticks finalised so far
+    float m_prev;        // max of the log-weights carried into the next tick
+    float u;             // resampling offset u / N of the latest resample
+    double s_prev;       // sum exp(l - m_prev) of the carried weights
+    double off;          // raw log-weight offset (reference log_lik semantics)
+    double ll_cum;       // FP64 cumulative log-likelihood
+    unsigned long long host_rng;   // host-side PCG32 stream of the reference
+    BpfTickStats last;
+};
+
+// Per-tile partial results of the propagate pass (structure of arrays).
+struct Partials {
+    float* m;      // max log-weight of the tile
+    double* s;     // sum e,        e = exp(l - m)
+    double* sx;    // sum e x
+    double* sxx;   // sum e x^2
+    double* sq;    // sum e^2
+};
+
+struct PropArgs {
+    float* x0;
+    float* x1;
+    float* lw;
+    Partials part;
+    const Ctl* ctl;
+    const float* y_ptr;   // observation in device memory, or null
+    float y;              // observation by value when y_ptr is null
+    uint64_t pcg_a;       // seed * M^(k+2)      (state = pcg_a + inc_i * pcg_g)
+    uint64_t pcg_g;       // sum_{j<k+2} M^j
+    float mu, phi, sig;
+    ObsModel obs;
+    int n;
+    int do_prop;          // 0 on tick 0 (ptx_assembly/bpf_kernels_full.ptx:294)
+    int store_l;          // keep log-weights for a possible carry
+};
+
+struct FinArgs {
+    Partials part;
+    Ctl* ctl;
+    BpfTickStats* ring;
+    double* ftil;         // f_b / S
+    double* btil;         // normalised exclusive tile prefix, G + 1 entries
+    int ring_cap;
+    int n_tiles;
+    int n;
+    float threshold;      // ESS/N threshold; <= 0 resamples every tick
+};
+
+struct ResArgs {
+    float* x0;
+    float* x1;
+    const float* lw;      // stored log-weights, or null: recompute from x
+    const float* m;       // tile maxima
+    const double* ftil;
+    const double* btil;
+    const Ctl* ctl;
+    const float* y_ptr;
+    float y;
+    ObsModel obs;
+    int n;
+    int n_tiles;
+};
+
+struct InitArgs {
+    float* x;
+    uint64_t pcg_a, pcg_g;
+    float mu, sd;
+    int n, n_pad;
+};
+
+}  // namespace bpf

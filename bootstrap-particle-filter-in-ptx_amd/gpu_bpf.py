@@ -1,0 +1,246 @@
+"""Python mirror of the filter's C ABI (include/gpu_bpf.h) over ctypes.
+
+The class methods follow the reference's C API one for one
+(gpu_bpf/gpu_bpf_full.cuh:144-301): create/step/step_async/get_result/destroy,
+set_ess_threshold, get_resample_count, get/set mu and rho, run_rmse, plus the
+build-owned batch and inspection entry points.  The compute path is the HIP
+library _build/libgpubpf.so; if it is missing this module raises instead of
+falling back to anything else.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+REPO_DIR = os.path.dirname(PKG_DIR)
+LIB_PATH = os.path.join(PKG_DIR, "_build", "libgpubpf.so")
+HEADER_PATH = os.path.join(REPO_DIR, "include", "gpu_bpf.h")
+
+
+class BpfResult(ctypes.Structure):
+    _fields_ = [("h_mean", ctypes.c_float), ("log_lik", ctypes.c_float)]
+
+
+STATS_DTYPE = np.dtype(
+    [
+        ("mean", "f8"),
+        ("var", "f8"),
+        ("ess", "f8"),
+        ("ll_incr", "f8"),
+        ("ll_cum", "f8"),
+        ("ll_ref", "f8"),
+        ("h_mean", "f4"),
+        ("log_lik", "f4"),
+        ("resampled", "i4"),
+        ("tick", "i4"),
+    ]
+)
+RESULT_DTYPE = np.dtype([("h_mean", "f4"), ("log_lik", "f4")])
+
+
+class GpuBpfError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def header_symbols(path=HEADER_PATH):
+    """Function names declared in include/gpu_bpf.h."""
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gpu_bpf_[a-z_0-9]+)\s*\(", text)))
+
+
+def load(path=LIB_PATH):
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise GpuBpfError(
+            f"{path} is missing: build it with `make -C {PKG_DIR}/csrc` "
+            "(or __graft_entry__.build()); there is no CPU fallback")
+    L = ctypes.CDLL(path)
+    P, F, I, D = ctypes.c_void_p, ctypes.c_float, ctypes.c_int, ctypes.c_double
+    sig = {
+        "gpu_bpf_create": (P, [I, F, F, F, F, F, I]),
+        "gpu_bpf_destroy": (None, [P]),
+        "gpu_bpf_step": (BpfResult, [P, F]),
+        "gpu_bpf_step_async": (None, [P, F]),
+        "gpu_bpf_get_result": (BpfResult, [P]),
+        "gpu_bpf_run_rmse": (D, [P, P, I, I, F, F, F, F, F, I]),
+        "gpu_bpf_set_ess_threshold": (None, [P, F]),
+        "gpu_bpf_get_resample_count": (I, [P]),
+        "gpu_bpf_get_mu": (F, [P]),
+        "gpu_bpf_get_rho": (F, [P]),
+        "gpu_bpf_set_mu": (None, [P, F]),
+        "gpu_bpf_set_rho": (None, [P, F]),
+        "gpu_bpf_get_sigma_z": (F, [P]),
+        "gpu_bpf_set_sigma_z": (None, [P, F]),
+        "gpu_bpf_enable_mu_learning": (None, [P, I, I, F, F, F]),
+        "gpu_bpf_disable_mu_learning": (None, [P]),
+        "gpu_bpf_enable_rho_learning": (None, [P, I]),
+        "gpu_bpf_set_bands": (None, [I, I, P, P]),
+        "gpu_bpf_last_error": (ctypes.c_char_p, []),
+        "gpu_bpf_run": (I, [P, P, I, P, P]),
+        "gpu_bpf_run_device": (I, [P, P, I]),
+        "gpu_bpf_copy_stats": (I, [P, P, I]),
+        "gpu_bpf_get_stats": (I, [P, P]),
+        "gpu_bpf_copy_particles": (I, [P, P, P]),
+        "gpu_bpf_get_layout": (I, [P, ctypes.POINTER(I), ctypes.POINTER(I)]),
+        "gpu_bpf_set_items_per_thread": (I, [P, I]),
+        "gpu_bpf_synchronize": (I, [P]),
+        "gpu_bpf_get_timestep": (I, [P]),
+        "gpu_bpf_enable_kernel_timing": (I, [P, I]),
+        "gpu_bpf_kernel_times": (I, [P, P, P, I]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def last_error():
+    return load().gpu_bpf_last_error().decode()
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class GpuBpf:
+    """One filter state (gpu_bpf_create ... gpu_bpf_destroy)."""
+
+    def __init__(self, n_particles, rho, sigma_z, mu, nu_state=0.0, nu_obs=0.0, seed=42,
+                 ess_threshold=None, items_per_thread=None):
+        self.L = load()
+        self.n = int(n_particles)
+        self.h = self.L.gpu_bpf_create(self.n, rho, sigma_z, mu, nu_state, nu_obs, int(seed))
+        if not self.h:
+            raise GpuBpfError("gpu_bpf_create failed: " + last_error())
+        if ess_threshold is not None:
+            self.set_ess_threshold(ess_threshold)
+        if items_per_thread is not None:
+            self._check(self.L.gpu_bpf_set_items_per_thread(self.h, int(items_per_thread)))
+
+    def _check(self, rc):
+        if rc < 0:
+            raise GpuBpfError(last_error())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.gpu_bpf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # reference API
+    def step(self, y):
+        r = self.L.gpu_bpf_step(self.h, float(y))
+        return r.h_mean, r.log_lik
+
+    def step_async(self, y):
+        self.L.gpu_bpf_step_async(self.h, float(y))
+
+    def get_result(self):
+        r = self.L.gpu_bpf_get_result(self.h)
+        return r.h_mean, r.log_lik
+
+    def set_ess_threshold(self, thr):
+        self.L.gpu_bpf_set_ess_threshold(self.h, float(thr))
+
+    def resample_count(self):
+        return self._check(self.L.gpu_bpf_get_resample_count(self.h))
+
+    def set_mu(self, mu):
+        self.L.gpu_bpf_set_mu(self.h, float(mu))
+
+    def set_rho(self, rho):
+        self.L.gpu_bpf_set_rho(self.h, float(rho))
+
+    def get_mu(self):
+        return self.L.gpu_bpf_get_mu(self.h)
+
+    def get_rho(self):
+        return self.L.gpu_bpf_get_rho(self.h)
+
+    # build-owned
+    def run(self, y, stats=True):
+        y = np.ascontiguousarray(y, np.float32)
+        res = np.zeros(len(y), RESULT_DTYPE)
+        st = np.zeros(len(y), STATS_DTYPE) if stats else None
+        self._check(self.L.gpu_bpf_run(self.h, _ptr(y), len(y), _ptr(res), _ptr(st)))
+        return (res, st) if stats else res
+
+    def stats(self):
+        st = np.zeros(1, STATS_DTYPE)
+        self._check(self.L.gpu_bpf_get_stats(self.h, _ptr(st)))
+        return st[0]
+
+    def particles(self, with_logw=False):
+        x = np.empty(self.n, np.float32)
+        lw = np.empty(self.n, np.float32) if with_logw else None
+        self._check(self.L.gpu_bpf_copy_particles(self.h, _ptr(x), _ptr(lw)))
+        return (x, lw) if with_logw else x
+
+    def layout(self):
+        t, g = ctypes.c_int(), ctypes.c_int()
+        self._check(self.L.gpu_bpf_get_layout(self.h, ctypes.byref(t), ctypes.byref(g)))
+        return t.value, g.value
+
+    def synchronize(self):
+        self._check(self.L.gpu_bpf_synchronize(self.h))
+
+    def timestep(self):
+        return self.L.gpu_bpf_get_timestep(self.h)
+
+    def set_mu_schedule_run(self, y, mu_t, stats=False):
+        """Run y with mu changed (set_mu) wherever mu_t changes value."""
+        y = np.ascontiguousarray(y, np.float32)
+        mu_t = np.asarray(mu_t, np.float32)
+        cuts = [0] + [t for t in range(1, len(y)) if mu_t[t] != mu_t[t - 1]] + [len(y)]
+        out = []
+        for a, b in zip(cuts[:-1], cuts[1:]):
+            self.set_mu(float(mu_t[a]))
+            if stats:
+                out.append(self.run(y[a:b])[1])
+            else:
+                yd = y[a:b]
+                self._check(self.L.gpu_bpf_run(self.h, _ptr(yd), len(yd), None, None))
+        return np.concatenate(out) if stats else None
+
+    def enqueue(self, y):
+        """Enqueue len(y) ticks without waiting (observations staged once)."""
+        y = np.ascontiguousarray(y, np.float32)
+        self._y_keep = y
+        self._check(self.L.gpu_bpf_run(self.h, _ptr(y), len(y), None, None))
+
+    def enable_kernel_timing(self, on=True):
+        self._check(self.L.gpu_bpf_enable_kernel_timing(self.h, 1 if on else 0))
+
+    def kernel_times(self, reset=True):
+        """(ms[3], launches[3]) for propagate, finalise, resample."""
+        ms = np.zeros(3, np.float64)
+        cnt = np.zeros(3, np.int32)
+        self._check(self.L.gpu_bpf_kernel_times(self.h, _ptr(ms), _ptr(cnt), 1 if reset else 0))
+        return ms, cnt
+
+
+def run_rmse(returns, true_h, n_particles, rho, sigma_z, mu, nu_state=0.0, nu_obs=0.0, seed=42):
+    r = np.ascontiguousarray(returns, np.float64)
+    h = np.ascontiguousarray(true_h, np.float64)
+    return load().gpu_bpf_run_rmse(_ptr(r), _ptr(h), len(r), int(n_particles), rho, sigma_z,
+                                   mu, nu_state, nu_obs, int(seed))

@@ -1,0 +1,172 @@
+/*
+ * gpu_bpf.h -- C ABI of the MI355X (gfx950) bootstrap particle filter for the
+ * stochastic-volatility model
+ *
+ *     x_t = mu + phi (x_{t-1} - mu) + sigma_eta eta_t,   y_t = exp(x_t / 2) eps_t
+ *
+ * (reference names: rho = phi, sigma_z = sigma_eta, h = x).
+ *
+ * Drop-in for the reference's plain-filter API (gpu_bpf/gpu_bpf_full.cuh and
+ * gpu_bpf/gpu_bpf.cuh): the same extern "C" symbols, argument meanings and
+ * BpfResult layout.  The state is opaque; callers that poked GpuBpfState
+ * fields use the setters below instead.  Plain C: no HIP or CUDA types.
+ *
+ * Threading: one HIP stream per filter state; distinct states may be used from
+ * distinct threads; a single state is not reentrant.
+ * Errors: functions that can fail return NULL / a negative code / NaN and set
+ * a thread-local message readable with gpu_bpf_last_error().  Nothing calls
+ * exit().  A NaN observation is propagated (NaN statistics, no resampling),
+ * not trapped.
+ */
+#ifndef GPU_BPF_H
+#define GPU_BPF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Per-tick output, returned by value (8-byte POD).
+ * Replaces BpfResult of gpu_bpf/gpu_bpf_full.cuh:33-36 (identical layout). */
+typedef struct {
+    float h_mean;   /* posterior mean E[x_t | y_{0:t}], pre-resample */
+    float log_lik;  /* reference semantics: max_lw + log(sum_w / N) of the
+                       log-weights accumulated since the last resample
+                       (ptx_assembly/bpf_kernels_full.ptx:755-775) */
+} BpfResult;
+
+/* Full per-tick statistics (build-owned extension, 64 bytes). */
+typedef struct {
+    double mean;     /* sum w x / sum w */
+    double var;      /* sum w x^2 / sum w - mean^2 (filtered variance) */
+    double ess;      /* (sum w)^2 / sum w^2 */
+    double ll_incr;  /* incremental log p(y_t | y_{0:t-1}) estimate */
+    double ll_cum;   /* FP64 running sum of ll_incr since create */
+    double ll_ref;   /* BpfResult.log_lik before the cast to float */
+    float  h_mean;   /* (float)mean */
+    float  log_lik;  /* (float)ll_ref */
+    int    resampled;
+    int    tick;
+} BpfTickStats;
+
+/* Opaque filter state.  Replaces the public struct GpuBpfState of
+ * gpu_bpf/gpu_bpf_full.cuh:80-126 (gpu_bpf/gpu_bpf.cuh:30-54). */
+typedef struct GpuBpfState GpuBpfState;
+
+/* ---- reference API (gpu_bpf/gpu_bpf_full.cuh) ---------------------------- */
+
+/* gpu_bpf_full.cuh:144 / gpu_bpf_ptx_full.cu:340.  Seeds per-particle PCG32
+ * streams and draws x_0 from N(mu, sigma_z^2 / max(1 - rho^2, 1e-6)).
+ * nu_obs > 0 selects Student-t observation noise.  nu_state > 0 (Student-t
+ * state noise) is not supported: returns NULL.  Default ESS threshold 0.5. */
+GpuBpfState* gpu_bpf_create(int n_particles, float rho, float sigma_z, float mu,
+                            float nu_state, float nu_obs, int seed);
+
+/* gpu_bpf_full.cuh:148.  NULL-safe. */
+void gpu_bpf_destroy(GpuBpfState* state);
+
+/* gpu_bpf_full.cuh:160 / gpu_bpf_ptx_full.cu:764.  One tick, synchronous. */
+BpfResult gpu_bpf_step(GpuBpfState* state, float y_t);
+
+/* gpu_bpf_full.cuh:168 / gpu_bpf_ptx_full.cu:550.  One tick, no host sync
+ * (the resampling decision is taken on the device). */
+void gpu_bpf_step_async(GpuBpfState* state, float y_t);
+
+/* gpu_bpf_full.cuh:174 / gpu_bpf_ptx_full.cu:754.  Waits for the state's
+ * stream, then returns the latest tick's result. */
+BpfResult gpu_bpf_get_result(GpuBpfState* state);
+
+/* gpu_bpf_full.cuh:297 / gpu_bpf_ptx_full.cu:834.  Creates a filter, steps
+ * all ticks, returns the RMSE of h_mean against true_h over ticks >= 100. */
+double gpu_bpf_run_rmse(const double* returns, const double* true_h, int n_ticks,
+                        int n_particles, float rho, float sigma_z, float mu,
+                        float nu_state, float nu_obs, int seed);
+
+/* gpu_bpf_full.cuh:282 / gpu_bpf_ptx_full.cu:858.  Resample when ESS/N is
+ * below the threshold; <= 0 resamples every tick. */
+void gpu_bpf_set_ess_threshold(GpuBpfState* state, float threshold);
+
+/* gpu_bpf_full.cuh:285 / gpu_bpf_ptx_full.cu:862. */
+int gpu_bpf_get_resample_count(GpuBpfState* state);
+
+/* gpu_bpf_full.cuh:207-230 / gpu_bpf_ptx_full.cu:806-831.  set_rho clamps to
+ * [0.001, 0.999] as the reference does.  Changes apply from the next tick. */
+float gpu_bpf_get_mu(GpuBpfState* state);
+float gpu_bpf_get_rho(GpuBpfState* state);
+void  gpu_bpf_set_mu(GpuBpfState* state, float mu);
+void  gpu_bpf_set_rho(GpuBpfState* state, float rho);
+
+/* gpu_bpf_full.cuh:197-204 (online mu/rho learning).  Not provided by this
+ * library: enable_* record an error and leave learning off; disable is a
+ * no-op.  Kept so that callers link. */
+void gpu_bpf_enable_mu_learning(GpuBpfState* state, int mode, int K,
+                                float c, float t0, float gamma);
+void gpu_bpf_disable_mu_learning(GpuBpfState* state);
+void gpu_bpf_enable_rho_learning(GpuBpfState* state, int enable);
+
+/* gpu_bpf_full.cuh:245-272 (sigma_z scale bands).  Only the trivial single
+ * band (n_bands <= 1, scale 1) is provided; anything else records an error
+ * and is ignored.  Kept so that callers link. */
+void gpu_bpf_set_bands(int n_particles, int n_bands, const float* fracs,
+                       const float* scales);
+void gpu_bpf_set_adaptive_bands(int n_particles,
+                                const float* calm_fracs, const float* calm_scales, int calm_nb,
+                                const float* alert_fracs, const float* alert_scales, int alert_nb,
+                                const float* panic_fracs, const float* panic_scales, int panic_nb,
+                                float thresh_alert, float thresh_panic);
+
+/* ---- build-owned extensions ----------------------------------------------- */
+
+/* Thread-local text of the last error ("" if none). */
+const char* gpu_bpf_last_error(void);
+
+/* Replaces reading GpuBpfState.sigma_z (gpu_bpf_full.cuh:107). */
+float gpu_bpf_get_sigma_z(GpuBpfState* state);
+void  gpu_bpf_set_sigma_z(GpuBpfState* state, float sigma_z);
+
+/* Run n_ticks ticks on y[0..n_ticks) with no host synchronisation inside the
+ * loop; per-tick results are copied out at the end (either pointer may be
+ * NULL).  Returns 0 or a negative error code. */
+int gpu_bpf_run(GpuBpfState* state, const float* y, int n_ticks,
+                BpfResult* results, BpfTickStats* stats);
+
+/* Same, with the observations already in device memory (y_dev is a device
+ * pointer to n_ticks floats).  Enqueues only: call gpu_bpf_synchronize
+ * before reading anything; stats of the last 'ring' ticks stay on the device
+ * and can be fetched with gpu_bpf_copy_stats. */
+int gpu_bpf_run_device(GpuBpfState* state, const float* y_dev, int n_ticks);
+
+/* Copy the statistics of the last n ticks (n <= ring capacity) to the host. */
+int gpu_bpf_copy_stats(GpuBpfState* state, BpfTickStats* stats, int n);
+
+/* Latest tick's full statistics (synchronises). */
+int gpu_bpf_get_stats(GpuBpfState* state, BpfTickStats* stats);
+
+/* Copy the current particle states (N floats) and, when the filter carries
+ * weights across ticks, the current log-weights (NULL to skip). */
+int gpu_bpf_copy_particles(GpuBpfState* state, float* x, float* log_w);
+
+/* Weight-tile geometry: particles per tile and number of tiles. */
+int gpu_bpf_get_layout(GpuBpfState* state, int* tile, int* n_tiles);
+
+/* Particles per lane of the per-tile kernels (1, 2, 4, 8 or 16); only before
+ * the first step.  0 restores the automatic choice. */
+int gpu_bpf_set_items_per_thread(GpuBpfState* state, int items);
+
+/* Wait for all work enqueued on the state's stream. */
+int gpu_bpf_synchronize(GpuBpfState* state);
+
+/* Number of ticks processed so far. */
+int gpu_bpf_get_timestep(GpuBpfState* state);
+
+/* Per-kernel timing with HIP events recorded on the state's own stream
+ * around each of the tick's three kernels (propagate, finalise, resample).
+ * Enable before enqueuing ticks; gpu_bpf_kernel_times synchronises, then
+ * returns the summed milliseconds and launch counts accumulated since the
+ * last reset (arrays of 3: propagate, finalise, resample). */
+int gpu_bpf_enable_kernel_timing(GpuBpfState* state, int enable);
+int gpu_bpf_kernel_times(GpuBpfState* state, double* ms, int* launches, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPU_BPF_H */

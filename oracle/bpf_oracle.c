@@ -1,0 +1,643 @@
+/*
+ * bpf_oracle.c -- CPU oracle for the SV bootstrap particle filter.
+ *
+ * TEST INFRASTRUCTURE ONLY (see bpf_oracle.h).  The shipped GPU library never
+ * links or calls this file.  Each function cites the reference lines it
+ * restates.  Compile with -ffp-contract=off: every fused multiply-add below is
+ * written as fmaf()/fma() on purpose, so results are bit-reproducible.
+ */
+#include "bpf_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static inline float f32_bits(uint32_t b) { float f; memcpy(&f, &b, 4); return f; }
+
+/* Planted-mistake switch for the tests that show each pinning check fails
+ * on a plausible error (0 = none).  See orc_set_fault in bpf_oracle.h. */
+static int g_fault = 0;
+void orc_set_fault(int code) { g_fault = code; }
+static inline uint32_t u32_of(float f) { uint32_t b; memcpy(&b, &f, 4); return b; }
+
+/* ========================================================================= */
+/* PCG32                                                                      */
+/* ========================================================================= */
+
+#define PCG_MULT 6364136223846793005ULL
+#define PCG_HOST_INC 1442695040888963407ULL
+
+/* XSH-RR output of the old state: ptx_assembly/bpf_kernels_full.ptx:146-157. */
+uint32_t orc_pcg_output(uint64_t old) {
+    uint32_t xsh = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xsh >> rot) | (xsh << ((-rot) & 31u));
+}
+
+/* Literal restatement of the reference seeding (ptx:74-85: inc = (2i+1)|1,
+ * state = (seed*M + inc)*M + inc, seed widened from int) followed by k steps
+ * s <- s*M + inc (ptx:140-144). */
+uint64_t orc_pcg_state_naive(int seed, uint32_t i, uint32_t k) {
+    uint64_t inc = (uint64_t)(((uint32_t)(2u * i + 1u)) | 1u);
+    uint64_t s = (uint64_t)(int64_t)seed * PCG_MULT + inc;
+    s = s * PCG_MULT + inc;
+    for (uint32_t j = 0; j < k; j++) s = s * PCG_MULT + inc;
+    return s;
+}
+
+/* Closed form: s_k(i) = seed*M^(k+2) + inc_i * G_(k+2), G_n = sum_{j<n} M^j.
+ * (M^n, G_n) come from binary powering of the affine map s -> M s + 1. */
+typedef struct { uint64_t a, g; } PcgJump;   /* s_k(i) = a + inc_i * g */
+
+static PcgJump pcg_jump(int seed, uint32_t k) {
+    uint64_t n = (uint64_t)k + 2u;
+    uint64_t acc_m = 1, acc_g = 0;        /* map for the steps taken so far */
+    uint64_t cur_m = PCG_MULT, cur_g = 1; /* map for 2^bit steps */
+    while (n) {
+        if (n & 1u) { acc_g = acc_g * cur_m + cur_g; acc_m = acc_m * cur_m; }
+        cur_g = cur_g * cur_m + cur_g;
+        cur_m = cur_m * cur_m;
+        n >>= 1;
+    }
+    PcgJump j = { (uint64_t)(int64_t)seed * acc_m, acc_g };
+    return j;
+}
+
+static inline uint64_t pcg_inc(uint32_t i) { return (uint64_t)(((uint32_t)(2u * i + 1u)) | 1u); }
+
+uint64_t orc_pcg_state(int seed, uint32_t i, uint32_t k) {
+    PcgJump j = pcg_jump(seed, k);
+    return j.a + pcg_inc(i) * j.g;
+}
+
+/* Host stream for the resampling offset: gpu_bpf/gpu_bpf_ptx_full.cu:241-251
+ * (default increment), seeded seed*67890+12345 with no warm-up (:353). */
+uint64_t orc_pcg_host_seed(int seed) {
+    return (uint64_t)(int64_t)seed * 67890ULL + 12345ULL;
+}
+
+uint32_t orc_pcg_host_next(uint64_t* state) {
+    uint64_t old = *state;
+    *state = old * PCG_MULT + PCG_HOST_INC;
+    return orc_pcg_output(old);
+}
+
+/* u = (r>>9)*2^-23 clamped to [1e-6, 0x3F7FFBCE]: ptx:159-164. */
+float orc_uniform(uint32_t r) {
+    float u = (float)(r >> 9) * (1.0f / 8388608.0f);
+    u = fmaxf(u, f32_bits(0x358637BDu));
+    u = fminf(u, f32_bits(0x3F7FFBCEu));
+    return u;
+}
+
+/* ========================================================================= */
+/* Scalar math                                                                */
+/* ========================================================================= */
+
+/* exp(x) after S. L. Moshier's Cephes expf (Cody-Waite reduction with
+ * C1 = 0.693359375, C2 = -2.12194440e-4, degree-5 polynomial).  Inputs below
+ * -86 return 0 (no subnormal results), above 88 return +inf. */
+float orc_expf(float x) {
+    if (!(x >= -86.0f)) return (x != x) ? x : 0.0f;
+    if (x > 88.0f) return INFINITY;
+    float z = rintf(x * 1.44269504088896341f);
+    float r = fmaf(z, -0.693359375f, x);
+    r = fmaf(z, 2.12194440e-4f, r);
+    float p = 1.9875691500e-4f;
+    p = fmaf(p, r, 1.3981999507e-3f);
+    p = fmaf(p, r, 8.3334519073e-3f);
+    p = fmaf(p, r, 4.1665795894e-2f);
+    p = fmaf(p, r, 1.6666665459e-1f);
+    p = fmaf(p, r, 5.0000001201e-1f);
+    float rr = r * r;
+    p = fmaf(p, rr, r);
+    p = p + 1.0f;
+    int n = (int)z;
+    return f32_bits(u32_of(p) + ((uint32_t)n << 23));
+}
+
+/* log(x) after Moshier's Cephes logf (frexp to [sqrt(1/2), sqrt(2)), degree-8
+ * polynomial, exponent added back in two parts). */
+float orc_logf(float x) {
+    if (x != x) return x;
+    if (x < 0.0f) return NAN;
+    if (x == 0.0f) return -INFINITY;
+    if (x == INFINITY) return x;
+    uint32_t b = u32_of(x);
+    int e;
+    if ((b >> 23) == 0) {                 /* subnormal: scale by 2^25 */
+        b = u32_of(x * 33554432.0f);
+        e = (int)(b >> 23) - 126 - 25;
+    } else {
+        e = (int)(b >> 23) - 126;
+    }
+    float m = f32_bits((b & 0x007FFFFFu) | 0x3F000000u);   /* [0.5, 1) */
+    if (m < 0.707106781186547524f) { e -= 1; m = m + m - 1.0f; }
+    else { m = m - 1.0f; }
+    float z = m * m;
+    float y = 7.0376836292e-2f;
+    y = fmaf(y, m, -1.1514610310e-1f);
+    y = fmaf(y, m, 1.1676998740e-1f);
+    y = fmaf(y, m, -1.2420140846e-1f);
+    y = fmaf(y, m, 1.4249322787e-1f);
+    y = fmaf(y, m, -1.6668057665e-1f);
+    y = fmaf(y, m, 2.0000714765e-1f);
+    y = fmaf(y, m, -2.4999993993e-1f);
+    y = fmaf(y, m, 3.3333331174e-1f);
+    y = (y * m) * z;
+    float fe = (float)e;
+    y = fmaf(fe, -2.12194440e-4f, y);
+    y = fmaf(-0.5f, z, y);
+    float r = m + y;
+    r = fmaf(fe, 0.693359375f, r);
+    return r;
+}
+
+/* Acklam (2003) inverse normal CDF, FP32 coefficients of ptx:172-203,
+ * central region |u-1/2| < 0.47575 (0x3EF39581, ptx:169), tails
+ * q = sqrt(-2 log p) negated for the upper tail (ptx:185-205).  Division and
+ * square root are IEEE; log is orc_logf. */
+float orc_icdf_f32(float u) {
+    float q = u - 0.5f;
+    if (fabsf(q) < f32_bits(0x3EF39581u)) {
+        float r = q * q;
+        float num = fmaf(r, -39.69683028665376f, 220.9460984245205f);
+        num = fmaf(r, num, -275.9285104469687f);
+        num = fmaf(r, num, 138.3577518672690f);
+        num = fmaf(r, num, -30.66479806614716f);
+        num = fmaf(r, num, 2.506628277459239f);
+        num = q * num;
+        float den = fmaf(r, -54.47609879822406f, 161.5858368580409f);
+        den = fmaf(r, den, -155.6989798598866f);
+        den = fmaf(r, den, 66.80131188771972f);
+        den = fmaf(r, den, -13.28068155288572f);
+        den = fmaf(r, den, 1.0f);
+        return num / den;
+    }
+    int low = u < 0.5f;
+    float p = low ? u : 1.0f - u;
+    float t = sqrtf(-2.0f * orc_logf(p));
+    float num = fmaf(t, -7.784894002430293e-03f, -3.223964580411365e-01f);
+    num = fmaf(t, num, -2.400758277161838e+00f);
+    num = fmaf(t, num, -2.549732539343734e+00f);
+    num = fmaf(t, num, 4.374664141464968e+00f);
+    num = fmaf(t, num, 2.938163982698783e+00f);
+    float den = fmaf(t, 7.784695709041462e-03f, 3.224671290700398e-01f);
+    den = fmaf(t, den, 2.445134137142996e+00f);
+    den = fmaf(t, den, 3.754408661907416e+00f);
+    den = fmaf(t, den, 1.0f);
+    float z = num / den;
+    return low ? z : -z;
+}
+
+/* Same algorithm in FP64 with Acklam's full-precision coefficients. */
+double orc_icdf_f64(double u) {
+    double q = u - 0.5;
+    if (fabs(q) < 0.47575) {
+        double r = q * q;
+        double num = (((((-3.969683028665376e+01 * r + 2.209460984245205e+02) * r
+                        - 2.759285104469687e+02) * r + 1.383577518672690e+02) * r
+                        - 3.066479806614716e+01) * r + 2.506628277459239e+00) * q;
+        double den = ((((-5.447609879822406e+01 * r + 1.615858368580409e+02) * r
+                        - 1.556989798598866e+02) * r + 6.680131188771972e+01) * r
+                        - 1.328068155288572e+01) * r + 1.0;
+        return num / den;
+    }
+    int low = u < 0.5;
+    double p = low ? u : 1.0 - u;
+    double t = sqrt(-2.0 * log(p));
+    double num = ((((-7.784894002430293e-03 * t - 3.223964580411365e-01) * t
+                    - 2.400758277161838e+00) * t - 2.549732539343734e+00) * t
+                    + 4.374664141464968e+00) * t + 2.938163982698783e+00;
+    double den = (((7.784695709041462e-03 * t + 3.224671290700398e-01) * t
+                   + 2.445134137142996e+00) * t + 3.754408661907416e+00) * t + 1.0;
+    double z = num / den;
+    return low ? z : -z;
+}
+
+float orc_normal(int seed, uint32_t i, uint32_t k) {
+    return orc_icdf_f32(orc_uniform(orc_pcg_output(orc_pcg_state(seed, i, k))));
+}
+
+static inline float normal_j(PcgJump j, uint32_t i) {
+    return orc_icdf_f32(orc_uniform(orc_pcg_output(j.a + pcg_inc(i) * j.g)));
+}
+
+/* ========================================================================= */
+/* DGP: tests/test_bpf_matched_dgp.cu:59-99 (PCG32 + FP64 Box-Muller) and     */
+/* :141-175 (SV series, h clamped to [-12, 4]).                                */
+/* ========================================================================= */
+
+typedef struct { uint64_t state, inc; } DgpRng;
+
+static uint32_t dgp_next(DgpRng* g) {
+    uint64_t old = g->state;
+    g->state = old * PCG_MULT + g->inc;
+    return orc_pcg_output(old);
+}
+
+static void dgp_seed(DgpRng* g, uint64_t seed) {
+    g->state = 0;
+    g->inc = (seed * 12345ULL) | 1u;
+    dgp_next(g);
+    g->state += seed * 67890ULL;
+    dgp_next(g);
+    dgp_next(g);
+}
+
+static double dgp_double(DgpRng* g) { return (dgp_next(g) >> 11) * (1.0 / 2097152.0); }
+
+static double dgp_gauss(DgpRng* g) {
+    double u1, u2;
+    do { u1 = dgp_double(g); } while (u1 < 1e-10);
+    u2 = dgp_double(g);
+    return sqrt(-2.0 * log(u1)) * cos(2.0 * 3.14159265358979323846 * u2);
+}
+
+static double dgp_student_t(DgpRng* g, double nu) {
+    if (nu > 100.0) return dgp_gauss(g);
+    double z = dgp_gauss(g);
+    double chi = 0.0;
+    int inu = (int)nu;
+    for (int i = 0; i < inu; i++) { double v = dgp_gauss(g); chi += v * v; }
+    return z / sqrt(chi / nu);
+}
+
+static void dgp_series(int n, double rho, double sigma_z, const double* mu_t, double mu0,
+                       double nu_state, double nu_obs, uint64_t seed,
+                       double* true_h, double* returns) {
+    DgpRng g;
+    dgp_seed(&g, seed);
+    double m0 = mu_t ? mu_t[0] : mu0;
+    double var_stat = (sigma_z * sigma_z) / fmax(1.0 - rho * rho, 1e-6);
+    double h = m0 + sqrt(var_stat) * dgp_gauss(&g);
+    for (int t = 0; t < n; t++) {
+        double mu = mu_t ? mu_t[t] : mu0;
+        if (t > 0) {
+            double eps = (nu_state > 0.0) ? dgp_student_t(&g, nu_state) : dgp_gauss(&g);
+            h = mu + rho * (h - mu) + sigma_z * eps;
+        }
+        if (h < -12.0) h = -12.0;
+        if (h > 4.0) h = 4.0;
+        if (true_h) true_h[t] = h;
+        double vol = exp(h / 2.0);
+        double eta = (nu_obs > 0.0) ? dgp_student_t(&g, nu_obs) : dgp_gauss(&g);
+        if (returns) returns[t] = vol * eta;
+    }
+}
+
+void orc_dgp(int n_ticks, double rho, double sigma_z, double mu, double nu_state,
+             double nu_obs, uint64_t seed, double* true_h, double* returns) {
+    dgp_series(n_ticks, rho, sigma_z, NULL, mu, nu_state, nu_obs, seed, true_h, returns);
+}
+
+void orc_dgp_regimes(int n_ticks, double rho, double sigma_z, const double* mu_t,
+                     double nu_obs, uint64_t seed, double* true_h, double* returns) {
+    dgp_series(n_ticks, rho, sigma_z, mu_t, 0.0, 0.0, nu_obs, seed, true_h, returns);
+}
+
+/* ========================================================================= */
+/* Resampling: target_i = u + (float)i/(float)n, wrapped once at 1           */
+/* (gpu_bpf/gpu_bpf.cu:297-310, IEEE division); ancestor = lower_bound of the */
+/* target in cdf[0..n-1] clamped to n-1 (ptx:816-830).                        */
+/* ========================================================================= */
+
+static inline float orc_target(float u, int i, int n) {
+    float t = u + (float)i / (float)n;
+    if (t >= 1.0f) t -= 1.0f;
+    return t;
+}
+
+void orc_resample_ancestors(const double* cdf, int n, float u, int* anc) {
+    for (int i = 0; i < n; i++) {
+        double t = (double)orc_target(u, i, n);
+        int lo = 0, hi = n - 1;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            int left = (g_fault == 3) ? (cdf[mid] <= t) : (cdf[mid] < t);
+            if (left) lo = mid + 1; else hi = mid;
+        }
+        anc[i] = lo;
+    }
+}
+
+/* ========================================================================= */
+/* The filter                                                                 */
+/* ========================================================================= */
+
+/* Log-weight of ptx:414-450: Gaussian -0.9189385 - y^2 e^{-x}/2 - x/2, or
+ * Student-t C_obs - (nu+1)/2 log(1 + y^2 e^{-x}/nu) - x/2. */
+static inline float lw_f32(float x, float ysq, float nu_obs, float c_obs, float nph) {
+    float eta2 = ysq * orc_expf(-x);
+    if (nu_obs > 0.0f) {
+        float ta = eta2 / nu_obs + 1.0f;
+        float lw = fmaf(-nph, orc_logf(ta), c_obs);
+        return fmaf(-0.5f, x, lw);
+    }
+    float lw = fmaf(-0.5f, eta2, f32_bits(0xBF6B3F8Eu));
+    return g_fault == 1 ? lw : fmaf(-0.5f, x, lw);
+}
+
+static inline double lw_f64(double x, double y, double nu_obs, double c_obs) {
+    double eta2 = y * y * exp(-x);
+    if (nu_obs > 0.0)
+        return c_obs - 0.5 * (nu_obs + 1.0) * log1p(eta2 / nu_obs) - 0.5 * x;
+    return -0.91893853320467274178 - 0.5 * eta2 - (g_fault == 1 ? 0.0 : 0.5 * x);
+}
+
+/* Student-t constant C_obs in FP64 (gpu_bpf/gpu_bpf_ptx_full.cu:407-412). */
+static double c_obs_f64(double nu) {
+    return lgamma((nu + 1.0) / 2.0) - lgamma(nu / 2.0) - 0.5 * log(nu * 3.14159265358979323846);
+}
+
+int orc_bpf_run(const OrcParams* p, int mode, const float* y, const float* mu_t,
+                int n_ticks, OrcTick* out, float* x_out, double* x64_out,
+                int* anc_out, int n_threads) {
+    const int n = p->n_particles;
+    if (n <= 0 || n_ticks < 0 || p->tile <= 0) return -1;
+    const int tile = p->tile;
+    const int n_tiles = (n + tile - 1) / tile;
+#ifdef _OPENMP
+    if (n_threads < 1) n_threads = 1;
+#else
+    (void)n_threads;
+#endif
+    const int do32 = (mode != ORC_MODE_F64);     /* FP32 restatement runs */
+    const int do64 = (mode != ORC_MODE_F32);     /* FP64 oracle runs */
+    const int f64 = do64;                        /* statistics from FP64 */
+
+    float* x = (float*)malloc(sizeof(float) * (size_t)n);
+    float* l = (float*)malloc(sizeof(float) * (size_t)n);
+    double* xd = (double*)malloc(sizeof(double) * (size_t)n);
+    double* ld = (double*)malloc(sizeof(double) * (size_t)n);
+    double* w = (double*)malloc(sizeof(double) * (size_t)n);
+    double* wd = (double*)malloc(sizeof(double) * (size_t)n);
+    double* cdf = (double*)malloc(sizeof(double) * (size_t)n);
+    int* anc = (int*)malloc(sizeof(int) * (size_t)n);
+    float* xt = (float*)malloc(sizeof(float) * (size_t)n);
+    double* xdt = (double*)malloc(sizeof(double) * (size_t)n);
+    float* tmax = (float*)malloc(sizeof(float) * (size_t)n_tiles);
+    float* tf = (float*)malloc(sizeof(float) * (size_t)n_tiles);
+    if (!x || !l || !xd || !ld || !w || !wd || !cdf || !anc || !xt || !xdt || !tmax || !tf) {
+        free(x); free(l); free(xd); free(ld); free(w); free(wd); free(cdf); free(anc);
+        free(xt); free(xdt); free(tmax); free(tf);
+        return -1;
+    }
+
+    const float rho = p->rho, sig = p->sigma_z;
+    const float nu_obs = p->nu_obs;
+    const double c_obs_d = nu_obs > 0.0f ? c_obs_f64((double)nu_obs) : 0.0;
+    const float c_obs = (float)c_obs_d;
+    const float nph = (nu_obs + 1.0f) * 0.5f;
+
+    /* Stationary start: std computed in float as gpu_bpf_ptx_full.cu:430. */
+    {
+        float mu0 = mu_t ? mu_t[0] : p->mu;
+        float sd = sqrtf((sig * sig) / fmaxf(1.0f - rho * rho, 1e-6f));
+        double sdd = sqrt(((double)sig * sig) / fmax(1.0 - (double)rho * rho, 1e-6));
+        PcgJump j0 = pcg_jump(p->seed, 0);
+#pragma omp parallel for num_threads(n_threads) schedule(static)
+        for (int i = 0; i < n; i++) {
+            float z = normal_j(j0, (uint32_t)i);
+            x[i] = fmaf(sd, z, mu0);
+            xd[i] = (double)mu0 + sdd * (double)z;
+        }
+    }
+
+    uint64_t hrng = orc_pcg_host_seed(p->seed);
+    int carry = 0, count = 0;
+    float m_prev = 0.0f;
+    double md_prev = 0.0, s_prev = 1.0, off = 0.0, ll_cum = 0.0;
+    const double ll_floor = (double)f32_bits(0x0D6BF94Eu);
+
+    for (int t = 0; t < n_ticks; t++) {
+        const float mu = mu_t ? mu_t[t] : p->mu;
+        const float yt = y[t];
+        const float ysq = yt * yt;
+        const PcgJump jt = pcg_jump(p->seed, (uint32_t)t);
+
+        double M = 0.0;
+        if (do32) {
+#pragma omp parallel for num_threads(n_threads) schedule(static)
+            for (int i = 0; i < n; i++) {
+                float xi = x[i];
+                if (t > 0) {
+                    float z = normal_j(jt, (uint32_t)i);
+                    float d = xi - mu;
+                    d = (g_fault == 2 ? -rho : rho) * d;
+                    xi = mu + d;
+                    xi = fmaf(sig, z, xi);
+                    x[i] = xi;
+                }
+                float lw = lw_f32(xi, ysq, nu_obs, c_obs, nph);
+                l[i] = (carry && g_fault != 4) ? lw + (l[i] - m_prev) : lw;
+            }
+            /* tile maxima (fmaxf: NaN ignored), global max */
+            float gm = -INFINITY;
+            for (int b = 0; b < n_tiles; b++) {
+                float m = -INFINITY;
+                int e = (b + 1) * tile < n ? (b + 1) * tile : n;
+                for (int i = b * tile; i < e; i++) m = fmaxf(m, l[i]);
+                tmax[b] = m;
+                gm = fmaxf(gm, m);
+            }
+            for (int b = 0; b < n_tiles; b++)
+                tf[b] = (tmax[b] == -INFINITY) ? 0.0f : orc_expf(tmax[b] - gm);
+#pragma omp parallel for num_threads(n_threads) schedule(static)
+            for (int i = 0; i < n; i++) {
+                int b = i / tile;
+                float e = (tmax[b] == -INFINITY) ? 0.0f : orc_expf(l[i] - tmax[b]);
+                w[i] = (double)e * (double)tf[b];
+            }
+            M = (double)gm;
+            m_prev = gm;
+        }
+        if (do64) {
+            const double mud = mu, rhod = rho, sigd = sig;
+            double gm = -INFINITY;
+            for (int i = 0; i < n; i++) {
+                if (t > 0) {
+                    double z = (double)normal_j(jt, (uint32_t)i);
+                    xd[i] = mud + (g_fault == 2 ? -rhod : rhod) * (xd[i] - mud) + sigd * z;
+                }
+                double lw = lw_f64(xd[i], (double)yt, (double)nu_obs, c_obs_d);
+                ld[i] = (carry && g_fault != 4) ? lw + (ld[i] - md_prev) : lw;
+                if (ld[i] > gm) gm = ld[i];
+            }
+            double* wt = do32 ? wd : w;
+            for (int i = 0; i < n; i++) wt[i] = (gm == -INFINITY) ? 0.0 : exp(ld[i] - gm);
+            M = gm;
+            md_prev = gm;
+        }
+        /* statistics come from FP64 weights when the FP64 oracle runs */
+        const double* ws = (do64 && do32) ? wd : w;
+
+        /* FP64 sums in particle order */
+        double S = 0.0, SX = 0.0, SXX = 0.0, Q = 0.0;
+        if (n_threads <= 1) {
+            for (int i = 0; i < n; i++) {
+                double xi = f64 ? xd[i] : (double)x[i];
+                double wi = ws[i];
+                S += wi;
+                SX += wi * xi;
+                SXX += wi * xi * xi;
+                Q += wi * wi;
+            }
+        } else {
+#pragma omp parallel for num_threads(n_threads) schedule(static) reduction(+:S,SX,SXX,Q)
+            for (int i = 0; i < n; i++) {
+                double xi = f64 ? xd[i] : (double)x[i];
+                double wi = ws[i];
+                S += wi;
+                SX += wi * xi;
+                SXX += wi * xi * xi;
+                Q += wi * wi;
+            }
+        }
+        /* the coupled mode takes the resampling decision and the CDF from
+         * the FP32 weights, so both filters share one genealogy */
+        double S32 = S, Q32 = Q;
+        if (do64 && do32) {
+            S32 = 0.0; Q32 = 0.0;
+            for (int i = 0; i < n; i++) { S32 += w[i]; Q32 += w[i] * w[i]; }
+        }
+
+        OrcTick r;
+        memset(&r, 0, sizeof(r));
+        r.tick = t;
+        int ok = isfinite(S) && S > 0.0 && isfinite(M);
+        if (ok) {
+            r.mean = SX / S;
+            r.var = SXX / S - r.mean * r.mean;
+            r.ess = (S * S) / Q;
+            r.ll_ref = off + M + log(fmax(S / (double)n, ll_floor));
+            r.ll_incr = carry ? (M + log(S)) - log(s_prev) : M + log(S / (double)n);
+        } else {
+            r.mean = r.var = r.ess = r.ll_ref = r.ll_incr = NAN;
+        }
+        ll_cum += r.ll_incr;
+        r.ll_cum = ll_cum;
+        r.h_mean = (float)r.mean;
+        r.log_lik = (float)r.ll_ref;
+
+        const double ess_dec = (S32 * S32) / Q32;
+        int resample = ok && isfinite(S32) && S32 > 0.0 &&
+                       (p->ess_threshold <= 0.0f || (ess_dec / (double)n) < (double)p->ess_threshold);
+        r.resampled = resample;
+        if (out) out[t] = r;
+
+        if (resample) {
+            uint32_t hr = orc_pcg_host_next(&hrng);
+            float u = ((float)(hr >> 9) * (1.0f / 8388608.0f)) / (float)n;
+            double c = 0.0;
+            for (int i = 0; i < n; i++) { c += w[i]; cdf[i] = c / S32; }
+            if (n_threads <= 1) {
+                orc_resample_ancestors(cdf, n, u, anc);
+            } else {
+#pragma omp parallel for num_threads(n_threads) schedule(static)
+                for (int i = 0; i < n; i++) {
+                    double tt = (double)orc_target(u, i, n);
+                    int lo = 0, hi = n - 1;
+                    while (lo < hi) {
+                        int mid = (lo + hi) >> 1;
+                        if (cdf[mid] < tt) lo = mid + 1; else hi = mid;
+                    }
+                    anc[i] = lo;
+                }
+            }
+            if (do64) {
+                for (int i = 0; i < n; i++) xdt[i] = xd[anc[i]];
+                double* sw = xd; xd = xdt; xdt = sw;
+            }
+            if (do32) {
+#pragma omp parallel for num_threads(n_threads) schedule(static)
+                for (int i = 0; i < n; i++) xt[i] = x[anc[i]];
+                float* sw = x; x = xt; xt = sw;
+            }
+            if (anc_out) memcpy(anc_out, anc, sizeof(int) * (size_t)n);
+            count++;
+            carry = 0;
+            off = 0.0;
+        } else {
+            carry = 1;
+            off += M;
+            s_prev = S;
+        }
+    }
+
+    if (x_out) memcpy(x_out, x, sizeof(float) * (size_t)n);
+    if (x64_out) memcpy(x64_out, xd, sizeof(double) * (size_t)n);
+    free(x); free(l); free(xd); free(ld); free(w); free(wd); free(cdf); free(anc);
+    free(xt); free(xdt); free(tmax); free(tf);
+    return count;
+}
+
+/* ========================================================================= */
+/* Point-mass filter: exact log p(y_{1:T}) of the Gaussian SV model with the  */
+/* reference's tick convention (tick 0 weights the stationary law, ticks >= 1 */
+/* propagate then weight; gpu_bpf/gpu_bpf_ptx_full.cu:591-599).               */
+/* ========================================================================= */
+
+double orc_grid_loglik(const float* y, int n_ticks, double rho, double sigma_z,
+                       double mu, int n_grid, double* ll_t) {
+    if (n_grid < 16) return NAN;
+    double sd = sqrt(sigma_z * sigma_z / fmax(1.0 - rho * rho, 1e-6));
+    double lo = mu - 10.0 * sd - 4.0 * sigma_z, hi = mu + 10.0 * sd + 4.0 * sigma_z;
+    double dx = (hi - lo) / (n_grid - 1);
+    double* g = (double*)malloc(sizeof(double) * (size_t)n_grid);
+    double* pi = (double*)malloc(sizeof(double) * (size_t)n_grid);
+    double* pr = (double*)malloc(sizeof(double) * (size_t)n_grid);
+    if (!g || !pi || !pr) { free(g); free(pi); free(pr); return NAN; }
+    for (int i = 0; i < n_grid; i++) g[i] = lo + dx * i;
+    /* stationary prior, normalised on the grid */
+    double z = 0.0;
+    for (int i = 0; i < n_grid; i++) {
+        double d = (g[i] - mu) / sd;
+        pi[i] = exp(-0.5 * d * d);
+        z += pi[i];
+    }
+    for (int i = 0; i < n_grid; i++) pi[i] /= z;
+    int band = (int)ceil(9.0 * sigma_z / dx) + 1;
+    double total = 0.0;
+    for (int t = 0; t < n_ticks; t++) {
+        if (t > 0) {
+            /* predict: pr[i] = sum_j K(g_i | g_j) pi[j], kernel normalised per j */
+            for (int i = 0; i < n_grid; i++) pr[i] = 0.0;
+            for (int j = 0; j < n_grid; j++) {
+                if (pi[j] == 0.0) continue;
+                double m = mu + rho * (g[j] - mu);
+                int c = (int)floor((m - lo) / dx);
+                int a = c - band < 0 ? 0 : c - band;
+                int b = c + band >= n_grid ? n_grid - 1 : c + band;
+                double kz = 0.0;
+                for (int i = a; i <= b; i++) {
+                    double d = (g[i] - m) / sigma_z;
+                    kz += exp(-0.5 * d * d);
+                }
+                if (kz <= 0.0) continue;
+                double sc = pi[j] / kz;
+                for (int i = a; i <= b; i++) {
+                    double d = (g[i] - m) / sigma_z;
+                    pr[i] += sc * exp(-0.5 * d * d);
+                }
+            }
+            for (int i = 0; i < n_grid; i++) pi[i] = pr[i];
+        }
+        double yy = (double)y[t];
+        double s = 0.0;
+        for (int i = 0; i < n_grid; i++) {
+            double lw = -0.91893853320467274178 - 0.5 * yy * yy * exp(-g[i]) - 0.5 * g[i];
+            pi[i] *= exp(lw);
+            s += pi[i];
+        }
+        double l = log(s);
+        if (ll_t) ll_t[t] = l;
+        total += l;
+        for (int i = 0; i < n_grid; i++) pi[i] /= s;
+    }
+    free(g); free(pi); free(pr);
+    return total;
+}

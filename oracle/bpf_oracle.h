@@ -1,0 +1,122 @@
+/*
+ * bpf_oracle.h -- CPU oracle for the SV bootstrap particle filter.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library links, loads or
+ * calls this code.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may use it, and only as the checker / the timed CPU
+ * baseline, never as the thing measured on the GPU.
+ *
+ * It restates, in plain C, the reference filter's per-tick algorithm:
+ *   - PCG32 per-particle streams seeded as in ptx_assembly/bpf_kernels_full.ptx:74-85
+ *     and stepped as in :140-164 (index-bound, one draw per propagating tick);
+ *   - the host resampling stream of gpu_bpf/gpu_bpf_ptx_full.cu:241-251,353;
+ *   - the Acklam (2003) inverse normal CDF with the FP32 coefficients of
+ *     ptx_assembly/bpf_kernels_full.ptx:166-207;
+ *   - the SV transition and Gaussian / Student-t log-weight of :406-450;
+ *   - ESS-triggered systematic resampling with weight carry
+ *     (gpu_bpf/gpu_bpf_ptx_full.cu:601-604,676-745; target rule of
+ *     gpu_bpf/gpu_bpf.cu:297-310);
+ *   - the matched-DGP generator of tests/test_bpf_matched_dgp.cu:59-175.
+ *
+ * Two filter modes:
+ *   ORC_MODE_F32 -- the FP32 restatement.  Per-particle arithmetic is written
+ *                   with explicit fmaf and correctly rounded + - * / sqrt only,
+ *                   so it is bit-reproducible on any IEEE machine; weights are
+ *                   formed tile-relative as the GPU forms them; sums, CDF and
+ *                   log-likelihood are FP64 sequential (textbook order).
+ *   ORC_MODE_F64 -- the FP64 oracle: same normals and same resampling
+ *                   uniforms, but states, log-weights, weights, CDF and
+ *                   log-likelihood in double with libm exp/log.
+ *   ORC_MODE_F64_COUPLED -- the FP64 oracle run beside the FP32 restatement
+ *                   and resampled with the FP32 run's decisions and ancestors,
+ *                   so both share one genealogy and the difference of their
+ *                   outputs is the arithmetic error of the FP32 pipeline.
+ *                   Outputs are the FP64 ones.
+ * Plus a point-mass (grid) filter giving the exact SV log-likelihood.
+ */
+#ifndef BPF_ORACLE_H
+#define BPF_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MODE_F32 0
+#define ORC_MODE_F64 1
+#define ORC_MODE_F64_COUPLED 2
+
+/* Filter parameters (reference names: rho = phi, sigma_z = sigma_eta). */
+typedef struct {
+    int   n_particles;
+    float rho;
+    float sigma_z;
+    float mu;
+    float nu_obs;        /* 0 = Gaussian observation noise */
+    float ess_threshold; /* <= 0: resample every tick */
+    int   seed;
+    int   tile;          /* particles per weight tile (must equal the GPU's) */
+} OrcParams;
+
+/* Per-tick outputs (same fields the GPU library reports). */
+typedef struct {
+    double mean;      /* sum w x / sum w, pre-resample */
+    double var;       /* sum w x^2 / sum w - mean^2 */
+    double ess;       /* (sum w)^2 / sum w^2 */
+    double ll_incr;   /* log p(y_t | y_{1:t-1}) estimate */
+    double ll_cum;    /* FP64 running sum of ll_incr */
+    double ll_ref;    /* reference-compatible BpfResult.log_lik */
+    float  h_mean;    /* (float)mean */
+    float  log_lik;   /* (float)ll_ref */
+    int    resampled;
+    int    tick;
+} OrcTick;
+
+/* ---- PCG32 ---------------------------------------------------------------- */
+uint64_t orc_pcg_state(int seed, uint32_t i, uint32_t k);   /* state of particle i before draw k */
+uint64_t orc_pcg_state_naive(int seed, uint32_t i, uint32_t k);
+uint32_t orc_pcg_output(uint64_t old_state);
+uint64_t orc_pcg_host_seed(int seed);
+uint32_t orc_pcg_host_next(uint64_t* state);
+float    orc_uniform(uint32_t r);
+
+/* ---- scalar math (bit-reproducible FP32) ---------------------------------- */
+float  orc_expf(float x);
+float  orc_logf(float x);
+float  orc_icdf_f32(float u);
+double orc_icdf_f64(double u);
+float  orc_normal(int seed, uint32_t i, uint32_t k);   /* z of particle i, draw k */
+
+/* ---- data generation ------------------------------------------------------ */
+void orc_dgp(int n_ticks, double rho, double sigma_z, double mu, double nu_state,
+             double nu_obs, uint64_t seed, double* true_h, double* returns);
+void orc_dgp_regimes(int n_ticks, double rho, double sigma_z, const double* mu_t,
+                     double nu_obs, uint64_t seed, double* true_h, double* returns);
+
+/* ---- filters -------------------------------------------------------------- */
+/* Run T ticks.  out[T] receives per-tick outputs; x_out (nullable, N floats)
+ * the particle states after the last tick; anc_out (nullable, N ints) the
+ * ancestors of the last resample; mu_t (nullable, T floats) overrides mu per
+ * tick.  Returns the resample count, or -1 on bad arguments. */
+int orc_bpf_run(const OrcParams* p, int mode, const float* y, const float* mu_t,
+                int n_ticks, OrcTick* out, float* x_out, double* x64_out,
+                int* anc_out, int n_threads);
+
+/* Systematic resampling given a normalised FP64 CDF: lower_bound of the float
+ * targets u + i/n (wrapped at 1) clamped to n-1. */
+void orc_resample_ancestors(const double* cdf, int n, float u, int* anc);
+
+/* Plant a mistake in the oracle (tests only): 0 none, 1 drop the -x/2 term
+ * of the log-weight, 2 negate phi in the transition, 3 upper_bound instead
+ * of lower_bound in resampling, 4 drop the weight carry.  Process-global. */
+void orc_set_fault(int code);
+
+/* Exact log p(y_{1:T}) of the Gaussian SV model by a point-mass filter. */
+double orc_grid_loglik(const float* y, int n_ticks, double rho, double sigma_z,
+                       double mu, int n_grid, double* ll_t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,213 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the timed CPU baseline.
+The shipped GPU library never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+
+MODE_F32 = 0
+MODE_F64 = 1
+MODE_F64_COUPLED = 2
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [
+        ("n_particles", ctypes.c_int),
+        ("rho", ctypes.c_float),
+        ("sigma_z", ctypes.c_float),
+        ("mu", ctypes.c_float),
+        ("nu_obs", ctypes.c_float),
+        ("ess_threshold", ctypes.c_float),
+        ("seed", ctypes.c_int),
+        ("tile", ctypes.c_int),
+    ]
+
+
+TICK_DTYPE = np.dtype(
+    [
+        ("mean", "f8"),
+        ("var", "f8"),
+        ("ess", "f8"),
+        ("ll_incr", "f8"),
+        ("ll_cum", "f8"),
+        ("ll_ref", "f8"),
+        ("h_mean", "f4"),
+        ("log_lik", "f4"),
+        ("resampled", "i4"),
+        ("tick", "i4"),
+    ]
+)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        u64, u32, f32, f64, i32 = (
+            ctypes.c_uint64,
+            ctypes.c_uint32,
+            ctypes.c_float,
+            ctypes.c_double,
+            ctypes.c_int,
+        )
+        P = ctypes.c_void_p
+        L.orc_pcg_state.restype = u64
+        L.orc_pcg_state.argtypes = [i32, u32, u32]
+        L.orc_pcg_state_naive.restype = u64
+        L.orc_pcg_state_naive.argtypes = [i32, u32, u32]
+        L.orc_pcg_output.restype = u32
+        L.orc_pcg_output.argtypes = [u64]
+        L.orc_pcg_host_seed.restype = u64
+        L.orc_pcg_host_seed.argtypes = [i32]
+        L.orc_pcg_host_next.restype = u32
+        L.orc_pcg_host_next.argtypes = [ctypes.POINTER(u64)]
+        L.orc_uniform.restype = f32
+        L.orc_uniform.argtypes = [u32]
+        L.orc_expf.restype = f32
+        L.orc_expf.argtypes = [f32]
+        L.orc_logf.restype = f32
+        L.orc_logf.argtypes = [f32]
+        L.orc_icdf_f32.restype = f32
+        L.orc_icdf_f32.argtypes = [f32]
+        L.orc_icdf_f64.restype = f64
+        L.orc_icdf_f64.argtypes = [f64]
+        L.orc_normal.restype = f32
+        L.orc_normal.argtypes = [i32, u32, u32]
+        L.orc_dgp.restype = None
+        L.orc_dgp.argtypes = [i32, f64, f64, f64, f64, f64, u64, P, P]
+        L.orc_dgp_regimes.restype = None
+        L.orc_dgp_regimes.argtypes = [i32, f64, f64, P, f64, u64, P, P]
+        L.orc_bpf_run.restype = i32
+        L.orc_bpf_run.argtypes = [ctypes.POINTER(OrcParams), i32, P, P, i32, P, P, P, P, i32]
+        L.orc_resample_ancestors.restype = None
+        L.orc_resample_ancestors.argtypes = [P, i32, f32, P]
+        L.orc_set_fault.restype = None
+        L.orc_set_fault.argtypes = [i32]
+        L.orc_grid_loglik.restype = f64
+        L.orc_grid_loglik.argtypes = [P, i32, f64, f64, f64, i32, P]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def expf(x):
+    return lib().orc_expf(float(x))
+
+
+def logf(x):
+    return lib().orc_logf(float(x))
+
+
+def icdf_f32(u):
+    return lib().orc_icdf_f32(float(u))
+
+
+def icdf_f64(u):
+    return lib().orc_icdf_f64(float(u))
+
+
+def pcg_state(seed, i, k):
+    return lib().orc_pcg_state(int(seed), int(i), int(k))
+
+
+def pcg_state_naive(seed, i, k):
+    return lib().orc_pcg_state_naive(int(seed), int(i), int(k))
+
+
+def pcg_output(s):
+    return lib().orc_pcg_output(int(s))
+
+
+def uniform(r):
+    return lib().orc_uniform(int(r))
+
+
+def normal(seed, i, k):
+    return lib().orc_normal(int(seed), int(i), int(k))
+
+
+def host_stream(seed, n):
+    st = ctypes.c_uint64(lib().orc_pcg_host_seed(int(seed)))
+    return [lib().orc_pcg_host_next(ctypes.byref(st)) for _ in range(n)]
+
+
+def dgp(n_ticks, rho=0.97, sigma_z=0.15, mu=-1.0, nu_state=0.0, nu_obs=0.0, seed=42):
+    h = np.empty(n_ticks, np.float64)
+    y = np.empty(n_ticks, np.float64)
+    lib().orc_dgp(n_ticks, rho, sigma_z, mu, nu_state, nu_obs, seed, _ptr(h), _ptr(y))
+    return h, y
+
+
+def dgp_regimes(mu_t, rho=0.97, sigma_z=0.15, nu_obs=0.0, seed=42):
+    mu_t = np.ascontiguousarray(mu_t, np.float64)
+    n = len(mu_t)
+    h = np.empty(n, np.float64)
+    y = np.empty(n, np.float64)
+    lib().orc_dgp_regimes(n, rho, sigma_z, _ptr(mu_t), nu_obs, seed, _ptr(h), _ptr(y))
+    return h, y
+
+
+def run_filter(y, n_particles, rho=0.97, sigma_z=0.15, mu=-1.0, nu_obs=0.0,
+               ess_threshold=0.0, seed=42, tile=256, mode=MODE_F32, mu_t=None,
+               want_x=False, want_anc=False, threads=1):
+    """Run the oracle filter; returns (ticks structured array, resample count, x, anc)."""
+    y = np.ascontiguousarray(y, np.float32)
+    T = len(y)
+    p = OrcParams(n_particles, rho, sigma_z, mu, nu_obs, ess_threshold, seed, tile)
+    out = np.zeros(T, TICK_DTYPE)
+    x = np.empty(n_particles, np.float32) if (want_x and mode == MODE_F32) else None
+    x64 = np.empty(n_particles, np.float64) if (want_x and mode != MODE_F32) else None
+    anc = np.empty(n_particles, np.int32) if want_anc else None
+    mt = None if mu_t is None else np.ascontiguousarray(mu_t, np.float32)
+    cnt = lib().orc_bpf_run(ctypes.byref(p), mode, _ptr(y), _ptr(mt), T, _ptr(out),
+                            _ptr(x), _ptr(x64), _ptr(anc), int(threads))
+    if cnt < 0:
+        raise ValueError("oracle rejected the arguments")
+    return out, cnt, (x if x is not None else x64), anc
+
+
+def resample_ancestors(cdf, u):
+    cdf = np.ascontiguousarray(cdf, np.float64)
+    anc = np.empty(len(cdf), np.int32)
+    lib().orc_resample_ancestors(_ptr(cdf), len(cdf), float(u), _ptr(anc))
+    return anc
+
+
+class planted_fault:
+    """Context manager planting mistake `code` in the oracle (see bpf_oracle.h)."""
+
+    def __init__(self, code):
+        self.code = code
+
+    def __enter__(self):
+        lib().orc_set_fault(self.code)
+
+    def __exit__(self, *exc):
+        lib().orc_set_fault(0)
+
+
+def grid_loglik(y, rho=0.97, sigma_z=0.15, mu=-1.0, n_grid=2048):
+    y = np.ascontiguousarray(y, np.float32)
+    llt = np.empty(len(y), np.float64)
+    tot = lib().orc_grid_loglik(_ptr(y), len(y), rho, sigma_z, mu, n_grid, _ptr(llt))
+    return tot, llt

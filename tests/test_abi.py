@@ -1,0 +1,55 @@
+"""CPU tests of the C-ABI library: it loads without a GPU and exports every
+function include/gpu_bpf.h declares (no compute calls)."""
+import ctypes
+import os
+import subprocess
+
+import pytest
+
+import gpu_bpf
+
+
+def test_header_declares_reference_surface():
+    syms = set(gpu_bpf.header_symbols())
+    # gpu_bpf/gpu_bpf_full.cuh:144-301 (plain-filter API)
+    ref = {
+        "gpu_bpf_create", "gpu_bpf_destroy", "gpu_bpf_step", "gpu_bpf_step_async",
+        "gpu_bpf_get_result", "gpu_bpf_run_rmse", "gpu_bpf_set_ess_threshold",
+        "gpu_bpf_get_resample_count", "gpu_bpf_enable_mu_learning",
+        "gpu_bpf_disable_mu_learning", "gpu_bpf_enable_rho_learning", "gpu_bpf_get_mu",
+        "gpu_bpf_get_rho", "gpu_bpf_set_mu", "gpu_bpf_set_rho", "gpu_bpf_set_bands",
+        "gpu_bpf_set_adaptive_bands",
+    }
+    assert ref <= syms
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(gpu_bpf.LIB_PATH):
+        pytest.fail(f"{gpu_bpf.LIB_PATH} not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(gpu_bpf.LIB_PATH)
+    missing = [s for s in gpu_bpf.header_symbols() if not hasattr(lib, s)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", gpu_bpf.LIB_PATH],
+                         capture_output=True, text=True, check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    assert set(gpu_bpf.header_symbols()) <= exported
+
+
+def test_header_is_plain_c():
+    """The boundary header compiles as C99 with no HIP/CUDA headers."""
+    src = '#include "gpu_bpf.h"\nint main(void){BpfResult r; (void)r; return (int)sizeof(BpfTickStats) - 64;}\n'
+    inc = os.path.join(gpu_bpf.REPO_DIR, "include")
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", inc, "-x", "c", "-",
+                        "-o", "/dev/null"], input=src, text=True, capture_output=True)
+    assert r.returncode == 0, r.stderr
+    includes = [ln for ln in open(gpu_bpf.HEADER_PATH) if ln.lstrip().startswith("#include")]
+    assert not includes, includes
+
+
+def test_python_mirror_fails_loudly_without_library(tmp_path):
+    import importlib
+    mod = importlib.reload(gpu_bpf)
+    mod._lib = None
+    with pytest.raises(mod.GpuBpfError):
+        mod.load(str(tmp_path / "missing.so"))
+    importlib.reload(gpu_bpf)

@@ -1,0 +1,222 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle.
+
+Bar: particle states bit-exact (same PCG32 streams, same FP32 per-particle
+arithmetic, same ancestors); FP64 statistics and log-likelihood within 1e-12
+relative (only the summation order differs); FP32 outputs within 1 ulp.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import gpu_bpf
+
+pytestmark = pytest.mark.gpu
+
+RHO, SIG, MU = 0.97, 0.15, -1.0
+STAT_FIELDS = ("mean", "var", "ess", "ll_incr", "ll_cum", "ll_ref")
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ydata(orc):
+    h, y = orc.dgp(1000, RHO, SIG, MU, seed=42)
+    return h, y.astype(np.float32)
+
+
+def gpu_run(n, y, thr, seed=42, nu_obs=0.0, items=None, mu=MU):
+    f = gpu_bpf.GpuBpf(n, RHO, SIG, mu, nu_obs=nu_obs, seed=seed, ess_threshold=thr,
+                       items_per_thread=items)
+    res, st = f.run(y)
+    x = f.particles()
+    tile, _ = f.layout()
+    cnt = f.resample_count()
+    f.close()
+    return res, st, x, tile, cnt
+
+
+def assert_stats_close(st, ref, rtol=1e-12):
+    np.testing.assert_array_equal(st["resampled"], ref["resampled"])
+    for k in STAT_FIELDS:
+        a, b = st[k], ref[k]
+        fin = np.isfinite(b)
+        np.testing.assert_array_equal(np.isfinite(a), fin, err_msg=k)
+        scale = np.maximum(np.abs(b[fin]), 1.0)
+        err = np.max(np.abs(a[fin] - b[fin]) / scale) if fin.any() else 0.0
+        assert err <= rtol, (k, err)
+    for k in ("h_mean", "log_lik"):
+        a, b = st[k].astype(np.float64), ref[k].astype(np.float64)
+        fin = np.isfinite(b)
+        ulp = np.spacing(np.abs(b[fin]).astype(np.float32)).astype(np.float64)
+        assert np.all(np.abs(a[fin] - b[fin]) <= ulp), k
+
+
+def test_init_streams_bitexact(orc):
+    for n in (1, 7, 4099, 65536):
+        say("init", n)
+        f = gpu_bpf.GpuBpf(n, RHO, SIG, MU, seed=42)
+        x = f.particles()
+        f.close()
+        _, _, xo, _ = orc.run_filter(np.zeros(0, np.float32), n, want_x=True)
+        np.testing.assert_array_equal(x, xo)
+
+
+@pytest.mark.parametrize("thr", [0.0, 0.5])
+def test_plain_filter_parity(orc, ydata, thr):
+    """Plain SV filter, N=65,536, seed 42: always-resample and ESS<0.5N."""
+    _, y = ydata
+    n, T = 65536, 400
+    say("plain", n, T, thr)
+    res, st, x, tile, cnt = gpu_run(n, y[:T], thr)
+    ref, rcnt, xo, _ = orc.run_filter(y[:T], n, ess_threshold=thr, tile=tile, want_x=True,
+                                      threads=16)
+    assert cnt == rcnt
+    assert_stats_close(st, ref)
+    np.testing.assert_array_equal(x, xo)
+    np.testing.assert_array_equal(res["h_mean"], st["h_mean"])
+    np.testing.assert_array_equal(res["log_lik"], st["log_lik"])
+
+
+@pytest.mark.parametrize("items", [1, 2, 4, 8, 16])
+def test_tile_geometries(orc, ydata, items):
+    """Every per-lane particle count, N not a multiple of the tile."""
+    _, y = ydata
+    n, T = 100_003, 60
+    say("tile", items, n, T)
+    _, st, x, tile, _ = gpu_run(n, y[:T], 0.5, items=items)
+    assert tile == 256 * items
+    ref, _, xo, _ = orc.run_filter(y[:T], n, ess_threshold=0.5, tile=tile, want_x=True,
+                                   threads=16)
+    assert_stats_close(st, ref)
+    np.testing.assert_array_equal(x, xo)
+
+
+def test_student_t_observations(orc, ydata):
+    _, y = ydata
+    n, T = 50_000, 200
+    say("student-t", n, T)
+    _, st, x, tile, _ = gpu_run(n, y[:T], 0.5, nu_obs=5.0)
+    ref, _, xo, _ = orc.run_filter(y[:T], n, nu_obs=5.0, ess_threshold=0.5, tile=tile,
+                                   want_x=True, threads=16)
+    assert_stats_close(st, ref)
+    np.testing.assert_array_equal(x, xo)
+
+
+@pytest.mark.parametrize("n", [1, 2, 255, 257, 4096])
+def test_small_particle_counts(orc, ydata, n):
+    _, y = ydata
+    T = 40
+    say("small", n)
+    _, st, x, tile, _ = gpu_run(n, y[:T], 0.0)
+    ref, _, xo, _ = orc.run_filter(y[:T], n, ess_threshold=0.0, tile=tile, want_x=True)
+    assert_stats_close(st, ref)
+    np.testing.assert_array_equal(x, xo)
+
+
+def test_million_particles_ess(orc, ydata):
+    """N = 2^20 (two particles per lane), ESS-triggered, 300 ticks."""
+    _, y = ydata
+    n, T = 1 << 20, 300
+    say("1M", n, T)
+    _, st, x, tile, _ = gpu_run(n, y[:T], 0.5)
+    ref, _, xo, _ = orc.run_filter(y[:T], n, ess_threshold=0.5, tile=tile, want_x=True,
+                                   threads=16)
+    assert_stats_close(st, ref)
+    np.testing.assert_array_equal(x, xo)
+
+
+def test_against_fp64_oracle(orc, ydata):
+    """Loglik error vs the FP64 oracle on the same genealogy <= 1e-4 relative."""
+    _, y = ydata
+    n, T = 65536, 1000
+    say("fp64", n, T)
+    _, st, _, tile, _ = gpu_run(n, y[:T], 0.0)
+    ref = orc.run_filter(y[:T], n, ess_threshold=0.0, tile=tile,
+                         mode=orc.MODE_F64_COUPLED)[0]
+    rel = abs(st["ll_cum"][-1] - ref["ll_cum"][-1]) / abs(ref["ll_cum"][-1])
+    assert rel <= 1e-4
+    np.testing.assert_allclose(st["mean"], ref["mean"], atol=1e-3)
+
+
+def test_sync_step_api_matches_batch(ydata):
+    """gpu_bpf_step / step_async+get_result give the batch run's results."""
+    _, y = ydata
+    n, T = 20_000, 50
+    say("step-api", n, T)
+    res, st, _, _, cnt = gpu_run(n, y[:T], 0.5)
+    f = gpu_bpf.GpuBpf(n, RHO, SIG, MU, seed=42)      # default threshold 0.5
+    for t in range(T):
+        if t % 2:
+            h, ll = f.step(y[t])
+        else:
+            f.step_async(y[t])
+            h, ll = f.get_result()
+        assert h == res["h_mean"][t] and ll == res["log_lik"][t]
+    assert f.resample_count() == cnt
+    assert f.timestep() == T
+    f.close()
+
+
+def test_regime_shift_set_mu(orc):
+    """set_mu between ticks (the large-filter workload's mu schedule)."""
+    mu_t = np.repeat(np.float32([-1.5, -0.5, 0.5, -1.5]), 50)
+    _, y = orc.dgp_regimes(mu_t.astype(np.float64), RHO, SIG, seed=42)
+    y = y.astype(np.float32)
+    n = 30_000
+    say("regimes", n)
+    f = gpu_bpf.GpuBpf(n, RHO, SIG, float(mu_t[0]), seed=42, ess_threshold=0.5)
+    sts = []
+    for blk in range(4):
+        f.set_mu(float(mu_t[blk * 50]))
+        assert f.get_mu() == np.float32(mu_t[blk * 50])
+        sts.append(f.run(y[blk * 50:(blk + 1) * 50])[1])
+    st = np.concatenate(sts)
+    x = f.particles()
+    tile, _ = f.layout()
+    f.close()
+    ref, _, xo, _ = orc.run_filter(y, n, mu=float(mu_t[0]), mu_t=mu_t, ess_threshold=0.5,
+                                   tile=tile, want_x=True, threads=16)
+    assert_stats_close(st, ref)
+    np.testing.assert_array_equal(x, xo)
+
+
+def test_nan_observation_propagates(orc, ydata):
+    _, y = ydata
+    y = y[:30].copy()
+    y[10] = np.nan
+    n = 5000
+    say("nan", n)
+    _, st, x, tile, _ = gpu_run(n, y, 0.5)
+    ref, _, xo, _ = orc.run_filter(y, n, ess_threshold=0.5, tile=tile, want_x=True)
+    assert math.isnan(st["mean"][10]) and st["resampled"][10] == 0
+    assert_stats_close(st, ref)
+    np.testing.assert_array_equal(x, xo)
+
+
+def test_determinism(ydata):
+    _, y = ydata
+    say("determinism")
+    a = gpu_run(200_000, y[:100], 0.5)
+    b = gpu_run(200_000, y[:100], 0.5)
+    np.testing.assert_array_equal(a[2], b[2])
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_run_rmse(ydata):
+    h, y = ydata
+    say("rmse")
+    r = gpu_bpf.run_rmse(y.astype(np.float64), h, 10_000, RHO, SIG, MU)
+    res, _, _, _, _ = gpu_run(10_000, y, 0.5)
+    e = res["h_mean"][100:].astype(np.float64) - h[100:]
+    assert abs(r - math.sqrt(np.mean(e * e))) < 1e-9
+    assert 0.05 < r < 0.8
+
+
+def test_errors_are_reported():
+    with pytest.raises(gpu_bpf.GpuBpfError, match="nu_state"):
+        gpu_bpf.GpuBpf(1000, RHO, SIG, MU, nu_state=5.0)
+    with pytest.raises(gpu_bpf.GpuBpfError, match="positive"):
+        gpu_bpf.GpuBpf(0, RHO, SIG, MU)
