@@ -104,6 +104,72 @@ __global__ __launch_bounds__(kBlock) void k_init(InitArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// The K standard normals of this lane for one draw index: PCG32 state by
+// jump-ahead (state = pcg_a + inc_i * pcg_g, inc_i = 2i + 1; the next particle
+// of the lane adds 2 pcg_g), uniform, Acklam ICDF.  Central draws are
+// evaluated in place; tail uniforms (4.85%) go through an LDS queue that the
+// whole block evaluates densely, so no wave runs the long tail branch for a
+// single lane.  Item (v, c) of lane tid is particle base + (v*kBlock+tid)*VEC+c.
+// Contains two block barriers; every thread of the block must call it.
+// ---------------------------------------------------------------------------
+
+template <int K>
+__device__ inline void lane_normals(uint64_t pcg_a, uint64_t pcg_g, long long base, int tid,
+                                    float* s_q, int* s_cnt, float (&z)[K]) {
+    constexpr int VEC = K < 4 ? K : 4;
+    constexpr int NV = K / VEC;
+    constexpr int NW = kBlock / kWave;
+    const int lane = tid & (kWave - 1);
+    const int wid = tid / kWave;
+    bool tail[K];
+    int nt = 0;
+    const uint64_t g2 = pcg_g * 2u;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        long long i0 = base + (long long)(v * kBlock + tid) * VEC;
+        uint64_t s = pcg_a + (uint64_t)(2 * i0 + 1) * pcg_g;
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) {
+            const int k = v * VEC + c;
+            float u = pcg_uniform(pcg_output(s));
+            s += g2;
+            bool cen = icdf_is_central(u);
+            float zc = icdf_central(u - 0.5f);
+            z[k] = cen ? zc : u;    // tail items carry u until the queue pass
+            tail[k] = !cen;
+            nt += cen ? 0 : 1;
+        }
+    }
+    // block-wide exclusive scan of the per-lane tail counts
+    int incl = wave_incl_scan(nt, lane);
+    if (lane == kWave - 1) s_cnt[wid] = incl;
+    __syncthreads();
+    int wbase = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        int c = s_cnt[w];
+        wbase += (w < wid) ? c : 0;
+        total += c;
+    }
+    const int first = wbase + incl - nt;
+    int pos = first;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (tail[k]) s_q[pos] = z[k];
+        pos += tail[k] ? 1 : 0;
+    }
+    __syncthreads();
+    for (int e = tid; e < total; e += kBlock) s_q[e] = icdf_tail(s_q[e]);
+    __syncthreads();
+    pos = first;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (tail[k]) z[k] = s_q[pos];
+        pos += tail[k] ? 1 : 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Propagate + weight + tile partials.
 // Tile b holds particles [b*TILE, (b+1)*TILE); lane `tid` owns VEC-wide
 // groups at (v*kBlock + tid)*VEC, so every load and store instruction of a
@@ -137,6 +203,8 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
 
     float x[K], l[K];
 #pragma unroll
+    for (int k = 0; k < K; ++k) l[k] = 0.0f;
+#pragma unroll
     for (int v = 0; v < NV; ++v) {
         long long off = base + (long long)(v * kBlock + tid) * VEC;
         load_vec<VEC>(xb + off, &x[v * VEC]);
@@ -149,6 +217,14 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
         }
     }
 
+    if (a.x_save) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            long long off = base + (long long)(v * kBlock + tid) * VEC;
+            store_vec<VEC>(a.x_save + off, &x[v * VEC]);
+        }
+    }
+
     ObsModel obs = a.obs;
     {
         float yv = a.y_ptr ? *a.y_ptr : a.y;
@@ -157,52 +233,7 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
 
     if (a.do_prop) {
         float z[K];
-        bool tail[K];
-        int nt = 0;
-        const uint64_t g2 = a.pcg_g * 2u;
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            long long i0 = base + (long long)(v * kBlock + tid) * VEC;
-            uint64_t s = a.pcg_a + (uint64_t)(2 * i0 + 1) * a.pcg_g;
-#pragma unroll
-            for (int c = 0; c < VEC; ++c) {
-                const int k = v * VEC + c;
-                float u = pcg_uniform(pcg_output(s));
-                s += g2;
-                bool cen = icdf_is_central(u);
-                float zc = icdf_central(u - 0.5f);
-                z[k] = cen ? zc : u;    // tail items carry u until the queue pass
-                tail[k] = !cen;
-                nt += cen ? 0 : 1;
-            }
-        }
-        // block-wide exclusive scan of the per-lane tail counts
-        int incl = wave_incl_scan(nt, lane);
-        if (lane == kWave - 1) s_cnt[wid] = incl;
-        __syncthreads();
-        int wbase = 0, total = 0;
-#pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            int c = s_cnt[w];
-            wbase += (w < wid) ? c : 0;
-            total += c;
-        }
-        const int first = wbase + incl - nt;
-        int pos = first;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            if (tail[k]) s_q[pos] = z[k];
-            pos += tail[k] ? 1 : 0;
-        }
-        __syncthreads();
-        for (int e = tid; e < total; e += kBlock) s_q[e] = icdf_tail(s_q[e]);
-        __syncthreads();
-        pos = first;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            if (tail[k]) z[k] = s_q[pos];
-            pos += tail[k] ? 1 : 0;
-        }
+        lane_normals<K>(a.pcg_a, a.pcg_g, base, tid, s_q, s_cnt, z);
 #pragma unroll
         for (int k = 0; k < K; ++k) x[k] = transition(x[k], a.mu, a.phi, a.sig, z[k]);
 #pragma unroll
@@ -388,6 +419,8 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
             }
             c->u = u;
             c->wrap_start = lo;
+            if (a.u_log) *a.u_log = u;
+            if (a.w_log) *a.w_log = lo;
             c->src = c->cur;
             c->dst = 1 - c->cur;
             c->cur = c->dst;
@@ -477,53 +510,30 @@ __device__ inline int count_le(double v, float u, int lo, int hi, int n, bool wr
     return lo;
 }
 
+// Builds the tile's CDF in s_c (TILE doubles) from the lane's weights e[K]
+// (coalesced layout) and returns the tile's output ranges in s_rng:
+// [0,1) unwrapped targets, [2,3) wrapped targets.  Returns the local index of
+// the tile's last valid particle.  s_c doubles as the staging area of e.
+// Contains block barriers; every thread of the block must call it.
 template <int K>
-__global__ __launch_bounds__(kBlock) void k_resample(ResArgs a) {
-    const Ctl* ctl = a.ctl;
-    if (!ctl->resample) return;
+__device__ inline int tile_cdf(const float (&e)[K], int b, int n, int n_tiles, double ft,
+                               double Bb, double Bb1, float u, int W, double* s_c,
+                               double* s_w, int* s_rng) {
     constexpr int VEC = K < 4 ? K : 4;
     constexpr int NV = K / VEC;
     constexpr int TILE = kBlock * K;
-    constexpr int NW = kBlock / kWave;
     constexpr int PAD = TILE + TILE / 16 + 1;
     static_assert(PAD * sizeof(float) <= TILE * sizeof(double), "e staging must fit in s_c");
-    __shared__ float s_x[TILE];
-    __shared__ double s_c[TILE];
-    __shared__ double s_w[NW];
-    __shared__ int s_rng[4];
-    float* s_e = reinterpret_cast<float*>(s_c);
-
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
-    const int b = blockIdx.x;
-    const int n = a.n;
     const long long base = (long long)b * TILE;
-    const float* __restrict__ xs = ctl->src ? a.x1 : a.x0;
-    float* __restrict__ xd = ctl->dst ? a.x1 : a.x0;
-    const float u = ctl->u;
-    const int W = ctl->wrap_start;
-    const float mb = a.m[b];
-    const bool dead = (mb == -__builtin_inff());
-
-    ObsModel obs = a.obs;
-    {
-        float yv = a.y_ptr ? *a.y_ptr : a.y;
-        obs.ysq = yv * yv;
-    }
+    float* s_e = reinterpret_cast<float*>(s_c);
 
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-        const int loc0 = (v * kBlock + tid) * VEC;
-        float xv[VEC], lv[VEC];
-        load_vec<VEC>(xs + base + loc0, xv);
-        if (a.lw) load_vec<VEC>(a.lw + base + loc0, lv);
 #pragma unroll
         for (int c = 0; c < VEC; ++c) {
-            const int loc = loc0 + c;
-            float lk = a.lw ? lv[c] : log_weight(xv[c], obs);
-            bool valid = base + loc < n;
-            float e = (dead || !valid) ? 0.0f : expf_cw(lk - mb);
-            s_x[loc] = xv[c];
-            s_e[loc + (loc >> 4)] = e;
+            const int loc = (v * kBlock + tid) * VEC + c;
+            s_e[loc + (loc >> 4)] = e[v * VEC + c];
         }
     }
     __syncthreads();
@@ -544,8 +554,7 @@ __global__ __launch_bounds__(kBlock) void k_resample(ResArgs a) {
     for (int w = 0; w < wid; ++w) excl += s_w[w];
     excl = (tid == 0) ? 0.0 : excl;
 
-    const double ft = a.ftil[b], Bb = a.btil[b], Bb1 = a.btil[b + 1];
-    const long long lastll = (b == a.n_tiles - 1) ? ((long long)n - 1 - base) : (long long)(TILE - 1);
+    const long long lastll = (b == n_tiles - 1) ? ((long long)n - 1 - base) : (long long)(TILE - 1);
     const int last = (int)lastll;
     double cv[K];
     double L = excl;
@@ -558,7 +567,7 @@ __global__ __launch_bounds__(kBlock) void k_resample(ResArgs a) {
     double pm = wave_incl_max(cv[K - 1], lane);
     double prev = __shfl_up(pm, 1, kWave);
     prev = (lane == 0) ? Bb : prev;
-    __syncthreads();   // s_w reuse and s_e (aliasing s_c) reads are done
+    __syncthreads();   // s_w reads and s_e (aliasing s_c) reads are done
     if (lane == kWave - 1) s_w[wid] = pm;
     __syncthreads();
     for (int w = 0; w < wid; ++w) prev = fmax(prev, s_w[w]);
@@ -573,31 +582,222 @@ __global__ __launch_bounds__(kBlock) void k_resample(ResArgs a) {
     if (tid < 4) {
         int r;
         if (tid == 0)      r = (b == 0) ? 0 : count_le(Bb, u, 0, W, n, false);
-        else if (tid == 1) r = (b == a.n_tiles - 1) ? W : count_le(Bb1, u, 0, W, n, false);
+        else if (tid == 1) r = (b == n_tiles - 1) ? W : count_le(Bb1, u, 0, W, n, false);
         else if (tid == 2) r = (b == 0) ? W : count_le(Bb, u, W, n, n, true);
-        else               r = (b == a.n_tiles - 1) ? n : count_le(Bb1, u, W, n, n, true);
+        else               r = (b == n_tiles - 1) ? n : count_le(Bb1, u, W, n, n, true);
         s_rng[tid] = r;
     }
     __syncthreads();
+    return last;
+}
+
+__device__ inline int lds_lower_bound(const double* c, int last, double t) {
+    int lo = 0, hi = last;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (c[mid] < t) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_resample(ResArgs a) {
+    const Ctl* ctl = a.ctl;
+    if (!ctl->resample) return;
+    constexpr int VEC = K < 4 ? K : 4;
+    constexpr int NV = K / VEC;
+    constexpr int TILE = kBlock * K;
+    constexpr int NW = kBlock / kWave;
+    __shared__ float s_x[TILE];
+    __shared__ double s_c[TILE];
+    __shared__ double s_w[NW];
+    __shared__ int s_rng[4];
+
+    const int tid = threadIdx.x;
+    const int b = blockIdx.x;
+    const int n = a.n;
+    const long long base = (long long)b * TILE;
+    const float* __restrict__ xs = ctl->src ? a.x1 : a.x0;
+    float* __restrict__ xd = ctl->dst ? a.x1 : a.x0;
+    const float u = ctl->u;
+    const int W = ctl->wrap_start;
+    const float mb = a.m[b];
+    const bool dead = (mb == -__builtin_inff());
+
+    ObsModel obs = a.obs;
+    {
+        float yv = a.y_ptr ? *a.y_ptr : a.y;
+        obs.ysq = yv * yv;
+    }
+
+    float e[K];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const int loc0 = (v * kBlock + tid) * VEC;
+        float xv[VEC], lv[VEC];
+        load_vec<VEC>(xs + base + loc0, xv);
+        if (a.lw) load_vec<VEC>(a.lw + base + loc0, lv);
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) {
+            const int loc = loc0 + c;
+            float lk = a.lw ? lv[c] : log_weight(xv[c], obs);
+            bool valid = base + loc < n;
+            e[v * VEC + c] = (dead || !valid) ? 0.0f : expf_cw(lk - mb);
+            s_x[loc] = xv[c];
+        }
+    }
+    const int last = tile_cdf<K>(e, b, n, a.n_tiles, a.ftil[b], a.btil[b], a.btil[b + 1], u, W,
+                                 s_c, s_w, s_rng);
 
     const int ilo = s_rng[0], ihi = s_rng[1], jlo = s_rng[2], jhi = s_rng[3];
     for (int i = ilo + tid; i < ihi; i += kBlock) {
         double t = (double)target_unwrapped(u, i, n);
-        int lo = 0, hi = last;
-        while (lo < hi) {
-            int mid = (lo + hi) >> 1;
-            if (s_c[mid] < t) lo = mid + 1; else hi = mid;
-        }
-        xd[i] = s_x[lo];
+        xd[i] = s_x[lds_lower_bound(s_c, last, t)];
     }
     for (int i = jlo + tid; i < jhi; i += kBlock) {
         double t = (double)(target_unwrapped(u, i, n) - 1.0f);
-        int lo = 0, hi = last;
-        while (lo < hi) {
-            int mid = (lo + hi) >> 1;
-            if (s_c[mid] < t) lo = mid + 1; else hi = mid;
+        xd[i] = s_x[lds_lower_bound(s_c, last, t)];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint of one always-resample tick (reverse mode, ancestors held fixed).
+// With LL = sum_t [logsumexp_j lw(x_t,j) - log N] and
+//   x_t,i = mu + phi (x_{t-1, a_i} - mu) + sigma z_t,i,
+// the adjoint of particle j at tick t is
+//   xbar_t,j = W_t,j lw'(x_t,j) + phi * sum_{i : a_t,i = j} xbar_{t+1,i},
+// and the tile accumulates sum_j xbar_t,j * dx_t,j/d(mu, phi, sigma).  The
+// children of j are the outputs of the tile's claimed target range with
+// c_{j-1} < t_i <= c_j: a contiguous run, found by binary search over the
+// non-decreasing targets.  The CDF is rebuilt bit for bit as in k_resample.
+// ---------------------------------------------------------------------------
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_adjoint(AdjArgs a) {
+    constexpr int VEC = K < 4 ? K : 4;
+    constexpr int NV = K / VEC;
+    constexpr int TILE = kBlock * K;
+    constexpr int NW = kBlock / kWave;
+    __shared__ int s_bu[TILE];     // children run end (unwrapped targets) per parent
+    __shared__ int s_cnt[NW];
+    __shared__ double s_c[TILE];
+    __shared__ double s_w[NW];
+    __shared__ int s_rng[4];
+    __shared__ double s_red[3][NW];
+    float* s_q = reinterpret_cast<float*>(s_bu);   // normal-draw queue, used before s_bu
+
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const int b = blockIdx.x;
+    const int n = a.n;
+    const long long base = (long long)b * TILE;
+    const float mb = a.m[b];
+    const bool dead = (mb == -__builtin_inff());
+
+    float x[K], pre[K], z[K];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        long long off = base + (long long)(v * kBlock + tid) * VEC;
+        load_vec<VEC>(a.pre + off, &pre[v * VEC]);
+    }
+    // draw t of the streams (draw 0, the initial one, at tick 0)
+    lane_normals<K>(a.pcg_a, a.pcg_g, base, tid, s_q, s_cnt, z);
+#pragma unroll
+    for (int k = 0; k < K; ++k) x[k] = (a.tick > 0) ? transition(pre[k], a.mu, a.phi, a.sig, z[k]) : pre[k];
+    ObsModel obs = a.obs;
+    obs.ysq = a.y * a.y;
+
+    float e[K], gp[K];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) {
+            const int k = v * VEC + c;
+            const bool valid = base + (long long)(v * kBlock + tid) * VEC + c < n;
+            float lk = log_weight(x[k], obs);
+            e[k] = (dead || !valid) ? 0.0f : expf_cw(lk - mb);
+            float eta2 = obs.ysq * expf_cw(-x[k]);
+            gp[k] = (obs.nu > 0.0f) ? obs.nph * eta2 / (obs.nu + eta2) - 0.5f : 0.5f * eta2 - 0.5f;
         }
-        xd[i] = s_x[lo];
+    }
+    const double ft = a.ftil[b];
+    const float u = *a.u_ptr;
+    const int W = *a.w_ptr;
+    const int last = tile_cdf<K>(e, b, n, a.n_tiles, ft, a.btil[b], a.btil[b + 1], u, W,
+                                 s_c, s_w, s_rng);
+    const int ilo = s_rng[0], ihi = s_rng[1], jlo = s_rng[2], jhi = s_rng[3];
+
+    // children run ends of every parent
+    if (a.xbar_next) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) {
+                const int loc = (v * kBlock + tid) * VEC + c;
+                const double cj = s_c[loc];
+                s_bu[loc] = (loc >= last) ? ihi : count_le(cj, u, ilo, ihi, n, false);
+            }
+        }
+    }
+    __syncthreads();
+
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    const double phi = (double)a.phi;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        float xb_out[VEC];
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) {
+            const int k = v * VEC + c;
+            const int loc = (v * kBlock + tid) * VEC + c;
+            const bool valid = base + loc < n;
+            double xb = (double)e[k] * ft * (double)gp[k];
+            if (a.xbar_next && valid) {
+                int u0 = (loc == 0) ? ilo : s_bu[loc - 1];
+                int u1 = s_bu[loc];
+                double cs = 0.0;
+                for (int i = u0; i < u1; ++i) cs += (double)a.xbar_next[i];
+                if (jlo < jhi) {   // wrapped targets (a few, usually tile 0's)
+                    int w0 = (loc == 0) ? jlo : count_le(s_c[loc - 1], u, jlo, jhi, n, true);
+                    int w1 = (loc >= last) ? jhi : count_le(s_c[loc], u, jlo, jhi, n, true);
+                    for (int i = w0; i < w1; ++i) cs += (double)a.xbar_next[i];
+                }
+                xb += phi * cs;
+            }
+            xb = valid ? xb : 0.0;
+            xb_out[c] = (float)xb;
+            if (a.tick > 0) {
+                g0 += xb * (1.0 - phi);
+                g1 += xb * ((double)pre[k] - (double)a.mu);
+                g2 += xb * (double)z[k];
+            } else if (a.x0_reparam) {
+                // x_0 = mu + sd(phi, sigma) z_0
+                g0 += xb;
+                g1 += xb * (double)z[k] * a.dsd_dphi;
+                g2 += xb * (double)z[k] * a.dsd_dsig;
+            }
+        }
+        long long off = base + (long long)(v * kBlock + tid) * VEC;
+        store_vec<VEC>(a.xbar_out + off, xb_out);
+    }
+    g0 = wave_sum(g0);
+    g1 = wave_sum(g1);
+    g2 = wave_sum(g2);
+    if (lane == 0) {
+        s_red[0][wid] = g0;
+        s_red[1][wid] = g1;
+        s_red[2][wid] = g2;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double r0 = s_red[0][0], r1 = s_red[1][0], r2 = s_red[2][0];
+        for (int w = 1; w < NW; ++w) {
+            r0 += s_red[0][w];
+            r1 += s_red[1][w];
+            r2 += s_red[2][w];
+        }
+        a.theta_acc[3 * b + 0] += r0;
+        a.theta_acc[3 * b + 1] += r1;
+        a.theta_acc[3 * b + 2] += r2;
     }
 }
 
@@ -634,6 +834,23 @@ hipError_t launch_tick(int K, const PropArgs& p, const FinArgs& f, const ResArgs
         case 4: return launch_tick_k<4>(p, f, r, st, ev);
         case 8: return launch_tick_k<8>(p, f, r, st, ev);
         case 16: return launch_tick_k<16>(p, f, r, st, ev);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int K>
+static hipError_t launch_adj_k(const AdjArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_adjoint<K>, dim3(a.n_tiles), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint(int K, const AdjArgs& a, hipStream_t st) {
+    switch (K) {
+        case 1: return launch_adj_k<1>(a, st);
+        case 2: return launch_adj_k<2>(a, st);
+        case 4: return launch_adj_k<4>(a, st);
+        case 8: return launch_adj_k<8>(a, st);
+        case 16: return launch_adj_k<16>(a, st);
         default: return hipErrorInvalidValue;
     }
 }

@@ -53,6 +53,7 @@ struct PropArgs {
     int n;
     int do_prop;          // 0 on tick 0 (ptx_assembly/bpf_kernels_full.ptx:294)
     int store_l;          // keep log-weights for a possible carry
+    float* x_save;        // optional copy of the pre-propagation states
 };
 
 struct FinArgs {
@@ -65,6 +66,8 @@ struct FinArgs {
     int n_tiles;
     int n;
     float threshold;      // ESS/N threshold; <= 0 resamples every tick
+    float* u_log;         // optional: resampling offset of this tick
+    int* w_log;           // optional: wrap start of this tick
 };
 
 struct ResArgs {
@@ -78,6 +81,28 @@ struct ResArgs {
     const float* y_ptr;
     float y;
     ObsModel obs;
+    int n;
+    int n_tiles;
+};
+
+// One reverse-mode tick of the pathwise score.
+struct AdjArgs {
+    const float* pre;        // pre-propagation states of the tick (x_0 at tick 0)
+    const float* xbar_next;  // adjoints of tick t+1, or null at the last tick
+    float* xbar_out;         // adjoints of tick t
+    const float* m;          // tile maxima of the tick
+    const double* ftil;      // f_b / S of the tick
+    const double* btil;      // tile CDF prefix of the tick
+    double* theta_acc;       // per-tile accumulators d/d(mu, phi, sigma)
+    uint64_t pcg_a, pcg_g;   // draw t (draw 0 at tick 0)
+    float mu, phi, sig;
+    float y;
+    const float* u_ptr;      // resampling offset of the tick (device)
+    const int* w_ptr;        // wrap start of the tick (device)
+    ObsModel obs;
+    double dsd_dphi, dsd_dsig;   // stationary sd derivatives (x_0 reparameterisation)
+    int x0_reparam;
+    int tick;
     int n;
     int n_tiles;
 };

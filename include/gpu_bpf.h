@@ -158,6 +158,21 @@ int gpu_bpf_synchronize(GpuBpfState* state);
 /* Number of ticks processed so far. */
 int gpu_bpf_get_timestep(GpuBpfState* state);
 
+/* Pathwise score of the log-likelihood with the ancestors held fixed
+ * (replaces the reference's autograd path, tests/diff_bpf.py:186-233, and its
+ * one-step kernel bpf_grad_alpha, ptx_assembly/bpf_kernels_full.ptx:1116).
+ * LL = sum_t [logsumexp_i lw_t,i - log N] of the always-resample filter.
+ * score_forward runs n_ticks ticks from a fresh filter (it sets the ESS
+ * threshold to 0), keeps the particle states every checkpoint_every ticks
+ * and returns LL (FP64).  score_backward replays each segment from its
+ * checkpoint and sweeps it in reverse; score[3] = dLL/d(mu, phi, sigma_z).
+ * x0_reparam = 1 also differentiates the stationary initial draw
+ * x_0 = mu + sigma_z / sqrt(1 - phi^2) z_0.  Memory: (T / checkpoint_every
+ * + checkpoint_every + 4) * N floats. */
+int gpu_bpf_score_forward(GpuBpfState* state, const float* y, int n_ticks,
+                          int checkpoint_every, double* loglik);
+int gpu_bpf_score_backward(GpuBpfState* state, int x0_reparam, double* score);
+
 /* Per-kernel timing with HIP events recorded on the state's own stream
  * around each of the tick's three kernels (propagate, finalise, resample).
  * Enable before enqueuing ticks; gpu_bpf_kernel_times synchronises, then

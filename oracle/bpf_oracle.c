@@ -641,3 +641,178 @@ double orc_grid_loglik(const float* y, int n_ticks, double rho, double sigma_z,
     free(g); free(pi); free(pr);
     return total;
 }
+
+/* ========================================================================= */
+/* Score of the log-likelihood (pathwise, ancestors held fixed).              */
+/*                                                                            */
+/* LL = sum_t [ logsumexp_i lw(x_t,i, y_t) - log N ] of the always-resample   */
+/* filter (the quantity tests/diff_bpf.py:218-222 differentiates).  With the  */
+/* ancestors a_t frozen, x_t,i = mu + phi (x_{t-1,a_{t-1,i}} - mu) +          */
+/* sigma z_t,i is smooth in theta = (mu, phi, sigma); x_0 = mu + sd z_0 with  */
+/* sd = sigma / sqrt(max(1 - phi^2, 1e-6)) (reparameterised when x0_reparam). */
+/* The score is carried forward as per-particle tangents D = dx/dtheta:       */
+/*   D_t = (1 - phi + phi D~_mu, (x~ - mu) + phi D~_phi, z + phi D~_sigma),   */
+/*   dLL += sum_i W_t,i lw'(x_t,i) D_t,i,                                      */
+/* D~ = D gathered by the ancestors.  FP64 throughout.                        */
+/* ========================================================================= */
+
+static inline double dlw_dx_f64(double x, double y, double nu_obs) {
+    double eta2 = y * y * exp(-x);
+    if (nu_obs > 0.0) return 0.5 * (nu_obs + 1.0) * eta2 / (nu_obs + eta2) - 0.5;
+    return 0.5 * eta2 - 0.5;
+}
+
+int orc_bpf_score(const OrcParams* p, const float* y, int n_ticks, int x0_reparam,
+                  const double* theta64, const int* anc_in, int* anc_out,
+                  double* ll_out, double* score_out, double* ll32_out) {
+    const int n = p->n_particles;
+    if (n <= 0 || n_ticks <= 0 || p->tile <= 0) return -1;
+    const int tile = p->tile;
+    const int n_tiles = (n + tile - 1) / tile;
+    const int replay = anc_in != NULL;
+    /* FP64 parameters: the FP32 ones, or theta64 (finite differences) */
+    const double mud = theta64 ? theta64[0] : (double)p->mu;
+    const double rhod = theta64 ? theta64[1] : (double)p->rho;
+    const double sigd = theta64 ? theta64[2] : (double)p->sigma_z;
+    const float rho = p->rho, sig = p->sigma_z, mu = p->mu;
+    const float nu_obs = p->nu_obs;
+    const double c_obs_d = nu_obs > 0.0f ? c_obs_f64((double)nu_obs) : 0.0;
+    const float c_obs = (float)c_obs_d;
+    const float nph = (nu_obs + 1.0f) * 0.5f;
+
+    float* x = (float*)malloc(sizeof(float) * (size_t)n);
+    float* xt = (float*)malloc(sizeof(float) * (size_t)n);
+    float* l = (float*)malloc(sizeof(float) * (size_t)n);
+    double* w = (double*)malloc(sizeof(double) * (size_t)n);
+    double* cdf = (double*)malloc(sizeof(double) * (size_t)n);
+    int* anc = (int*)malloc(sizeof(int) * (size_t)n);
+    double* xd = (double*)malloc(sizeof(double) * (size_t)n);
+    double* xdt = (double*)malloc(sizeof(double) * (size_t)n);
+    double* D = (double*)malloc(sizeof(double) * 3 * (size_t)n);
+    double* Dt = (double*)malloc(sizeof(double) * 3 * (size_t)n);
+    double* lw = (double*)malloc(sizeof(double) * (size_t)n);
+    float* tmax = (float*)malloc(sizeof(float) * (size_t)n_tiles);
+    float* tf = (float*)malloc(sizeof(float) * (size_t)n_tiles);
+    if (!x || !xt || !l || !w || !cdf || !anc || !xd || !xdt || !D || !Dt || !lw || !tmax || !tf) {
+        free(x); free(xt); free(l); free(w); free(cdf); free(anc); free(xd); free(xdt);
+        free(D); free(Dt); free(lw); free(tmax); free(tf);
+        return -1;
+    }
+
+    /* initial state */
+    {
+        float sd = sqrtf((sig * sig) / fmaxf(1.0f - rho * rho, 1e-6f));
+        double om = 1.0 - rhod * rhod;
+        double sdd, dsd_dphi, dsd_dsig;
+        if (om > 1e-6) {
+            sdd = sigd / sqrt(om);
+            dsd_dsig = 1.0 / sqrt(om);
+            dsd_dphi = sigd * rhod / (om * sqrt(om));
+        } else {
+            sdd = sigd / sqrt(1e-6);
+            dsd_dsig = 1.0 / sqrt(1e-6);
+            dsd_dphi = 0.0;
+        }
+        PcgJump j0 = pcg_jump(p->seed, 0);
+        for (int i = 0; i < n; i++) {
+            float z = normal_j(j0, (uint32_t)i);
+            x[i] = fmaf(sd, z, mu);
+            xd[i] = mud + sdd * (double)z;
+            D[3 * i + 0] = x0_reparam ? 1.0 : 0.0;
+            D[3 * i + 1] = x0_reparam ? (double)z * dsd_dphi : 0.0;
+            D[3 * i + 2] = x0_reparam ? (double)z * dsd_dsig : 0.0;
+        }
+    }
+
+    uint64_t hrng = orc_pcg_host_seed(p->seed);
+    double ll = 0.0, ll32 = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    for (int t = 0; t < n_ticks; t++) {
+        const float yt = y[t];
+        const float ysq = yt * yt;
+        const PcgJump jt = pcg_jump(p->seed, (uint32_t)t);
+        /* propagate (FP32 restatement for the genealogy, FP64 + tangents) */
+        for (int i = 0; i < n; i++) {
+            if (t > 0) {
+                float z = normal_j(jt, (uint32_t)i);
+                if (!replay) {
+                    float d = x[i] - mu;
+                    d = rho * d;
+                    float xi = mu + d;
+                    x[i] = fmaf(sig, z, xi);
+                }
+                double pre = xd[i];
+                double* Di = &D[3 * i];
+                xd[i] = mud + rhod * (pre - mud) + sigd * (double)z;
+                Di[0] = (1.0 - rhod) + rhod * Di[0];
+                Di[1] = (pre - mud) + rhod * Di[1];
+                Di[2] = (double)z + rhod * Di[2];
+            }
+        }
+        /* FP64 weights, log-likelihood and score increment */
+        double M = -INFINITY;
+        for (int i = 0; i < n; i++) {
+            lw[i] = lw_f64(xd[i], (double)yt, (double)nu_obs, c_obs_d);
+            if (lw[i] > M) M = lw[i];
+        }
+        double S = 0.0;
+        for (int i = 0; i < n; i++) S += exp(lw[i] - M);
+        ll += M + log(S / (double)n);
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int i = 0; i < n; i++) {
+            double Wg = exp(lw[i] - M) / S * dlw_dx_f64(xd[i], (double)yt, (double)nu_obs);
+            a0 += Wg * D[3 * i + 0];
+            a1 += Wg * D[3 * i + 1];
+            a2 += Wg * D[3 * i + 2];
+        }
+        g0 += a0; g1 += a1; g2 += a2;
+        /* ancestors: replayed, or from the FP32 restatement (as the GPU) */
+        const int* at;
+        if (replay) {
+            at = anc_in + (size_t)t * (size_t)n;
+        } else {
+            for (int i = 0; i < n; i++) l[i] = lw_f32(x[i], ysq, nu_obs, c_obs, nph);
+            float gm = -INFINITY;
+            for (int b = 0; b < n_tiles; b++) {
+                float m = -INFINITY;
+                int e = (b + 1) * tile < n ? (b + 1) * tile : n;
+                for (int i = b * tile; i < e; i++) m = fmaxf(m, l[i]);
+                tmax[b] = m;
+                gm = fmaxf(gm, m);
+            }
+            for (int b = 0; b < n_tiles; b++)
+                tf[b] = (tmax[b] == -INFINITY) ? 0.0f : orc_expf(tmax[b] - gm);
+            double S32 = 0.0;
+            for (int i = 0; i < n; i++) {
+                int b = i / tile;
+                float e = (tmax[b] == -INFINITY) ? 0.0f : orc_expf(l[i] - tmax[b]);
+                w[i] = (double)e * (double)tf[b];
+                S32 += w[i];
+            }
+            ll32 += (double)gm + log(S32 / (double)n);
+            uint32_t hr = orc_pcg_host_next(&hrng);
+            float u = ((float)(hr >> 9) * (1.0f / 8388608.0f)) / (float)n;
+            double c = 0.0;
+            for (int i = 0; i < n; i++) { c += w[i]; cdf[i] = c / S32; }
+            orc_resample_ancestors(cdf, n, u, anc);
+            for (int i = 0; i < n; i++) xt[i] = x[anc[i]];
+            float* sw = x; x = xt; xt = sw;
+            at = anc;
+        }
+        if (anc_out) memcpy(anc_out + (size_t)t * (size_t)n, at, sizeof(int) * (size_t)n);
+        for (int i = 0; i < n; i++) {
+            int a = at[i];
+            xdt[i] = xd[a];
+            Dt[3 * i + 0] = D[3 * a + 0];
+            Dt[3 * i + 1] = D[3 * a + 1];
+            Dt[3 * i + 2] = D[3 * a + 2];
+        }
+        double* s1 = xd; xd = xdt; xdt = s1;
+        double* s2 = D; D = Dt; Dt = s2;
+    }
+    if (ll_out) *ll_out = ll;
+    if (ll32_out) *ll32_out = ll32;
+    if (score_out) { score_out[0] = g0; score_out[1] = g1; score_out[2] = g2; }
+    free(x); free(xt); free(l); free(w); free(cdf); free(anc); free(xd); free(xdt);
+    free(D); free(Dt); free(lw); free(tmax); free(tf);
+    return 0;
+}

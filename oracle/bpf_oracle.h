@@ -107,6 +107,17 @@ int orc_bpf_run(const OrcParams* p, int mode, const float* y, const float* mu_t,
  * targets u + i/n (wrapped at 1) clamped to n-1. */
 void orc_resample_ancestors(const double* cdf, int n, float u, int* anc);
 
+/* Score d LL / d(mu, phi, sigma) of the always-resample filter with the
+ * ancestors held fixed (FP64 forward-mode tangents).  Ancestors come from the
+ * FP32 restatement (as on the GPU) unless anc_in (T*N ints) replays given
+ * ones; anc_out (T*N, nullable) records them.  theta64 (nullable) overrides
+ * the FP64 parameters (finite differences under common random numbers).
+ * x0_reparam: 1 = differentiate through x_0 = mu + sd(phi, sigma) z_0.
+ * ll_out: FP64 LL; ll32_out: LL of the FP32 restatement. */
+int orc_bpf_score(const OrcParams* p, const float* y, int n_ticks, int x0_reparam,
+                  const double* theta64, const int* anc_in, int* anc_out,
+                  double* ll_out, double* score_out, double* ll32_out);
+
 /* Plant a mistake in the oracle (tests only): 0 none, 1 drop the -x/2 term
  * of the log-weight, 2 negate phi in the transition, 3 upper_bound instead
  * of lower_bound in resampling, 4 drop the weight carry.  Process-global. */

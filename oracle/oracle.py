@@ -98,6 +98,8 @@ def lib():
         L.orc_bpf_run.argtypes = [ctypes.POINTER(OrcParams), i32, P, P, i32, P, P, P, P, i32]
         L.orc_resample_ancestors.restype = None
         L.orc_resample_ancestors.argtypes = [P, i32, f32, P]
+        L.orc_bpf_score.restype = i32
+        L.orc_bpf_score.argtypes = [ctypes.POINTER(OrcParams), P, i32, i32, P, P, P, P, P, P]
         L.orc_set_fault.restype = None
         L.orc_set_fault.argtypes = [i32]
         L.orc_grid_loglik.restype = f64
@@ -184,6 +186,26 @@ def run_filter(y, n_particles, rho=0.97, sigma_z=0.15, mu=-1.0, nu_obs=0.0,
     if cnt < 0:
         raise ValueError("oracle rejected the arguments")
     return out, cnt, (x if x is not None else x64), anc
+
+
+def score(y, n_particles, rho=0.97, sigma_z=0.15, mu=-1.0, nu_obs=0.0, seed=42, tile=256,
+          x0_reparam=True, theta64=None, anc_in=None, want_anc=False):
+    """Pathwise score of the always-resample filter with frozen ancestors.
+    Returns (ll64, score[3], ll32, anc[T,N] or None)."""
+    y = np.ascontiguousarray(y, np.float32)
+    T = len(y)
+    p = OrcParams(n_particles, rho, sigma_z, mu, nu_obs, 0.0, seed, tile)
+    th = None if theta64 is None else np.ascontiguousarray(theta64, np.float64)
+    ai = None if anc_in is None else np.ascontiguousarray(anc_in, np.int32)
+    ao = np.empty((T, n_particles), np.int32) if want_anc else None
+    ll = ctypes.c_double()
+    ll32 = ctypes.c_double()
+    sc = np.zeros(3, np.float64)
+    rc = lib().orc_bpf_score(ctypes.byref(p), _ptr(y), T, 1 if x0_reparam else 0, _ptr(th),
+                             _ptr(ai), _ptr(ao), ctypes.byref(ll), _ptr(sc), ctypes.byref(ll32))
+    if rc < 0:
+        raise ValueError("oracle rejected the arguments")
+    return ll.value, sc, ll32.value, ao
 
 
 def resample_ancestors(cdf, u):
