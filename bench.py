@@ -118,10 +118,13 @@ def main():
     f.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    tick_ms = f.kernel_tick_times(K)
     ms, cnt = f.kernel_times()
     st = np.zeros(K, gpu_bpf.STATS_DTYPE)
     ring = min(K, 1024)
     rc = L.gpu_bpf_copy_stats(f.h, st[K - ring:].ctypes.data, ring)
+    rs_mask = st["resampled"][K - ring:] == 1
+    tm = tick_ms[K - ring:]
     rcount = f.resample_count()
     f.close()
     if rc < 0:
@@ -170,6 +173,8 @@ def main():
                 "propagate": ms[0] / max(cnt[0], 1),
                 "finalize": ms[1] / max(cnt[1], 1),
                 "resample": ms[2] / max(cnt[2], 1),
+                "resample_on_resample_ticks": float(tm[rs_mask, 2].mean()) if rs_mask.any() else None,
+                "resample_on_other_ticks": float(tm[~rs_mask, 2].mean()) if (~rs_mask).any() else None,
             },
         },
         "roofline": {

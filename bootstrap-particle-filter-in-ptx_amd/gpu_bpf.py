@@ -95,6 +95,9 @@ def load(path=LIB_PATH):
         "gpu_bpf_synchronize": (I, [P]),
         "gpu_bpf_get_timestep": (I, [P]),
         "gpu_bpf_enable_kernel_timing": (I, [P, I]),
+        "gpu_bpf_score_forward": (I, [P, P, I, I, ctypes.POINTER(D)]),
+        "gpu_bpf_kernel_tick_times": (I, [P, P, I]),
+        "gpu_bpf_score_backward": (I, [P, I, P]),
         "gpu_bpf_kernel_times": (I, [P, P, P, I]),
     }
     for name, (res, args) in sig.items():
@@ -228,8 +231,28 @@ class GpuBpf:
         self._y_keep = y
         self._check(self.L.gpu_bpf_run(self.h, _ptr(y), len(y), None, None))
 
+    def score_forward(self, y, checkpoint_every=64):
+        """Checkpointed forward pass of the pathwise score; returns LL (FP64)."""
+        y = np.ascontiguousarray(y, np.float32)
+        ll = ctypes.c_double()
+        self._check(self.L.gpu_bpf_score_forward(self.h, _ptr(y), len(y), int(checkpoint_every),
+                                                 ctypes.byref(ll)))
+        return ll.value
+
+    def score_backward(self, x0_reparam=True):
+        """Reverse sweep; returns dLL/d(mu, phi, sigma_z)."""
+        g = np.zeros(3, np.float64)
+        self._check(self.L.gpu_bpf_score_backward(self.h, 1 if x0_reparam else 0, _ptr(g)))
+        return g
+
     def enable_kernel_timing(self, on=True):
         self._check(self.L.gpu_bpf_enable_kernel_timing(self.h, 1 if on else 0))
+
+    def kernel_tick_times(self, n_ticks):
+        """[n, 3] per-tick milliseconds (propagate, finalise, resample)."""
+        ms = np.zeros((n_ticks, 3), np.float32)
+        got = self._check(self.L.gpu_bpf_kernel_tick_times(self.h, _ptr(ms), n_ticks))
+        return ms[n_ticks - got:] if got < n_ticks else ms
 
     def kernel_times(self, reset=True):
         """(ms[3], launches[3]) for propagate, finalise, resample."""

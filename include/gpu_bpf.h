@@ -180,6 +180,9 @@ int gpu_bpf_score_backward(GpuBpfState* state, int x0_reparam, double* score);
  * last reset (arrays of 3: propagate, finalise, resample). */
 int gpu_bpf_enable_kernel_timing(GpuBpfState* state, int enable);
 int gpu_bpf_kernel_times(GpuBpfState* state, double* ms, int* launches, int reset);
+/* The per-tick milliseconds (3 per tick) of the last n_ticks timed ticks;
+ * returns how many ticks were copied. */
+int gpu_bpf_kernel_tick_times(GpuBpfState* state, float* ms, int n_ticks);
 
 #ifdef __cplusplus
 }
