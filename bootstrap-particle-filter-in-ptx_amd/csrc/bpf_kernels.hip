@@ -62,6 +62,34 @@ __device__ inline double wave_incl_max(double v, int lane) {
     return v;
 }
 
+// LDS index padded by one element per 16, so lane-contiguous runs of 16
+// (stride 16 elements between lanes) fall into distinct banks.
+__device__ inline int pad16(int j) { return j + (j >> 4); }
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// Streaming (non-temporal) variants for particle arrays far larger than the
+// caches: each element is read once and written once per tick.
+template <int VEC>
+__device__ inline void load_vec_nt(const float* p, float* v) {
+    if constexpr (VEC == 4) {
+        v4f t = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(p));
+        v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+    } else {
+        for (int c = 0; c < VEC; ++c) v[c] = __builtin_nontemporal_load(p + c);
+    }
+}
+
+template <int VEC>
+__device__ inline void store_vec_nt(float* p, const float* v) {
+    if constexpr (VEC == 4) {
+        v4f t = {v[0], v[1], v[2], v[3]};
+        __builtin_nontemporal_store(t, reinterpret_cast<v4f*>(p));
+    } else {
+        for (int c = 0; c < VEC; ++c) __builtin_nontemporal_store(v[c], p + c);
+    }
+}
+
 template <int VEC>
 __device__ inline void load_vec(const float* p, float* v) {
     if constexpr (VEC == 4) {
@@ -170,6 +198,42 @@ __device__ inline void lane_normals(uint64_t pcg_a, uint64_t pcg_g, long long ba
 }
 
 // ---------------------------------------------------------------------------
+// Group fold (one wave): the group's tile partials, written by the previous
+// kernel, rescaled to the group maximum and summed in tile order.
+// ---------------------------------------------------------------------------
+
+__device__ inline void group_fold(const Partials& p, int g, int lane) {
+    const int t = g * kGroup + lane;
+    const bool in = t < p.n_tiles && lane < kGroup;
+    constexpr float NEG_INF = -__builtin_inff();
+    float m = in ? p.m[t] : NEG_INF;
+    double s = in ? p.s[t] : 0.0;
+    double sx = in ? p.sx[t] : 0.0;
+    double sxx = in ? p.sxx[t] : 0.0;
+    double sq = in ? p.sq[t] : 0.0;
+    const float gmax = wave_max(m);
+    const double f = (m == NEG_INF) ? 0.0 : (double)expf_cw(m - gmax);
+    double vs = f * s, vx = f * sx, vxx = f * sxx, vq = (f * f) * sq;
+    // the tile's scale and in-group prefix for the resample pass
+    const double pin = wave_incl_max(wave_incl_scan(vs, lane), lane);
+    if (in) {
+        p.tf[t] = f;
+        p.tp[t] = pin;
+    }
+    vs = wave_sum(vs);
+    vx = wave_sum(vx);
+    vxx = wave_sum(vxx);
+    vq = wave_sum(vq);
+    if (lane == 0) {
+        p.gm[g] = gmax;
+        p.gs[g] = vs;
+        p.gsx[g] = vx;
+        p.gsxx[g] = vxx;
+        p.gsq[g] = vq;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Propagate + weight + tile partials.
 // Tile b holds particles [b*TILE, (b+1)*TILE); lane `tid` owns VEC-wide
 // groups at (v*kBlock + tid)*VEC, so every load and store instruction of a
@@ -185,6 +249,7 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
     constexpr int NV = K / VEC;
     constexpr int TILE = kBlock * K;
     constexpr int NW = kBlock / kWave;
+    constexpr bool NT = false;     // non-temporal streams measured slower on gfx950 here
     __shared__ float s_q[TILE];
     __shared__ int s_cnt[NW];
     __shared__ float s_m[NW];
@@ -207,13 +272,15 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         long long off = base + (long long)(v * kBlock + tid) * VEC;
-        load_vec<VEC>(xb + off, &x[v * VEC]);
+        if constexpr (NT) load_vec_nt<VEC>(xb + off, &x[v * VEC]);
+        else load_vec<VEC>(xb + off, &x[v * VEC]);
     }
     if (carry) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             long long off = base + (long long)(v * kBlock + tid) * VEC;
-            load_vec<VEC>(a.lw + off, &l[v * VEC]);
+            if constexpr (NT) load_vec_nt<VEC>(a.lw + off, &l[v * VEC]);
+            else load_vec<VEC>(a.lw + off, &l[v * VEC]);
         }
     }
 
@@ -239,7 +306,8 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             long long off = base + (long long)(v * kBlock + tid) * VEC;
-            store_vec<VEC>(xb + off, &x[v * VEC]);
+            if constexpr (NT) store_vec_nt<VEC>(xb + off, &x[v * VEC]);
+            else store_vec<VEC>(xb + off, &x[v * VEC]);
         }
     }
 
@@ -262,7 +330,8 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             long long off = base + (long long)(v * kBlock + tid) * VEC;
-            store_vec<VEC>(a.lw + off, &l[v * VEC]);
+            if constexpr (NT) store_vec_nt<VEC>(a.lw + off, &l[v * VEC]);
+            else store_vec<VEC>(a.lw + off, &l[v * VEC]);
         }
     }
 
@@ -323,7 +392,12 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
 // Finalise (one block of 1024 threads).
 // ---------------------------------------------------------------------------
 
-constexpr int kFinBlock = 1024;
+// Finalise: one block per reduction group folds the group's tiles; the block
+// that arrives last (arrival counter; each block publishes its group record
+// with an agent-scope fence first -- cheap here, since the propagate kernel's
+// boundary already wrote its data back) reduces the group records, takes the
+// tick's decisions and, on resampling ticks, builds the FP64 group prefix.
+constexpr int kFinBlock = 256;
 
 __device__ inline float host_pcg_float(unsigned long long* st) {
     unsigned long long old = *st;
@@ -339,12 +413,26 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
     __shared__ int s_flag;
     __shared__ double s_S;
     __shared__ float s_M;
+    __shared__ int s_last;
 
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
-    const int G = a.n_tiles;
+    const int Gg = a.part.n_groups;
+    constexpr float NEG_INF = -__builtin_inff();
 
-    float m = -__builtin_inff();
-    for (int b = tid; b < G; b += kFinBlock) m = fmaxf(m, a.part.m[b]);
+    if (tid < kWave) group_fold(a.part, blockIdx.x, lane);
+    if (tid == 0) {
+        __threadfence();
+        const int old = atomicAdd(a.part.gcnt, 1);
+        s_last = (old == (int)gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    if (tid == 0) *a.part.gcnt = 0;
+
+    float m = NEG_INF;
+    for (int g = tid; g < Gg; g += kFinBlock)
+        m = fmaxf(m, __hip_atomic_load(&a.part.gm[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     m = wave_max(m);
     if (lane == 0) s_f[wid] = m;
     __syncthreads();
@@ -357,14 +445,13 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
     const float M = s_M;
 
     double S = 0.0, SX = 0.0, SXX = 0.0, Q = 0.0;
-    for (int b = tid; b < G; b += kFinBlock) {
-        float mb = a.part.m[b];
-        if (mb == -__builtin_inff()) continue;
-        double f = (double)expf_cw(mb - M);
-        S += f * a.part.s[b];
-        SX += f * a.part.sx[b];
-        SXX += f * a.part.sxx[b];
-        Q += (f * f) * a.part.sq[b];
+    for (int g = tid; g < Gg; g += kFinBlock) {
+        const float gm = __hip_atomic_load(&a.part.gm[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const double f = (gm == NEG_INF) ? 0.0 : (double)expf_cw(gm - M);
+        S += f * __hip_atomic_load(&a.part.gs[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        SX += f * __hip_atomic_load(&a.part.gsx[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        SXX += f * __hip_atomic_load(&a.part.gsxx[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        Q += (f * f) * __hip_atomic_load(&a.part.gsq[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     S = wave_sum(S);
     SX = wave_sum(SX);
@@ -433,6 +520,8 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
             c->s_prev = S;
         }
         c->m_prev = M;
+        c->M = M;
+        c->S = S;
         c->resample = resample;
         c->last = r;
         a.ring[c->tick % a.ring_cap] = r;
@@ -443,34 +532,33 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
     __syncthreads();
     if (!s_flag) return;
 
-    // FP64 exclusive prefix of the tile weights v_b = f_b * s_b, normalised
-    // by S, clamped to [0, 1] and made non-decreasing by a prefix maximum.
+    // FP64 exclusive prefix of the group weights v_g = f_g * gs_g, normalised
+    // by S, clamped to [0, 1], made non-decreasing by a prefix maximum.
+    // Thread tid owns the contiguous groups [tid*C, tid*C + C).
     S = s_S;
-    const int C = (G + kFinBlock - 1) / kFinBlock;
-    const int b0 = tid * C;
-    const int b1 = min(b0 + C, G);
+    const int C = (Gg + kFinBlock - 1) / kFinBlock;
+    const int g0 = tid * C;
+    const int g1 = min(g0 + C, Gg);
     double loc = 0.0;
-    for (int b = b0; b < b1; ++b) {
-        float mb = a.part.m[b];
-        double f = (mb == -__builtin_inff()) ? 0.0 : (double)expf_cw(mb - M);
-        loc += f * a.part.s[b];
+    for (int g = g0; g < g1; ++g) {
+        const float gm = __hip_atomic_load(&a.part.gm[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const double gs = __hip_atomic_load(&a.part.gs[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        loc += ((gm == NEG_INF) ? 0.0 : (double)expf_cw(gm - M)) * gs;
     }
     double incl = wave_incl_scan(loc, lane);
     if (lane == kWave - 1) s_scan[wid] = incl;
     __syncthreads();
-    double wpre = 0.0;
-    for (int w = 0; w < wid; ++w) wpre += s_scan[w];
-    double B = wpre + (incl - loc);
-    B = (tid == 0) ? 0.0 : B;
+    double B = __shfl_up(incl, 1, kWave);
+    B = (lane == 0) ? 0.0 : B;
+    for (int w = 0; w < wid; ++w) B += s_scan[w];
     double hi_v = 0.0;
-    for (int b = b0; b < b1; ++b) {
-        float mb = a.part.m[b];
-        double f = (mb == -__builtin_inff()) ? 0.0 : (double)expf_cw(mb - M);
-        double bt = fmin(fmax(B / S, 0.0), 1.0);
-        a.btil[b] = bt;
-        a.ftil[b] = f / S;
+    for (int g = g0; g < g1; ++g) {
+        const float gm = __hip_atomic_load(&a.part.gm[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const double gs = __hip_atomic_load(&a.part.gs[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const double bt = fmin(fmax(B / S, 0.0), 1.0);
+        a.gptil[g] = bt;
         hi_v = bt;
-        B += f * a.part.s[b];
+        B += ((gm == NEG_INF) ? 0.0 : (double)expf_cw(gm - M)) * gs;
     }
     __syncthreads();
     double pm = wave_incl_max(hi_v, lane);
@@ -480,13 +568,12 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
     __syncthreads();
     double pre = wl;
     for (int w = 0; w < wid; ++w) pre = fmax(pre, s_scan[w]);
-    for (int b = b0; b < b1; ++b) {
-        double bt = a.btil[b];
-        if (bt < pre) a.btil[b] = pre;
+    for (int g = g0; g < g1; ++g) {
+        if (a.gptil[g] < pre) a.gptil[g] = pre;
     }
     if (tid == 0) {
-        a.btil[0] = 0.0;
-        a.btil[G] = 1.0;
+        a.gptil[0] = 0.0;
+        a.gptil[Gg] = 1.0;
     }
 }
 
@@ -510,20 +597,41 @@ __device__ inline int count_le(double v, float u, int lo, int hi, int n, bool wr
     return lo;
 }
 
+// Same count for unwrapped targets, from the estimate i ~ (v - u) N and an
+// exact correction with the float target rule; falls back to the binary
+// search if the estimate is off by more than a few (it is off by at most
+// about N * 2^-24 positions, so this does not happen below ~2^28 particles).
+__device__ inline int count_le_fast(double v, float u, int lo, int hi, int n) {
+    if (lo >= hi) return lo;
+    double est = floor((v - (double)u) * (double)n) + 1.0;
+    est = fmin(fmax(est, (double)lo), (double)hi);
+    int g = (int)est;
+    int steps = 0;
+    while (g > lo && (double)target_unwrapped(u, g - 1, n) > v) {
+        --g;
+        if (++steps > 8) return count_le(v, u, lo, hi, n, false);
+    }
+    while (g < hi && (double)target_unwrapped(u, g, n) <= v) {
+        ++g;
+        if (++steps > 16) return count_le(v, u, lo, hi, n, false);
+    }
+    return g;
+}
+
 // Builds the tile's CDF in s_c (TILE doubles) from the lane's weights e[K]
 // (coalesced layout) and returns the tile's output ranges in s_rng:
 // [0,1) unwrapped targets, [2,3) wrapped targets.  Returns the local index of
 // the tile's last valid particle.  s_c doubles as the staging area of e.
 // Contains block barriers; every thread of the block must call it.
-template <int K>
+template <int BLOCK, int K>
 __device__ inline int tile_cdf(const float (&e)[K], int b, int n, int n_tiles, double ft,
                                double Bb, double Bb1, float u, int W, double* s_c,
                                double* s_w, int* s_rng) {
     constexpr int VEC = K < 4 ? K : 4;
     constexpr int NV = K / VEC;
-    constexpr int TILE = kBlock * K;
+    constexpr int TILE = BLOCK * K;
     constexpr int PAD = TILE + TILE / 16 + 1;
-    static_assert(PAD * sizeof(float) <= TILE * sizeof(double), "e staging must fit in s_c");
+    static_assert(PAD * sizeof(float) <= (TILE + TILE / 16) * sizeof(double), "e staging must fit in s_c");
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const long long base = (long long)b * TILE;
     float* s_e = reinterpret_cast<float*>(s_c);
@@ -532,7 +640,7 @@ __device__ inline int tile_cdf(const float (&e)[K], int b, int n, int n_tiles, d
     for (int v = 0; v < NV; ++v) {
 #pragma unroll
         for (int c = 0; c < VEC; ++c) {
-            const int loc = (v * kBlock + tid) * VEC + c;
+            const int loc = (v * BLOCK + tid) * VEC + c;
             s_e[loc + (loc >> 4)] = e[v * VEC + c];
         }
     }
@@ -577,12 +685,12 @@ __device__ inline int tile_cdf(const float (&e)[K], int b, int n, int n_tiles, d
         double c = fmax(cv[q], prev);
         c = (j == last) ? Bb1 : c;
         c = (j > last) ? __builtin_inf() : c;
-        s_c[j] = c;
+        s_c[pad16(j)] = c;
     }
     if (tid < 4) {
         int r;
-        if (tid == 0)      r = (b == 0) ? 0 : count_le(Bb, u, 0, W, n, false);
-        else if (tid == 1) r = (b == n_tiles - 1) ? W : count_le(Bb1, u, 0, W, n, false);
+        if (tid == 0)      r = (b == 0) ? 0 : count_le_fast(Bb, u, 0, W, n);
+        else if (tid == 1) r = (b == n_tiles - 1) ? W : count_le_fast(Bb1, u, 0, W, n);
         else if (tid == 2) r = (b == 0) ? W : count_le(Bb, u, W, n, n, true);
         else               r = (b == n_tiles - 1) ? n : count_le(Bb1, u, W, n, n, true);
         s_rng[tid] = r;
@@ -591,38 +699,103 @@ __device__ inline int tile_cdf(const float (&e)[K], int b, int n, int n_tiles, d
     return last;
 }
 
+// Weight scale and CDF interval of tile b, from the finalise pass's group
+// prefix and the tile's stored scale and in-group prefix: tile b owns the
+// interval (b0, b1] of the normalised CDF and its b1 is, bit for bit, tile
+// b+1's b0 (same operations on the same stored values).
+struct TileScale {
+    double ft;   // normalised weight of particle j = e_j * ft
+    double b0, b1;
+};
+
+__device__ inline TileScale tile_scale(const Partials& p, const double* gptil, float M, double S,
+                                       int b) {
+    constexpr float NEG_INF = -__builtin_inff();
+    const int g = b / kGroup;
+    const int l = b - g * kGroup;
+    const float gm = p.gm[g];
+    const double fg = (gm == NEG_INF) ? 0.0 : (double)expf_cw(gm - M);
+    const double rS = 1.0 / S;
+    const double g0 = gptil[g], g1 = gptil[g + 1];
+    const double pex = (l == 0) ? 0.0 : p.tp[b - 1];
+    const double pin = p.tp[b];
+    TileScale ts;
+    ts.ft = (p.tf[b] * fg) * rS;
+    ts.b0 = fmin(fma(fg * pex, rS, g0), g1);
+    ts.b1 = (l == kGroup - 1 || b == p.n_tiles - 1) ? g1 : fmin(fma(fg * pin, rS, g0), g1);
+    return ts;
+}
+
+// lower_bound over the padded in-tile CDF c[pad16(0..last)].
 __device__ inline int lds_lower_bound(const double* c, int last, double t) {
     int lo = 0, hi = last;
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
-        if (c[mid] < t) lo = mid + 1; else hi = mid;
+        if (c[pad16(mid)] < t) lo = mid + 1; else hi = mid;
     }
     return lo;
 }
 
-template <int K>
-__global__ __launch_bounds__(kBlock) void k_resample(ResArgs a) {
+__device__ inline int wave_incl_max_int(int v, int lane) {
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        int o = __shfl_up(v, off, kWave);
+        v = (lane >= off) ? max(v, o) : v;
+    }
+    return v;
+}
+
+// Children runs longer than this are written by the whole block, so that a
+// heavy parent (degenerate weights) cannot hold one lane for the tile; up to
+// kLongRuns of them per tile (further ones are written by their lanes).
+constexpr int kLaneRunCap = 48;
+constexpr int kLongRuns = 64;
+
+template <int BLOCK, int K>
+__global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
     const Ctl* ctl = a.ctl;
     if (!ctl->resample) return;
     constexpr int VEC = K < 4 ? K : 4;
     constexpr int NV = K / VEC;
-    constexpr int TILE = kBlock * K;
-    constexpr int NW = kBlock / kWave;
-    __shared__ float s_x[TILE];
-    __shared__ double s_c[TILE];
+    constexpr int TILE = BLOCK * K;
+    constexpr int TP = TILE + TILE / 16;   // padded length
+    constexpr int NW = BLOCK / kWave;
+    __shared__ float s_x[TP];
+    __shared__ double s_c[TP];
     __shared__ double s_w[NW];
     __shared__ int s_rng[4];
+    __shared__ int s_edge[NW];
+    __shared__ int s_nq;
+    __shared__ int s_q[kLongRuns];     // lanes whose children run is long
+    __shared__ int s_qa[kLongRuns];
+    __shared__ int s_qb[kLongRuns];
 
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const int b = blockIdx.x;
     const int n = a.n;
     const long long base = (long long)b * TILE;
     const float* __restrict__ xs = ctl->src ? a.x1 : a.x0;
     float* __restrict__ xd = ctl->dst ? a.x1 : a.x0;
+
+    float xv[K], lv[K];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const int loc0 = (v * BLOCK + tid) * VEC;
+        load_vec<VEC>(xs + base + loc0, &xv[v * VEC]);
+        if (a.lw) load_vec<VEC>(a.lw + base + loc0, &lv[v * VEC]);
+    }
     const float u = ctl->u;
     const int W = ctl->wrap_start;
-    const float mb = a.m[b];
+    const float mb = a.part.m[b];
     const bool dead = (mb == -__builtin_inff());
+    const TileScale ts = tile_scale(a.part, a.gptil, ctl->M, ctl->S, b);
+    if (a.m_log && tid == 0) {
+        a.m_log[b] = mb;
+        a.ft_log[b] = ts.ft;
+        a.b_log[b] = ts.b0;
+        if (b == a.part.n_tiles - 1) a.b_log[b + 1] = ts.b1;
+    }
+    if (tid == 0) s_nq = 0;
 
     ObsModel obs = a.obs;
     {
@@ -633,30 +806,74 @@ __global__ __launch_bounds__(kBlock) void k_resample(ResArgs a) {
     float e[K];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-        const int loc0 = (v * kBlock + tid) * VEC;
-        float xv[VEC], lv[VEC];
-        load_vec<VEC>(xs + base + loc0, xv);
-        if (a.lw) load_vec<VEC>(a.lw + base + loc0, lv);
 #pragma unroll
         for (int c = 0; c < VEC; ++c) {
-            const int loc = loc0 + c;
-            float lk = a.lw ? lv[c] : log_weight(xv[c], obs);
+            const int k = v * VEC + c;
+            const int loc = (v * BLOCK + tid) * VEC + c;
+            float lk = a.lw ? lv[k] : log_weight(xv[k], obs);
             bool valid = base + loc < n;
-            e[v * VEC + c] = (dead || !valid) ? 0.0f : expf_cw(lk - mb);
-            s_x[loc] = xv[c];
+            e[k] = (dead || !valid) ? 0.0f : expf_cw(lk - mb);
+            s_x[pad16(loc)] = xv[k];
         }
     }
-    const int last = tile_cdf<K>(e, b, n, a.n_tiles, a.ftil[b], a.btil[b], a.btil[b + 1], u, W,
-                                 s_c, s_w, s_rng);
-
+    const int last = tile_cdf<BLOCK, K>(e, b, n, a.part.n_tiles, ts.ft, ts.b0, ts.b1, u, W, s_c, s_w,
+                                        s_rng);
     const int ilo = s_rng[0], ihi = s_rng[1], jlo = s_rng[2], jhi = s_rng[3];
-    for (int i = ilo + tid; i < ihi; i += kBlock) {
-        double t = (double)target_unwrapped(u, i, n);
-        xd[i] = s_x[lds_lower_bound(s_c, last, t)];
+
+    // Lane tid owns parents j in [tid*K, tid*K+K): their CDF values and states.
+    double cl[K];
+    float xl[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        cl[q] = s_c[pad16(tid * K + q)];
+        xl[q] = s_x[pad16(tid * K + q)];
     }
-    for (int i = jlo + tid; i < jhi; i += kBlock) {
+    // End of the lane's children run among the unwrapped targets: the first i
+    // whose target exceeds its last CDF value; the start is the previous
+    // lane's end (ilo for lane 0).
+    const int lastj = tid * K + K - 1;
+    int ib = (lastj >= last) ? ihi : count_le_fast(cl[K - 1], u, ilo, ihi, n);
+    int ia = __shfl_up(ib, 1, kWave);
+    if (lane == kWave - 1) s_edge[wid] = ib;
+    __syncthreads();
+    ia = (lane == 0) ? ((wid == 0) ? ilo : s_edge[wid - 1]) : ia;
+
+    int slot = kLongRuns;
+    if (ib - ia > kLaneRunCap) slot = atomicAdd(&s_nq, 1);
+    if (slot < kLongRuns) {
+        s_q[slot] = tid;
+        s_qa[slot] = ia;
+        s_qb[slot] = ib;
+    } else {
+        for (int i = ia; i < ib; ++i) {
+            const double t = (double)target_unwrapped(u, i, n);
+            // first owned parent with c >= t: the count of smaller values
+            int q = 0;
+#pragma unroll
+            for (int r = 0; r < K - 1; ++r) q += (cl[r] < t) ? 1 : 0;
+            float v = xl[0];
+#pragma unroll
+            for (int r = 1; r < K; ++r) v = (q == r) ? xl[r] : v;
+            xd[i] = v;
+        }
+    }
+    __syncthreads();
+    // long runs: the whole block writes them, coalesced
+    const int nq = min(s_nq, kLongRuns);
+    for (int k = 0; k < nq; ++k) {
+        const int owner = s_q[k], qa = s_qa[k], qb = s_qb[k];
+        for (int i = qa + tid; i < qb; i += BLOCK) {
+            const double t = (double)target_unwrapped(u, i, n);
+            int q = 0;
+#pragma unroll
+            for (int r = 0; r < K - 1; ++r) q += (s_c[pad16(owner * K + r)] < t) ? 1 : 0;
+            xd[i] = s_x[pad16(owner * K + q)];
+        }
+    }
+    // wrapped targets (a few at most)
+    for (int i = jlo + tid; i < jhi; i += BLOCK) {
         double t = (double)(target_unwrapped(u, i, n) - 1.0f);
-        xd[i] = s_x[lds_lower_bound(s_c, last, t)];
+        xd[i] = s_x[pad16(lds_lower_bound(s_c, last, t))];
     }
 }
 
@@ -680,7 +897,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(AdjArgs a) {
     constexpr int NW = kBlock / kWave;
     __shared__ int s_bu[TILE];     // children run end (unwrapped targets) per parent
     __shared__ int s_cnt[NW];
-    __shared__ double s_c[TILE];
+    __shared__ double s_c[TILE + TILE / 16];
     __shared__ double s_w[NW];
     __shared__ int s_rng[4];
     __shared__ double s_red[3][NW];
@@ -722,7 +939,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(AdjArgs a) {
     const double ft = a.ftil[b];
     const float u = *a.u_ptr;
     const int W = *a.w_ptr;
-    const int last = tile_cdf<K>(e, b, n, a.n_tiles, ft, a.btil[b], a.btil[b + 1], u, W,
+    const int last = tile_cdf<kBlock, K>(e, b, n, a.n_tiles, ft, a.btil[b], a.btil[b + 1], u, W,
                                  s_c, s_w, s_rng);
     const int ilo = s_rng[0], ihi = s_rng[1], jlo = s_rng[2], jhi = s_rng[3];
 
@@ -733,8 +950,8 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(AdjArgs a) {
 #pragma unroll
             for (int c = 0; c < VEC; ++c) {
                 const int loc = (v * kBlock + tid) * VEC + c;
-                const double cj = s_c[loc];
-                s_bu[loc] = (loc >= last) ? ihi : count_le(cj, u, ilo, ihi, n, false);
+                const double cj = s_c[pad16(loc)];
+                s_bu[loc] = (loc >= last) ? ihi : count_le_fast(cj, u, ilo, ihi, n);
             }
         }
     }
@@ -757,8 +974,8 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(AdjArgs a) {
                 double cs = 0.0;
                 for (int i = u0; i < u1; ++i) cs += (double)a.xbar_next[i];
                 if (jlo < jhi) {   // wrapped targets (a few, usually tile 0's)
-                    int w0 = (loc == 0) ? jlo : count_le(s_c[loc - 1], u, jlo, jhi, n, true);
-                    int w1 = (loc >= last) ? jhi : count_le(s_c[loc], u, jlo, jhi, n, true);
+                    int w0 = (loc == 0) ? jlo : count_le(s_c[pad16(loc - 1)], u, jlo, jhi, n, true);
+                    int w1 = (loc >= last) ? jhi : count_le(s_c[pad16(loc)], u, jlo, jhi, n, true);
                     for (int i = w0; i < w1; ++i) cs += (double)a.xbar_next[i];
                 }
                 xb += phi * cs;
@@ -817,11 +1034,15 @@ template <int K>
 static hipError_t launch_tick_k(const PropArgs& p, const FinArgs& f, const ResArgs& r,
                                 hipStream_t st, hipEvent_t* ev) {
     if (ev) (void)hipEventRecord(ev[0], st);
-    hipLaunchKernelGGL(k_propagate<K>, dim3(f.n_tiles), dim3(kBlock), 0, st, p);
+    hipLaunchKernelGGL(k_propagate<K>, dim3(p.part.n_tiles), dim3(kBlock), 0, st, p);
     if (ev) (void)hipEventRecord(ev[1], st);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, st, f);
+    hipLaunchKernelGGL(k_finalize, dim3(f.part.n_groups), dim3(kFinBlock), 0, st, f);
     if (ev) (void)hipEventRecord(ev[2], st);
-    hipLaunchKernelGGL(k_resample<K>, dim3(f.n_tiles), dim3(kBlock), 0, st, r);
+    // resample runs the same tile with 512 lanes (TILE >= 512): twice the
+    // waves per CU and half the per-lane chain of the 256-lane layout
+    constexpr int TILE = kBlock * K;
+    constexpr int RB = TILE >= 512 ? 512 : TILE;
+    hipLaunchKernelGGL((k_resample<RB, TILE / RB>), dim3(r.part.n_tiles), dim3(RB), 0, st, r);
     if (ev) (void)hipEventRecord(ev[3], st);
     return hipGetLastError();
 }

@@ -8,8 +8,13 @@
 
 namespace bpf {
 
-// Device-resident control block.  Written only by the single-block finalise
-// kernel of each tick and read by the next kernels on the same stream, so
+// Tiles per reduction group: one finalise block folds each group of tile
+// partials into a group partial, and the block that finishes last reduces
+// G / kGroup group records instead of G tile records.
+constexpr int kGroup = 64;
+
+// Device-resident control block.  Written only by the finalise kernel's last
+// block each tick and read by the next kernels on the same stream, so
 // the tick needs no host round trip (the reference syncs twice per tick,
 // gpu_bpf/gpu_bpf_ptx_full.cu:677-685,766).
 struct Ctl {
@@ -22,6 +27,9 @@ struct Ctl {
     int tick;            // ticks finalised so far
     float m_prev;        // max of the log-weights carried into the next tick
     float u;             // resampling offset u / N of the latest resample
+    float M;             // global max log-weight of the latest tick
+    float pad_;
+    double S;            // sum exp(l - M) of the latest tick
     double s_prev;       // sum exp(l - m_prev) of the carried weights
     double off;          // raw log-weight offset (reference log_lik semantics)
     double ll_cum;       // FP64 cumulative log-likelihood
@@ -29,13 +37,24 @@ struct Ctl {
     BpfTickStats last;
 };
 
-// Per-tile partial results of the propagate pass (structure of arrays).
+// Per-tile partial results of the propagate pass (structure of arrays), and
+// the per-group folds of them.
 struct Partials {
     float* m;      // max log-weight of the tile
     double* s;     // sum e,        e = exp(l - m)
     double* sx;    // sum e x
     double* sxx;   // sum e x^2
     double* sq;    // sum e^2
+    float* gm;     // group max of the tile maxima
+    double* gs;    // sum over the group's tiles of f s, f = exp(m - gm)
+    double* gsx;
+    double* gsxx;
+    double* gsq;   // sum f^2 sq
+    double* tf;    // per tile: f = exp(m - gm) (FP32 value held in FP64)
+    double* tp;    // per tile: inclusive in-group prefix of f s, non-decreasing
+    int* gcnt;     // arrival counter (0 between launches)
+    int n_tiles;
+    int n_groups;
 };
 
 struct PropArgs {
@@ -60,10 +79,8 @@ struct FinArgs {
     Partials part;
     Ctl* ctl;
     BpfTickStats* ring;
-    double* ftil;         // f_b / S
-    double* btil;         // normalised exclusive tile prefix, G + 1 entries
+    double* gptil;        // normalised exclusive group prefix, n_groups + 1 entries
     int ring_cap;
-    int n_tiles;
     int n;
     float threshold;      // ESS/N threshold; <= 0 resamples every tick
     float* u_log;         // optional: resampling offset of this tick
@@ -74,15 +91,16 @@ struct ResArgs {
     float* x0;
     float* x1;
     const float* lw;      // stored log-weights, or null: recompute from x
-    const float* m;       // tile maxima
-    const double* ftil;
-    const double* btil;
+    Partials part;
+    const double* gptil;
     const Ctl* ctl;
     const float* y_ptr;
     float y;
     ObsModel obs;
     int n;
-    int n_tiles;
+    float* m_log;         // optional per-tile copies for the score's replay
+    double* ft_log;
+    double* b_log;        // n_tiles + 1
 };
 
 // One reverse-mode tick of the pathwise score.
@@ -91,8 +109,8 @@ struct AdjArgs {
     const float* xbar_next;  // adjoints of tick t+1, or null at the last tick
     float* xbar_out;         // adjoints of tick t
     const float* m;          // tile maxima of the tick
-    const double* ftil;      // f_b / S of the tick
-    const double* btil;      // tile CDF prefix of the tick
+    const double* ftil;      // tile weight scales of the tick
+    const double* btil;      // tile CDF prefix of the tick (n_tiles + 1)
     double* theta_acc;       // per-tile accumulators d/d(mu, phi, sigma)
     uint64_t pcg_a, pcg_g;   // draw t (draw 0 at tick 0)
     float mu, phi, sig;

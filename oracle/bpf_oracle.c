@@ -353,6 +353,42 @@ static double c_obs_f64(double nu) {
     return lgamma((nu + 1.0) / 2.0) - lgamma(nu / 2.0) - 0.5 * log(nu * 3.14159265358979323846);
 }
 
+/* Weights as the GPU forms them: w_i = e_i * f_tile * f_group with
+ * e_i = exp(l_i - m_tile), f_tile = exp(m_tile - m_group),
+ * f_group = exp(m_group - M) (each an FP32 orc_expf; the two scales multiply
+ * exactly in FP64), tiles of `tile` particles, groups of 64 tiles.
+ * Returns M.  tmax/tf (n_tiles) receive the tile maxima and FP64 scales. */
+#define ORC_GROUP 64
+static float tiled_weights(const float* l, int n, int tile, float* tmax, double* tf, double* w,
+                           int n_threads) {
+    const int n_tiles = (n + tile - 1) / tile;
+    float gm = -INFINITY;
+    for (int b = 0; b < n_tiles; b++) {
+        float m = -INFINITY;
+        int e = (b + 1) * tile < n ? (b + 1) * tile : n;
+        for (int i = b * tile; i < e; i++) m = fmaxf(m, l[i]);
+        tmax[b] = m;
+        gm = fmaxf(gm, m);
+    }
+    for (int g0 = 0; g0 < n_tiles; g0 += ORC_GROUP) {
+        int g1 = g0 + ORC_GROUP < n_tiles ? g0 + ORC_GROUP : n_tiles;
+        float grp = -INFINITY;
+        for (int b = g0; b < g1; b++) grp = fmaxf(grp, tmax[b]);
+        float fg = (grp == -INFINITY) ? 0.0f : orc_expf(grp - gm);
+        for (int b = g0; b < g1; b++) {
+            float fb = (tmax[b] == -INFINITY) ? 0.0f : orc_expf(tmax[b] - grp);
+            tf[b] = (double)fb * (double)fg;
+        }
+    }
+#pragma omp parallel for num_threads(n_threads) schedule(static)
+    for (int i = 0; i < n; i++) {
+        int b = i / tile;
+        float e = (tmax[b] == -INFINITY) ? 0.0f : orc_expf(l[i] - tmax[b]);
+        w[i] = (double)e * tf[b];
+    }
+    return gm;
+}
+
 int orc_bpf_run(const OrcParams* p, int mode, const float* y, const float* mu_t,
                 int n_ticks, OrcTick* out, float* x_out, double* x64_out,
                 int* anc_out, int n_threads) {
@@ -380,10 +416,10 @@ int orc_bpf_run(const OrcParams* p, int mode, const float* y, const float* mu_t,
     float* xt = (float*)malloc(sizeof(float) * (size_t)n);
     double* xdt = (double*)malloc(sizeof(double) * (size_t)n);
     float* tmax = (float*)malloc(sizeof(float) * (size_t)n_tiles);
-    float* tf = (float*)malloc(sizeof(float) * (size_t)n_tiles);
-    if (!x || !l || !xd || !ld || !w || !wd || !cdf || !anc || !xt || !xdt || !tmax || !tf) {
+    double* tfd = (double*)malloc(sizeof(double) * (size_t)n_tiles);
+    if (!x || !l || !xd || !ld || !w || !wd || !cdf || !anc || !xt || !xdt || !tmax || !tfd) {
         free(x); free(l); free(xd); free(ld); free(w); free(wd); free(cdf); free(anc);
-        free(xt); free(xdt); free(tmax); free(tf);
+        free(xt); free(xdt); free(tmax); free(tfd);
         return -1;
     }
 
@@ -435,23 +471,8 @@ int orc_bpf_run(const OrcParams* p, int mode, const float* y, const float* mu_t,
                 float lw = lw_f32(xi, ysq, nu_obs, c_obs, nph);
                 l[i] = (carry && g_fault != 4) ? lw + (l[i] - m_prev) : lw;
             }
-            /* tile maxima (fmaxf: NaN ignored), global max */
-            float gm = -INFINITY;
-            for (int b = 0; b < n_tiles; b++) {
-                float m = -INFINITY;
-                int e = (b + 1) * tile < n ? (b + 1) * tile : n;
-                for (int i = b * tile; i < e; i++) m = fmaxf(m, l[i]);
-                tmax[b] = m;
-                gm = fmaxf(gm, m);
-            }
-            for (int b = 0; b < n_tiles; b++)
-                tf[b] = (tmax[b] == -INFINITY) ? 0.0f : orc_expf(tmax[b] - gm);
-#pragma omp parallel for num_threads(n_threads) schedule(static)
-            for (int i = 0; i < n; i++) {
-                int b = i / tile;
-                float e = (tmax[b] == -INFINITY) ? 0.0f : orc_expf(l[i] - tmax[b]);
-                w[i] = (double)e * (double)tf[b];
-            }
+            /* tile maxima (fmaxf: NaN ignored), group and global scales */
+            float gm = tiled_weights(l, n, tile, tmax, tfd, w, n_threads);
             M = (double)gm;
             m_prev = gm;
         }
@@ -571,7 +592,7 @@ int orc_bpf_run(const OrcParams* p, int mode, const float* y, const float* mu_t,
     if (x_out) memcpy(x_out, x, sizeof(float) * (size_t)n);
     if (x64_out) memcpy(x64_out, xd, sizeof(double) * (size_t)n);
     free(x); free(l); free(xd); free(ld); free(w); free(wd); free(cdf); free(anc);
-    free(xt); free(xdt); free(tmax); free(tf);
+    free(xt); free(xdt); free(tmax); free(tfd);
     return count;
 }
 
@@ -692,10 +713,10 @@ int orc_bpf_score(const OrcParams* p, const float* y, int n_ticks, int x0_repara
     double* Dt = (double*)malloc(sizeof(double) * 3 * (size_t)n);
     double* lw = (double*)malloc(sizeof(double) * (size_t)n);
     float* tmax = (float*)malloc(sizeof(float) * (size_t)n_tiles);
-    float* tf = (float*)malloc(sizeof(float) * (size_t)n_tiles);
-    if (!x || !xt || !l || !w || !cdf || !anc || !xd || !xdt || !D || !Dt || !lw || !tmax || !tf) {
+    double* tfd = (double*)malloc(sizeof(double) * (size_t)n_tiles);
+    if (!x || !xt || !l || !w || !cdf || !anc || !xd || !xdt || !D || !Dt || !lw || !tmax || !tfd) {
         free(x); free(xt); free(l); free(w); free(cdf); free(anc); free(xd); free(xdt);
-        free(D); free(Dt); free(lw); free(tmax); free(tf);
+        free(D); free(Dt); free(lw); free(tmax); free(tfd);
         return -1;
     }
 
@@ -771,23 +792,9 @@ int orc_bpf_score(const OrcParams* p, const float* y, int n_ticks, int x0_repara
             at = anc_in + (size_t)t * (size_t)n;
         } else {
             for (int i = 0; i < n; i++) l[i] = lw_f32(x[i], ysq, nu_obs, c_obs, nph);
-            float gm = -INFINITY;
-            for (int b = 0; b < n_tiles; b++) {
-                float m = -INFINITY;
-                int e = (b + 1) * tile < n ? (b + 1) * tile : n;
-                for (int i = b * tile; i < e; i++) m = fmaxf(m, l[i]);
-                tmax[b] = m;
-                gm = fmaxf(gm, m);
-            }
-            for (int b = 0; b < n_tiles; b++)
-                tf[b] = (tmax[b] == -INFINITY) ? 0.0f : orc_expf(tmax[b] - gm);
+            float gm = tiled_weights(l, n, tile, tmax, tfd, w, 1);
             double S32 = 0.0;
-            for (int i = 0; i < n; i++) {
-                int b = i / tile;
-                float e = (tmax[b] == -INFINITY) ? 0.0f : orc_expf(l[i] - tmax[b]);
-                w[i] = (double)e * (double)tf[b];
-                S32 += w[i];
-            }
+            for (int i = 0; i < n; i++) S32 += w[i];
             ll32 += (double)gm + log(S32 / (double)n);
             uint32_t hr = orc_pcg_host_next(&hrng);
             float u = ((float)(hr >> 9) * (1.0f / 8388608.0f)) / (float)n;
@@ -813,6 +820,6 @@ int orc_bpf_score(const OrcParams* p, const float* y, int n_ticks, int x0_repara
     if (ll32_out) *ll32_out = ll32;
     if (score_out) { score_out[0] = g0; score_out[1] = g1; score_out[2] = g2; }
     free(x); free(xt); free(l); free(w); free(cdf); free(anc); free(xd); free(xdt);
-    free(D); free(Dt); free(lw); free(tmax); free(tf);
+    free(D); free(Dt); free(lw); free(tmax); free(tfd);
     return 0;
 }

@@ -220,3 +220,20 @@ def test_errors_are_reported():
         gpu_bpf.GpuBpf(1000, RHO, SIG, MU, nu_state=5.0)
     with pytest.raises(gpu_bpf.GpuBpfError, match="positive"):
         gpu_bpf.GpuBpf(0, RHO, SIG, MU)
+
+
+@pytest.mark.parametrize("items", [1, 8, 16])
+@pytest.mark.parametrize("spike", [6.0, 12.0, 40.0])
+def test_degenerate_weights(orc, ydata, items, spike):
+    """Observation spikes concentrate the weights on few particles: long
+    offspring runs (block-cooperative path) and many heavy lanes per tile."""
+    _, y = ydata
+    y = y[:40].copy()
+    y[[5, 17, 30]] = spike
+    n = 300_000
+    say("degenerate", items, spike)
+    _, st, x, tile, _ = gpu_run(n, y, 0.0, items=items)
+    ref, _, xo, _ = orc.run_filter(y, n, ess_threshold=0.0, tile=tile, want_x=True, threads=16)
+    assert_stats_close(st, ref)
+    np.testing.assert_array_equal(x, xo)
+    assert st["ess"][5] < n / 20
