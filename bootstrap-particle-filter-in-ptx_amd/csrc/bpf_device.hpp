@@ -27,21 +27,27 @@ __device__ __host__ inline uint32_t pcg_output(uint64_t old) {
 }
 
 // Uniform of ptx:159-164: (r >> 9) * 2^-23 clamped to [1e-6, 0x3F7FFBCE].
-__device__ __host__ inline float pcg_uniform(uint32_t r) {
+// u is never NaN, so one median-of-three equals the max-then-min clamp.
+__device__ inline float pcg_uniform(uint32_t r) {
     float u = (float)(r >> 9) * (1.0f / 8388608.0f);
-    u = fmaxf(u, bits_f32(0x358637BDu));
-    u = fminf(u, bits_f32(0x3F7FFBCEu));
-    return u;
+    return __builtin_amdgcn_fmed3f(u, bits_f32(0x358637BDu), bits_f32(0x3F7FFBCEu));
 }
 
 // exp(x) after S. L. Moshier's Cephes expf: Cody-Waite reduction by ln 2
 // (C1 = 0.693359375, C2 = -2.12194440e-4), degree-5 polynomial, exponent
 // added in the bits.  x < -86 gives 0 (no subnormal results), x > 88 gives
-// +inf, NaN propagates.
+// +inf, NaN propagates.  The two selects at the end decide every x outside
+// [-86, 88], so the reduction runs unclamped: within the range the integer
+// part z = rint(x log2 e) is found by adding and subtracting 1.5 * 2^23
+// (round to nearest even, exact for |z| < 2^22) and read back from the bits
+// of the sum, and 2^z is applied by ldexp, exact for these normal results;
+// outside the range the discarded lanes compute finite or NaN values
+// without any undefined conversion, and a NaN x stays NaN through ldexp.
 __device__ inline float expf_cw(float x) {
-    float xc = fminf(fmaxf(x, -86.0f), 88.0f);
-    float z = __builtin_rintf(xc * 1.44269504088896341f);
-    float r = __builtin_fmaf(z, -0.693359375f, xc);
+    constexpr float kShift = 12582912.0f;        // 1.5 * 2^23
+    float zs = x * 1.44269504088896341f + kShift;
+    float z = zs - kShift;
+    float r = __builtin_fmaf(z, -0.693359375f, x);
     r = __builtin_fmaf(z, 2.12194440e-4f, r);
     float p = 1.9875691500e-4f;
     p = __builtin_fmaf(p, r, 1.3981999507e-3f);
@@ -52,11 +58,32 @@ __device__ inline float expf_cw(float x) {
     float rr = r * r;
     p = __builtin_fmaf(p, rr, r);
     p = p + 1.0f;
-    int n = (int)z;
-    float res = bits_f32(f32_bits(p) + ((uint32_t)n << 23));
+    float res = __builtin_amdgcn_ldexpf(p, (int)(f32_bits(zs) - 0x4B400000u));
     res = (x > 88.0f) ? __builtin_inff() : res;
-    res = (x >= -86.0f || x > 88.0f) ? res : ((x != x) ? x : 0.0f);
+    res = (x < -86.0f) ? 0.0f : res;       // NaN fails both tests and stays NaN
     return res;
+}
+
+// expf_cw for arguments that cannot exceed 0 (a log-weight minus a maximum
+// taken over the same values, or -inf, or NaN): the overflow select of
+// expf_cw can never fire, so the results are those of expf_cw.
+__device__ inline float expf_cw_nonpos(float x) {
+    constexpr float kShift = 12582912.0f;
+    float zs = x * 1.44269504088896341f + kShift;
+    float z = zs - kShift;
+    float r = __builtin_fmaf(z, -0.693359375f, x);
+    r = __builtin_fmaf(z, 2.12194440e-4f, r);
+    float p = 1.9875691500e-4f;
+    p = __builtin_fmaf(p, r, 1.3981999507e-3f);
+    p = __builtin_fmaf(p, r, 8.3334519073e-3f);
+    p = __builtin_fmaf(p, r, 4.1665795894e-2f);
+    p = __builtin_fmaf(p, r, 1.6666665459e-1f);
+    p = __builtin_fmaf(p, r, 5.0000001201e-1f);
+    float rr = r * r;
+    p = __builtin_fmaf(p, rr, r);
+    p = p + 1.0f;
+    float res = __builtin_amdgcn_ldexpf(p, (int)(f32_bits(zs) - 0x4B400000u));
+    return (x < -86.0f) ? 0.0f : res;
 }
 
 // log(x) after Moshier's Cephes logf: x = m 2^e with m in [sqrt(1/2),
@@ -93,6 +120,25 @@ __device__ inline float logf_cw(float x) {
     return r;
 }
 
+// Correctly rounded n / d for operands whose quotient needs no scaling:
+// d in [2^-60, 2^60], |n| in {0} U [2^-60, 2^60], both finite.  This is the
+// reciprocal refinement and two Markstein corrections of the IEEE division
+// sequence the compiler emits for '/', without its operand scaling and its
+// special-value fix-up, which are the identity for such operands.  The
+// central inverse-CDF rational (|n| <= 0.3, d in [0.0026, 1]) qualifies; the
+// device inverse CDF is checked against IEEE division for every uniform the
+// generator can produce.
+__device__ inline float div_rn_normal(float n, float d) {
+    float y = __builtin_amdgcn_rcpf(d);
+    float e = __builtin_fmaf(-d, y, 1.0f);
+    y = __builtin_fmaf(e, y, y);
+    float q = n * y;
+    float rem = __builtin_fmaf(-d, q, n);
+    q = __builtin_fmaf(rem, y, q);
+    rem = __builtin_fmaf(-d, q, n);
+    return __builtin_fmaf(rem, y, q);
+}
+
 // Acklam (2003) inverse normal CDF with the FP32 coefficients of ptx:172-203.
 // Central region |u - 1/2| < 0.47575 (0x3EF39581, ptx:169): rational in
 // r = q^2.  Valid for every q; callers select it only for central lanes.
@@ -109,7 +155,7 @@ __device__ inline float icdf_central(float q) {
     den = __builtin_fmaf(r, den, 66.80131188771972f);
     den = __builtin_fmaf(r, den, -13.28068155288572f);
     den = __builtin_fmaf(r, den, 1.0f);
-    return num / den;
+    return div_rn_normal(num, den);
 }
 
 __device__ inline bool icdf_is_central(float u) {
@@ -156,15 +202,21 @@ struct ObsModel {
     float nph;      // (nu + 1) / 2
 };
 
-__device__ inline float log_weight(float x, const ObsModel& o) {
-    float eta2 = o.ysq * expf_cw(-x);
-    if (o.nu > 0.0f) {
-        float ta = eta2 / o.nu + 1.0f;
-        float lw = __builtin_fmaf(-o.nph, logf_cw(ta), o.c_obs);
-        return __builtin_fmaf(-0.5f, x, lw);
-    }
+__device__ inline float log_weight_gauss(float x, float ysq) {
+    float eta2 = ysq * expf_cw(-x);
     float lw = __builtin_fmaf(-0.5f, eta2, bits_f32(0xBF6B3F8Eu));
     return __builtin_fmaf(-0.5f, x, lw);
+}
+
+__device__ inline float log_weight_t(float x, const ObsModel& o) {
+    float eta2 = o.ysq * expf_cw(-x);
+    float ta = eta2 / o.nu + 1.0f;
+    float lw = __builtin_fmaf(-o.nph, logf_cw(ta), o.c_obs);
+    return __builtin_fmaf(-0.5f, x, lw);
+}
+
+__device__ inline float log_weight(float x, const ObsModel& o) {
+    return o.nu > 0.0f ? log_weight_t(x, o) : log_weight_gauss(x, o.ysq);
 }
 
 // Systematic-resampling target of gpu_bpf/gpu_bpf.cu:300-302 (IEEE

@@ -243,6 +243,57 @@ __device__ inline void group_fold(const Partials& p, int g, int lane) {
 // compacted into an LDS queue and evaluated densely by the whole block.
 // ---------------------------------------------------------------------------
 
+// Log-weights of the lane's K particles (Gaussian or Student-t chosen once
+// per block, so the unrolled items interleave), carry added, padding set to
+// -inf when MASK (the tile's last particles are padding); returns the lane
+// maximum.  Item (v, c) is tile particle (v*kBlock + tid)*VEC + c; rem = live
+// particles of the tile.
+template <int K, bool STUDENT, bool MASK>
+__device__ inline float lane_log_weights(const float (&x)[K], float (&l)[K], const ObsModel& obs,
+                                         float moff, int tid, int rem) {
+    constexpr int VEC = K < 4 ? K : 4;
+    constexpr int NV = K / VEC;
+    float mloc = -__builtin_inff();
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) {
+            const int k = v * VEC + c;
+            const int li = (v * kBlock + tid) * VEC + c;
+            const float lw = STUDENT ? log_weight_t(x[k], obs) : log_weight_gauss(x[k], obs.ysq);
+            const float lk = lw + (l[k] - moff);
+            l[k] = (!MASK || li < rem) ? lk : -__builtin_inff();
+            mloc = fmaxf(mloc, l[k]);
+        }
+    }
+    return mloc;
+}
+
+// FP64 sums e, e x, e x^2, e^2 of the lane's particles, e = exp(l - mb);
+// padding has l = -inf, so e = 0, and its x is replaced by 0 when MASK.
+template <int K, bool MASK>
+__device__ inline void lane_weight_sums(const float (&x)[K], const float (&l)[K], float mb, int tid,
+                                        int rem, double& S, double& SX, double& SXX, double& Q) {
+    constexpr int VEC = K < 4 ? K : 4;
+    constexpr int NV = K / VEC;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) {
+            const int k = v * VEC + c;
+            const int li = (v * kBlock + tid) * VEC + c;
+            const float e = expf_cw_nonpos(l[k] - mb);
+            const double de = (double)e;
+            const double dx = (!MASK || li < rem) ? (double)x[k] : 0.0;
+            const double t = de * dx;
+            S += de;
+            SX += t;
+            SXX = fma(t, dx, SXX);
+            Q = fma(de, de, Q);
+        }
+    }
+}
+
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
     constexpr int VEC = K < 4 ? K : 4;
@@ -312,20 +363,15 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
     }
 
     // log-weights, with the carry of gpu_bpf_ptx_full.cu:601-604 restated
-    // relative to the previous maximum
-    float mloc = -__builtin_inff();
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-#pragma unroll
-        for (int c = 0; c < VEC; ++c) {
-            const int k = v * VEC + c;
-            long long i = base + (long long)(v * kBlock + tid) * VEC + c;
-            float lw = log_weight(x[k], obs);
-            float lk = carry ? lw + (l[k] - mprev) : lw;
-            l[k] = (i < a.n) ? lk : -__builtin_inff();
-            mloc = fmaxf(mloc, l[k]);
-        }
-    }
+    // relative to the previous maximum; without a carry l holds zeros and the
+    // offset is zero, so the same expression adds exact zeros
+    const float moff = carry ? mprev : 0.0f;
+    const int rem = (a.n - base >= TILE) ? TILE : (int)(a.n - base);   // live particles
+    const bool full = rem == TILE;
+    float mloc_l;
+    if (obs.nu > 0.0f) mloc_l = lane_log_weights<K, true, true>(x, l, obs, moff, tid, rem);
+    else if (full) mloc_l = lane_log_weights<K, false, false>(x, l, obs, moff, tid, rem);
+    else mloc_l = lane_log_weights<K, false, true>(x, l, obs, moff, tid, rem);
     if (a.store_l) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -335,7 +381,7 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
         }
     }
 
-    mloc = wave_max(mloc);
+    const float mloc = wave_max(mloc_l);
     if (lane == 0) s_m[wid] = mloc;
     __syncthreads();
     float mb = s_m[0];
@@ -343,22 +389,9 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
     for (int w = 1; w < NW; ++w) mb = fmaxf(mb, s_m[w]);
 
     double S = 0.0, SX = 0.0, SXX = 0.0, Q = 0.0;
-    const bool dead = (mb == -__builtin_inff());
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-#pragma unroll
-        for (int c = 0; c < VEC; ++c) {
-            const int k = v * VEC + c;
-            long long i = base + (long long)(v * kBlock + tid) * VEC + c;
-            float e = dead ? 0.0f : expf_cw(l[k] - mb);
-            double de = (double)e;
-            double dx = (i < a.n) ? (double)x[k] : 0.0;
-            double t = de * dx;
-            S += de;
-            SX += t;
-            SXX = fma(t, dx, SXX);
-            Q = fma(de, de, Q);
-        }
+    if (mb != -__builtin_inff()) {      // a tile without live weight keeps zero sums
+        if (full) lane_weight_sums<K, false>(x, l, mb, tid, rem, S, SX, SXX, Q);
+        else lane_weight_sums<K, true>(x, l, mb, tid, rem, S, SX, SXX, Q);
     }
     S = wave_sum(S);
     SX = wave_sum(SX);
@@ -1021,6 +1054,30 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(AdjArgs a) {
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
+
+// Element-wise evaluation of the per-particle arithmetic on given inputs:
+// fn 0 = inverse normal CDF of a uniform (central rational or tail branch, as
+// the propagate pass selects them), 1 = expf_cw, 2 = expf_cw_nonpos,
+// 3 = logf_cw.
+__global__ __launch_bounds__(kBlock) void k_device_math(int fn, const float* in, float* out, int n) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float v = in[i];
+    float r;
+    switch (fn) {
+        case 0: r = icdf_is_central(v) ? icdf_central(v - 0.5f) : icdf_tail(v); break;
+        case 1: r = expf_cw(v); break;
+        case 2: r = expf_cw_nonpos(v); break;
+        default: r = logf_cw(v); break;
+    }
+    out[i] = r;
+}
+
+hipError_t launch_device_math(int fn, const float* in, float* out, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_device_math, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, fn, in,
+                       out, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_init(const InitArgs& a, hipStream_t st) {
     int grid = (a.n_pad + kBlock - 1) / kBlock;

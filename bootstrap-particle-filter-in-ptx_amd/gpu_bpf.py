@@ -99,6 +99,7 @@ def load(path=LIB_PATH):
         "gpu_bpf_kernel_tick_times": (I, [P, P, I]),
         "gpu_bpf_score_backward": (I, [P, I, P]),
         "gpu_bpf_kernel_times": (I, [P, P, P, I]),
+        "gpu_bpf_eval_device_math": (I, [I, P, P, I]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -267,3 +268,16 @@ def run_rmse(returns, true_h, n_particles, rho, sigma_z, mu, nu_state=0.0, nu_ob
     h = np.ascontiguousarray(true_h, np.float64)
     return load().gpu_bpf_run_rmse(_ptr(r), _ptr(h), len(r), int(n_particles), rho, sigma_z,
                                    mu, nu_state, nu_obs, int(seed))
+
+
+MATH_ICDF, MATH_EXP, MATH_EXP_NONPOS, MATH_LOG = 0, 1, 2, 3
+
+
+def eval_device_math(fn, x):
+    """The device's per-particle arithmetic evaluated element-wise on the GPU
+    (fn: MATH_ICDF, MATH_EXP, MATH_EXP_NONPOS, MATH_LOG)."""
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(x)
+    if load().gpu_bpf_eval_device_math(int(fn), _ptr(x), _ptr(out), x.size) != 0:
+        raise GpuBpfError(last_error())
+    return out

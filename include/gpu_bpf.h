@@ -184,6 +184,13 @@ int gpu_bpf_kernel_times(GpuBpfState* state, double* ms, int* launches, int rese
  * returns how many ticks were copied. */
 int gpu_bpf_kernel_tick_times(GpuBpfState* state, float* ms, int n_ticks);
 
+/* Verification aid (no reference counterpart): evaluates the device's
+ * per-particle arithmetic element-wise on n host inputs into n host outputs.
+ * fn 0 = inverse normal CDF of a uniform (ptx_assembly/bpf_kernels_full.ptx:
+ * 166-205), 1 = exp, 2 = exp for arguments <= 0, 3 = log (the FP32 exp/log
+ * of the weight and tail paths).  Returns 0, or -1 with gpu_bpf_last_error(). */
+int gpu_bpf_eval_device_math(int fn, const float* in, float* out, int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -119,6 +119,14 @@ float orc_expf(float x) {
     return f32_bits(u32_of(p) + ((uint32_t)n << 23));
 }
 
+void orc_math_array(int fn, const float* in, float* out, int n) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        float v = in[i];
+        out[i] = fn == 0 ? orc_icdf_f32(v) : (fn == 3 ? orc_logf(v) : orc_expf(v));
+    }
+}
+
 /* log(x) after Moshier's Cephes logf (frexp to [sqrt(1/2), sqrt(2)), degree-8
  * polynomial, exponent added back in two parts). */
 float orc_logf(float x) {

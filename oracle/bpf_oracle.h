@@ -87,6 +87,8 @@ float  orc_logf(float x);
 float  orc_icdf_f32(float u);
 double orc_icdf_f64(double u);
 float  orc_normal(int seed, uint32_t i, uint32_t k);   /* z of particle i, draw k */
+/* element-wise: fn 0 = orc_icdf_f32, 1 and 2 = orc_expf, 3 = orc_logf */
+void   orc_math_array(int fn, const float* in, float* out, int n);
 
 /* ---- data generation ------------------------------------------------------ */
 void orc_dgp(int n_ticks, double rho, double sigma_z, double mu, double nu_state,

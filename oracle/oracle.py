@@ -102,6 +102,8 @@ def lib():
         L.orc_bpf_score.argtypes = [ctypes.POINTER(OrcParams), P, i32, i32, P, P, P, P, P, P]
         L.orc_set_fault.restype = None
         L.orc_set_fault.argtypes = [i32]
+        L.orc_math_array.restype = None
+        L.orc_math_array.argtypes = [i32, P, P, i32]
         L.orc_grid_loglik.restype = f64
         L.orc_grid_loglik.argtypes = [P, i32, f64, f64, f64, i32, P]
         _lib = L
@@ -122,6 +124,17 @@ def logf(x):
 
 def icdf_f32(u):
     return lib().orc_icdf_f32(float(u))
+
+
+MATH_ICDF, MATH_EXP, MATH_EXP_NONPOS, MATH_LOG = 0, 1, 2, 3
+
+
+def math_array(fn, x):
+    """Element-wise oracle arithmetic over a float32 array (fn as above)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty_like(x)
+    lib().orc_math_array(fn, _ptr(x), _ptr(out), x.size)
+    return out
 
 
 def icdf_f64(u):
