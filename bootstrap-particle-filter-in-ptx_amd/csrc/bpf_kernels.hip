@@ -198,38 +198,67 @@ __device__ inline void lane_normals(uint64_t pcg_a, uint64_t pcg_g, long long ba
 }
 
 // ---------------------------------------------------------------------------
-// Group fold (one wave): the group's tile partials, written by the previous
-// kernel, rescaled to the group maximum and summed in tile order.
+// Group fold (one block of kGroup threads, thread t <-> tile g*kGroup + t):
+// the group's tile partials, written by the previous kernel, rescaled to the
+// group maximum, with each tile's scale f and in-group inclusive prefix of
+// f*s (made non-decreasing within each wave; across waves the running offset
+// ends every wave exactly where the next begins) for the resample pass, and
+// the group's sums.
 // ---------------------------------------------------------------------------
 
-__device__ inline void group_fold(const Partials& p, int g, int lane) {
-    const int t = g * kGroup + lane;
-    const bool in = t < p.n_tiles && lane < kGroup;
+__device__ inline void group_fold(const Partials& p, int g, int tid, float* s_f, double* s_a,
+                                  double (*s_d)[kGroup / kWave]) {
+    constexpr int NW = kGroup / kWave;
+    const int lane = tid & (kWave - 1), wid = tid / kWave;
+    const int t = g * kGroup + tid;
+    const bool in = t < p.n_tiles;
     constexpr float NEG_INF = -__builtin_inff();
-    float m = in ? p.m[t] : NEG_INF;
-    double s = in ? p.s[t] : 0.0;
-    double sx = in ? p.sx[t] : 0.0;
-    double sxx = in ? p.sxx[t] : 0.0;
-    double sq = in ? p.sq[t] : 0.0;
-    const float gmax = wave_max(m);
-    const double f = (m == NEG_INF) ? 0.0 : (double)expf_cw(m - gmax);
+    const float m = in ? p.m[t] : NEG_INF;
+    const double s = in ? p.s[t] : 0.0;
+    const double sx = in ? p.sx[t] : 0.0;
+    const double sxx = in ? p.sxx[t] : 0.0;
+    const double sq = in ? p.sq[t] : 0.0;
+    const float wm = wave_max(m);
+    if (lane == 0) s_f[wid] = wm;
+    __syncthreads();
+    float gmax = s_f[0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) gmax = fmaxf(gmax, s_f[w]);
+    const double f = (m == NEG_INF) ? 0.0 : (double)expf_cw_nonpos(m - gmax);
     double vs = f * s, vx = f * sx, vxx = f * sxx, vq = (f * f) * sq;
-    // the tile's scale and in-group prefix for the resample pass
     const double pin = wave_incl_max(wave_incl_scan(vs, lane), lane);
-    if (in) {
-        p.tf[t] = f;
-        p.tp[t] = pin;
-    }
+    if (lane == kWave - 1) s_a[wid] = pin;
     vs = wave_sum(vs);
     vx = wave_sum(vx);
     vxx = wave_sum(vxx);
     vq = wave_sum(vq);
     if (lane == 0) {
+        s_d[0][wid] = vs;
+        s_d[1][wid] = vx;
+        s_d[2][wid] = vxx;
+        s_d[3][wid] = vq;
+    }
+    __syncthreads();
+    double off = 0.0;
+    for (int w = 0; w < wid; ++w) off += s_a[w];
+    if (in) {
+        p.tf[t] = f;
+        p.tp[t] = off + pin;
+    }
+    if (tid == 0) {
+        double r[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double acc = s_d[q][0];
+#pragma unroll
+            for (int w = 1; w < NW; ++w) acc += s_d[q][w];
+            r[q] = acc;
+        }
         p.gm[g] = gmax;
-        p.gs[g] = vs;
-        p.gsx[g] = vx;
-        p.gsxx[g] = vxx;
-        p.gsq[g] = vq;
+        p.gs[g] = r[0];
+        p.gsx[g] = r[1];
+        p.gsxx[g] = r[2];
+        p.gsq[g] = r[3];
     }
 }
 
@@ -422,15 +451,13 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Finalise (one block of 1024 threads).
-// ---------------------------------------------------------------------------
-
 // Finalise: one block per reduction group folds the group's tiles; the block
 // that arrives last (arrival counter; each block publishes its group record
-// with an agent-scope fence first -- cheap here, since the propagate kernel's
-// boundary already wrote its data back) reduces the group records, takes the
+// with an agent-scope fence first) reduces the group records, takes the
 // tick's decisions and, on resampling ticks, builds the FP64 group prefix.
-constexpr int kFinBlock = 256;
+// ---------------------------------------------------------------------------
+constexpr int kFinBlock = kGroup;
+constexpr int kFinPer = kMaxGroups / kFinBlock;   // groups per thread of the last block
 
 __device__ inline float host_pcg_float(unsigned long long* st) {
     unsigned long long old = *st;
@@ -438,21 +465,86 @@ __device__ inline float host_pcg_float(unsigned long long* st) {
     return (float)(pcg_output(old) >> 9) * (1.0f / 8388608.0f);
 }
 
+// W = first i in [0, n] with u + i/n >= 1 (FP32, gpu_bpf/gpu_bpf.cu:300-302);
+// the targets are non-decreasing in i.  Starts from i ~ (1 - u) n and
+// searches a window around it, widened to [0, n] if it does not bracket W.
+__device__ inline int wrap_start(float u, int n) {
+    long long g = (long long)ceil((1.0 - (double)u) * (double)n);
+    int lo = (int)max(0LL, min((long long)n, g - 512));
+    int hi = (int)max(0LL, min((long long)n, g + 512));
+    if (lo > 0 && target_unwrapped(u, lo - 1, n) >= 1.0f) lo = 0;
+    if (hi < n && target_unwrapped(u, hi, n) < 1.0f) hi = n;
+    while (lo < hi) {
+        int mid = (int)(((unsigned)lo + (unsigned)hi) >> 1);
+        if (target_unwrapped(u, mid, n) >= 1.0f) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+
+// The tick's statistics and decisions from the global sums (one thread, on
+// a register copy of the control block).
+__device__ inline void finalize_tick(Ctl& c, const FinArgs& a, float M, double S, double SX,
+                                     double SXX, double Q) {
+    const int carry = c.carry;
+    const double n = (double)a.n;
+    const double Md = (double)M;
+    BpfTickStats r;
+    r.tick = c.tick;
+    const bool ok = isfinite(S) && S > 0.0 && isfinite(Md);
+    if (ok) {
+        r.mean = SX / S;
+        r.var = SXX / S - r.mean * r.mean;
+        r.ess = (S * S) / Q;
+        r.ll_ref = c.off + Md + log(fmax(S / n, (double)bits_f32(0x0D6BF94Eu)));
+        r.ll_incr = carry ? (Md + log(S)) - log(c.s_prev) : Md + log(S / n);
+    } else {
+        const double qn = __builtin_nan("");
+        r.mean = r.var = r.ess = r.ll_ref = r.ll_incr = qn;
+    }
+    c.ll_cum += r.ll_incr;
+    r.ll_cum = c.ll_cum;
+    r.h_mean = (float)r.mean;
+    r.log_lik = (float)r.ll_ref;
+    const int resample = ok && (a.threshold <= 0.0f || (r.ess / n) < (double)a.threshold);
+    r.resampled = resample;
+    if (resample) {
+        const float u = host_pcg_float(&c.host_rng) / (float)a.n;
+        c.u = u;
+        c.wrap_start = wrap_start(u, a.n);
+        c.src = c.cur;
+        c.dst = 1 - c.cur;
+        c.cur = c.dst;
+        c.carry = 0;
+        c.off = 0.0;
+        c.resample_count += 1;
+        c.tile_next = 0;
+    } else {
+        c.carry = 1;
+        c.off += Md;
+        c.s_prev = S;
+    }
+    c.m_prev = M;
+    c.M = M;
+    c.S = S;
+    c.resample = resample;
+    c.last = r;
+    c.tick += 1;
+}
+
 __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
     constexpr int NW = kFinBlock / kWave;
     __shared__ float s_f[NW];
+    __shared__ double s_a[NW];
     __shared__ double s_d[4][NW];
-    __shared__ double s_scan[NW];
     __shared__ int s_flag;
     __shared__ double s_S;
-    __shared__ float s_M;
     __shared__ int s_last;
 
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const int Gg = a.part.n_groups;
     constexpr float NEG_INF = -__builtin_inff();
 
-    if (tid < kWave) group_fold(a.part, blockIdx.x, lane);
+    group_fold(a.part, blockIdx.x, tid, s_f, s_a, s_d);
     if (tid == 0) {
         __threadfence();
         const int old = atomicAdd(a.part.gcnt, 1);
@@ -463,28 +555,45 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
     __threadfence();
     if (tid == 0) *a.part.gcnt = 0;
 
+    // thread tid owns the contiguous groups [tid*C, tid*C + C), C <= kFinPer;
+    // their records are read once
+    const int C = (Gg + kFinBlock - 1) / kFinBlock;
+    const int g0 = tid * C;
+    float gm[kFinPer];
+    double gs[kFinPer], gsx[kFinPer], gsxx[kFinPer], gsq[kFinPer];
     float m = NEG_INF;
-    for (int g = tid; g < Gg; g += kFinBlock)
-        m = fmaxf(m, __hip_atomic_load(&a.part.gm[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+#pragma unroll
+    for (int j = 0; j < kFinPer; ++j) {
+        const int g = g0 + j;
+        const bool in = j < C && g < Gg;
+        gm[j] = in ? __hip_atomic_load(&a.part.gm[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : NEG_INF;
+        gs[j] = in ? __hip_atomic_load(&a.part.gs[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+        gsx[j] = in ? __hip_atomic_load(&a.part.gsx[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+        gsxx[j] = in ? __hip_atomic_load(&a.part.gsxx[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+        gsq[j] = in ? __hip_atomic_load(&a.part.gsq[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+        m = fmaxf(m, gm[j]);
+    }
+    Ctl c;
+    if (tid == 0) c = *a.ctl;          // read once, written back once
+
     m = wave_max(m);
     if (lane == 0) s_f[wid] = m;
     __syncthreads();
-    if (tid == 0) {
-        float mm = s_f[0];
-        for (int w = 1; w < NW; ++w) mm = fmaxf(mm, s_f[w]);
-        s_M = mm;
-    }
-    __syncthreads();
-    const float M = s_M;
+    float M = s_f[0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) M = fmaxf(M, s_f[w]);
 
+    // group weights v_g = f_g gs_g, f_g = exp(gm_g - M)
+    double v[kFinPer];
     double S = 0.0, SX = 0.0, SXX = 0.0, Q = 0.0;
-    for (int g = tid; g < Gg; g += kFinBlock) {
-        const float gm = __hip_atomic_load(&a.part.gm[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const double f = (gm == NEG_INF) ? 0.0 : (double)expf_cw(gm - M);
-        S += f * __hip_atomic_load(&a.part.gs[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        SX += f * __hip_atomic_load(&a.part.gsx[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        SXX += f * __hip_atomic_load(&a.part.gsxx[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        Q += (f * f) * __hip_atomic_load(&a.part.gsq[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int j = 0; j < kFinPer; ++j) {
+        const double f = (gm[j] == NEG_INF) ? 0.0 : (double)expf_cw_nonpos(gm[j] - M);
+        v[j] = f * gs[j];
+        S += v[j];
+        SX += f * gsx[j];
+        SXX += f * gsxx[j];
+        Q += (f * f) * gsq[j];
     }
     S = wave_sum(S);
     SX = wave_sum(SX);
@@ -500,114 +609,62 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
 
     if (tid == 0) {
         double t4[4];
+#pragma unroll
         for (int q = 0; q < 4; ++q) {
             double acc = s_d[q][0];
+#pragma unroll
             for (int w = 1; w < NW; ++w) acc += s_d[q][w];
             t4[q] = acc;
         }
-        S = t4[0]; SX = t4[1]; SXX = t4[2]; Q = t4[3];
-        Ctl* c = a.ctl;
-        const int carry = c->carry;
-        const double n = (double)a.n;
-        const double Md = (double)M;
-        BpfTickStats r;
-        r.tick = c->tick;
-        const bool ok = isfinite(S) && S > 0.0 && isfinite(Md);
-        if (ok) {
-            r.mean = SX / S;
-            r.var = SXX / S - r.mean * r.mean;
-            r.ess = (S * S) / Q;
-            r.ll_ref = c->off + Md + log(fmax(S / n, (double)bits_f32(0x0D6BF94Eu)));
-            r.ll_incr = carry ? (Md + log(S)) - log(c->s_prev) : Md + log(S / n);
-        } else {
-            const double qn = __builtin_nan("");
-            r.mean = r.var = r.ess = r.ll_ref = r.ll_incr = qn;
+        finalize_tick(c, a, M, t4[0], t4[1], t4[2], t4[3]);
+        *a.ctl = c;
+        a.ring[c.last.tick % a.ring_cap] = c.last;
+        if (c.resample) {
+            if (a.u_log) *a.u_log = c.u;
+            if (a.w_log) *a.w_log = c.wrap_start;
         }
-        c->ll_cum += r.ll_incr;
-        r.ll_cum = c->ll_cum;
-        r.h_mean = (float)r.mean;
-        r.log_lik = (float)r.ll_ref;
-        const int resample = ok && (a.threshold <= 0.0f || (r.ess / n) < (double)a.threshold);
-        r.resampled = resample;
-        if (resample) {
-            float u = host_pcg_float(&c->host_rng) / (float)a.n;
-            // W = first i with u + i/N >= 1 (targets are non-decreasing in i)
-            int lo = 0, hi = a.n;
-            while (lo < hi) {
-                int mid = (int)(((unsigned)lo + (unsigned)hi) >> 1);
-                if (target_unwrapped(u, mid, a.n) >= 1.0f) hi = mid; else lo = mid + 1;
-            }
-            c->u = u;
-            c->wrap_start = lo;
-            if (a.u_log) *a.u_log = u;
-            if (a.w_log) *a.w_log = lo;
-            c->src = c->cur;
-            c->dst = 1 - c->cur;
-            c->cur = c->dst;
-            c->carry = 0;
-            c->off = 0.0;
-            c->resample_count += 1;
-        } else {
-            c->carry = 1;
-            c->off += Md;
-            c->s_prev = S;
-        }
-        c->m_prev = M;
-        c->M = M;
-        c->S = S;
-        c->resample = resample;
-        c->last = r;
-        a.ring[c->tick % a.ring_cap] = r;
-        c->tick += 1;
-        s_flag = resample;
-        s_S = S;
+        s_flag = c.resample;
+        s_S = c.S;
     }
     __syncthreads();
     if (!s_flag) return;
 
-    // FP64 exclusive prefix of the group weights v_g = f_g * gs_g, normalised
-    // by S, clamped to [0, 1], made non-decreasing by a prefix maximum.
-    // Thread tid owns the contiguous groups [tid*C, tid*C + C).
+    // FP64 exclusive prefix of the group weights, normalised by S, clamped to
+    // [0, 1] and made non-decreasing by a prefix maximum (the block scan's
+    // association differs between groups, so rounding alone could break the
+    // order)
     S = s_S;
-    const int C = (Gg + kFinBlock - 1) / kFinBlock;
-    const int g0 = tid * C;
-    const int g1 = min(g0 + C, Gg);
     double loc = 0.0;
-    for (int g = g0; g < g1; ++g) {
-        const float gm = __hip_atomic_load(&a.part.gm[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const double gs = __hip_atomic_load(&a.part.gs[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        loc += ((gm == NEG_INF) ? 0.0 : (double)expf_cw(gm - M)) * gs;
-    }
-    double incl = wave_incl_scan(loc, lane);
-    if (lane == kWave - 1) s_scan[wid] = incl;
+#pragma unroll
+    for (int j = 0; j < kFinPer; ++j) loc += v[j];
+    const double incl = wave_incl_scan(loc, lane);
+    if (lane == kWave - 1) s_a[wid] = incl;
     __syncthreads();
     double B = __shfl_up(incl, 1, kWave);
     B = (lane == 0) ? 0.0 : B;
-    for (int w = 0; w < wid; ++w) B += s_scan[w];
+    for (int w = 0; w < wid; ++w) B += s_a[w];
+    double bt[kFinPer];
     double hi_v = 0.0;
-    for (int g = g0; g < g1; ++g) {
-        const float gm = __hip_atomic_load(&a.part.gm[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const double gs = __hip_atomic_load(&a.part.gs[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const double bt = fmin(fmax(B / S, 0.0), 1.0);
-        a.gptil[g] = bt;
-        hi_v = bt;
-        B += ((gm == NEG_INF) ? 0.0 : (double)expf_cw(gm - M)) * gs;
+#pragma unroll
+    for (int j = 0; j < kFinPer; ++j) {
+        bt[j] = fmin(fmax(B / S, 0.0), 1.0);
+        if (j < C) hi_v = fmax(hi_v, bt[j]);
+        B += v[j];
     }
     __syncthreads();
-    double pm = wave_incl_max(hi_v, lane);
-    double wl = __shfl_up(pm, 1, kWave);
-    wl = (lane == 0) ? 0.0 : wl;
-    if (lane == kWave - 1) s_scan[wid] = pm;
+    const double pm = wave_incl_max(hi_v, lane);
+    double pre = __shfl_up(pm, 1, kWave);
+    pre = (lane == 0) ? 0.0 : pre;
+    if (lane == kWave - 1) s_a[wid] = pm;
     __syncthreads();
-    double pre = wl;
-    for (int w = 0; w < wid; ++w) pre = fmax(pre, s_scan[w]);
-    for (int g = g0; g < g1; ++g) {
-        if (a.gptil[g] < pre) a.gptil[g] = pre;
+    for (int w = 0; w < wid; ++w) pre = fmax(pre, s_a[w]);
+#pragma unroll
+    for (int j = 0; j < kFinPer; ++j) {
+        const int g = g0 + j;
+        pre = fmax(pre, bt[j]);
+        if (j < C && g < Gg) a.gptil[g] = (g == 0) ? 0.0 : pre;
     }
-    if (tid == 0) {
-        a.gptil[0] = 0.0;
-        a.gptil[Gg] = 1.0;
-    }
+    if (tid == 0) a.gptil[Gg] = 1.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -786,7 +843,7 @@ constexpr int kLongRuns = 64;
 
 template <int BLOCK, int K>
 __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
-    const Ctl* ctl = a.ctl;
+    Ctl* ctl = a.ctl;
     if (!ctl->resample) return;
     constexpr int VEC = K < 4 ? K : 4;
     constexpr int NV = K / VEC;
@@ -802,111 +859,130 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
     __shared__ int s_q[kLongRuns];     // lanes whose children run is long
     __shared__ int s_qa[kLongRuns];
     __shared__ int s_qb[kLongRuns];
+    __shared__ int s_tile;
 
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
-    const int b = blockIdx.x;
     const int n = a.n;
-    const long long base = (long long)b * TILE;
     const float* __restrict__ xs = ctl->src ? a.x1 : a.x0;
     float* __restrict__ xd = ctl->dst ? a.x1 : a.x0;
-
-    float xv[K], lv[K];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        const int loc0 = (v * BLOCK + tid) * VEC;
-        load_vec<VEC>(xs + base + loc0, &xv[v * VEC]);
-        if (a.lw) load_vec<VEC>(a.lw + base + loc0, &lv[v * VEC]);
-    }
     const float u = ctl->u;
     const int W = ctl->wrap_start;
-    const float mb = a.part.m[b];
-    const bool dead = (mb == -__builtin_inff());
-    const TileScale ts = tile_scale(a.part, a.gptil, ctl->M, ctl->S, b);
-    if (a.m_log && tid == 0) {
-        a.m_log[b] = mb;
-        a.ft_log[b] = ts.ft;
-        a.b_log[b] = ts.b0;
-        if (b == a.part.n_tiles - 1) a.b_log[b + 1] = ts.b1;
-    }
-    if (tid == 0) s_nq = 0;
-
+    const float Mg = ctl->M;
+    const double Sg = ctl->S;
     ObsModel obs = a.obs;
     {
         float yv = a.y_ptr ? *a.y_ptr : a.y;
         obs.ysq = yv * yv;
     }
 
-    float e[K];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-#pragma unroll
-        for (int c = 0; c < VEC; ++c) {
-            const int k = v * VEC + c;
-            const int loc = (v * BLOCK + tid) * VEC + c;
-            float lk = a.lw ? lv[k] : log_weight(xv[k], obs);
-            bool valid = base + loc < n;
-            e[k] = (dead || !valid) ? 0.0f : expf_cw(lk - mb);
-            s_x[pad16(loc)] = xv[k];
-        }
-    }
-    const int last = tile_cdf<BLOCK, K>(e, b, n, a.part.n_tiles, ts.ft, ts.b0, ts.b1, u, W, s_c, s_w,
-                                        s_rng);
-    const int ilo = s_rng[0], ihi = s_rng[1], jlo = s_rng[2], jhi = s_rng[3];
-
-    // Lane tid owns parents j in [tid*K, tid*K+K): their CDF values and states.
-    double cl[K];
-    float xl[K];
-#pragma unroll
-    for (int q = 0; q < K; ++q) {
-        cl[q] = s_c[pad16(tid * K + q)];
-        xl[q] = s_x[pad16(tid * K + q)];
-    }
-    // End of the lane's children run among the unwrapped targets: the first i
-    // whose target exceeds its last CDF value; the start is the previous
-    // lane's end (ilo for lane 0).
-    const int lastj = tid * K + K - 1;
-    int ib = (lastj >= last) ? ihi : count_le_fast(cl[K - 1], u, ilo, ihi, n);
-    int ia = __shfl_up(ib, 1, kWave);
-    if (lane == kWave - 1) s_edge[wid] = ib;
+    // A grid of resident blocks takes tiles from a work counter (reset by
+    // the finalise pass): one launch wave, so a tick without resampling costs
+    // one short wave of early exits, and heavy tiles (long offspring runs)
+    // do not pile up on a few blocks.  Each block claims its next tile while
+    // it loads the current one and publishes the claim before it stores
+    // anything (waiting on the claim then waits on no store); every block
+    // leaves once the counter has passed the last tile.
+    int nb = 0;
+    if (tid == 0) s_tile = atomicAdd(&ctl->tile_next, 1);
     __syncthreads();
-    ia = (lane == 0) ? ((wid == 0) ? ilo : s_edge[wid - 1]) : ia;
+    for (;;) {
+        const int b = s_tile;
+        if (b >= a.part.n_tiles) break;
+        if (tid == 0) nb = atomicAdd(&ctl->tile_next, 1);
+        const long long base = (long long)b * TILE;
 
-    int slot = kLongRuns;
-    if (ib - ia > kLaneRunCap) slot = atomicAdd(&s_nq, 1);
-    if (slot < kLongRuns) {
-        s_q[slot] = tid;
-        s_qa[slot] = ia;
-        s_qb[slot] = ib;
-    } else {
-        for (int i = ia; i < ib; ++i) {
-            const double t = (double)target_unwrapped(u, i, n);
-            // first owned parent with c >= t: the count of smaller values
-            int q = 0;
+        float xv[K], lv[K];
 #pragma unroll
-            for (int r = 0; r < K - 1; ++r) q += (cl[r] < t) ? 1 : 0;
-            float v = xl[0];
-#pragma unroll
-            for (int r = 1; r < K; ++r) v = (q == r) ? xl[r] : v;
-            xd[i] = v;
+        for (int v = 0; v < NV; ++v) {
+            const int loc0 = (v * BLOCK + tid) * VEC;
+            load_vec<VEC>(xs + base + loc0, &xv[v * VEC]);
+            if (a.lw) load_vec<VEC>(a.lw + base + loc0, &lv[v * VEC]);
         }
-    }
-    __syncthreads();
-    // long runs: the whole block writes them, coalesced
-    const int nq = min(s_nq, kLongRuns);
-    for (int k = 0; k < nq; ++k) {
-        const int owner = s_q[k], qa = s_qa[k], qb = s_qb[k];
-        for (int i = qa + tid; i < qb; i += BLOCK) {
-            const double t = (double)target_unwrapped(u, i, n);
-            int q = 0;
-#pragma unroll
-            for (int r = 0; r < K - 1; ++r) q += (s_c[pad16(owner * K + r)] < t) ? 1 : 0;
-            xd[i] = s_x[pad16(owner * K + q)];
+        const float mb = a.part.m[b];
+        const bool dead = (mb == -__builtin_inff());
+        const TileScale ts = tile_scale(a.part, a.gptil, Mg, Sg, b);
+        if (a.m_log && tid == 0) {
+            a.m_log[b] = mb;
+            a.ft_log[b] = ts.ft;
+            a.b_log[b] = ts.b0;
+            if (b == a.part.n_tiles - 1) a.b_log[b + 1] = ts.b1;
         }
-    }
-    // wrapped targets (a few at most)
-    for (int i = jlo + tid; i < jhi; i += BLOCK) {
-        double t = (double)(target_unwrapped(u, i, n) - 1.0f);
-        xd[i] = s_x[pad16(lds_lower_bound(s_c, last, t))];
+        if (tid == 0) s_nq = 0;
+
+        float e[K];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) {
+                const int k = v * VEC + c;
+                const int loc = (v * BLOCK + tid) * VEC + c;
+                float lk = a.lw ? lv[k] : log_weight(xv[k], obs);
+                bool valid = base + loc < n;
+                e[k] = (dead || !valid) ? 0.0f : expf_cw(lk - mb);
+                s_x[pad16(loc)] = xv[k];
+            }
+        }
+        const int last = tile_cdf<BLOCK, K>(e, b, n, a.part.n_tiles, ts.ft, ts.b0, ts.b1, u, W, s_c, s_w,
+                                            s_rng);
+        const int ilo = s_rng[0], ihi = s_rng[1], jlo = s_rng[2], jhi = s_rng[3];
+        if (tid == 0) s_tile = nb;     // every thread has read b (barriers in tile_cdf)
+
+        // Lane tid owns parents j in [tid*K, tid*K+K): their CDF values and states.
+        double cl[K];
+        float xl[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            cl[q] = s_c[pad16(tid * K + q)];
+            xl[q] = s_x[pad16(tid * K + q)];
+        }
+        // End of the lane's children run among the unwrapped targets: the first i
+        // whose target exceeds its last CDF value; the start is the previous
+        // lane's end (ilo for lane 0).
+        const int lastj = tid * K + K - 1;
+        int ib = (lastj >= last) ? ihi : count_le_fast(cl[K - 1], u, ilo, ihi, n);
+        int ia = __shfl_up(ib, 1, kWave);
+        if (lane == kWave - 1) s_edge[wid] = ib;
+        __syncthreads();
+        ia = (lane == 0) ? ((wid == 0) ? ilo : s_edge[wid - 1]) : ia;
+
+        int slot = kLongRuns;
+        if (ib - ia > kLaneRunCap) slot = atomicAdd(&s_nq, 1);
+        if (slot < kLongRuns) {
+            s_q[slot] = tid;
+            s_qa[slot] = ia;
+            s_qb[slot] = ib;
+        } else {
+            for (int i = ia; i < ib; ++i) {
+                const double t = (double)target_unwrapped(u, i, n);
+                // first owned parent with c >= t: the count of smaller values
+                int q = 0;
+#pragma unroll
+                for (int r = 0; r < K - 1; ++r) q += (cl[r] < t) ? 1 : 0;
+                float v = xl[0];
+#pragma unroll
+                for (int r = 1; r < K; ++r) v = (q == r) ? xl[r] : v;
+                xd[i] = v;
+            }
+        }
+        __syncthreads();
+        // long runs: the whole block writes them, coalesced
+        const int nq = min(s_nq, kLongRuns);
+        for (int k = 0; k < nq; ++k) {
+            const int owner = s_q[k], qa = s_qa[k], qb = s_qb[k];
+            for (int i = qa + tid; i < qb; i += BLOCK) {
+                const double t = (double)target_unwrapped(u, i, n);
+                int q = 0;
+#pragma unroll
+                for (int r = 0; r < K - 1; ++r) q += (s_c[pad16(owner * K + r)] < t) ? 1 : 0;
+                xd[i] = s_x[pad16(owner * K + q)];
+            }
+        }
+        // wrapped targets (a few at most)
+        for (int i = jlo + tid; i < jhi; i += BLOCK) {
+            double t = (double)(target_unwrapped(u, i, n) - 1.0f);
+            xd[i] = s_x[pad16(lds_lower_bound(s_c, last, t))];
+        }
+        __syncthreads();          // the next tile reuses the LDS arrays
     }
 }
 
@@ -1099,7 +1175,21 @@ static hipError_t launch_tick_k(const PropArgs& p, const FinArgs& f, const ResAr
     // waves per CU and half the per-lane chain of the 256-lane layout
     constexpr int TILE = kBlock * K;
     constexpr int RB = TILE >= 512 ? 512 : TILE;
-    hipLaunchKernelGGL((k_resample<RB, TILE / RB>), dim3(r.part.n_tiles), dim3(RB), 0, st, r);
+    static int resident = 0;     // resample blocks the device holds at once
+    if (resident == 0) {
+        int per_cu = 0, cus = 0, dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_resample<RB, TILE / RB>, RB, 0);
+        if (e != hipSuccess) return e;
+        resident = per_cu > 0 && cus > 0 ? per_cu * cus : 1024;
+    }
+    // three blocks per resident slot: measured best at 51.2M particles (the
+    // claims of a block overlap its neighbours' tiles; many more blocks make
+    // the counter the bottleneck)
+    const int grid = r.part.n_tiles < 3 * resident ? r.part.n_tiles : 3 * resident;
+    hipLaunchKernelGGL((k_resample<RB, TILE / RB>), dim3(grid), dim3(RB), 0, st, r);
     if (ev) (void)hipEventRecord(ev[3], st);
     return hipGetLastError();
 }

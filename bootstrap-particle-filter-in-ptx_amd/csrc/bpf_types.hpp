@@ -8,10 +8,13 @@
 
 namespace bpf {
 
-// Tiles per reduction group: one finalise block folds each group of tile
-// partials into a group partial, and the block that finishes last reduces
-// G / kGroup group records instead of G tile records.
-constexpr int kGroup = 64;
+// Tiles per reduction group: one finalise block (one thread per tile) folds
+// each group of tile partials into a group partial, and the block that
+// finishes last reduces G / kGroup group records instead of G tile records.
+constexpr int kGroup = 256;
+// Largest group count the finalise kernel's last block holds (256 threads x
+// 4 groups): 2^30 particles at 16 per lane.
+constexpr int kMaxGroups = 1024;
 
 // Device-resident control block.  Written only by the finalise kernel's last
 // block each tick and read by the next kernels on the same stream, so
@@ -25,10 +28,10 @@ struct Ctl {
     int wrap_start;      // first i whose unwrapped target u + i/N is >= 1
     int resample_count;
     int tick;            // ticks finalised so far
+    int tile_next;       // resample work counter: next tile to hand out
     float m_prev;        // max of the log-weights carried into the next tick
     float u;             // resampling offset u / N of the latest resample
     float M;             // global max log-weight of the latest tick
-    float pad_;
     double S;            // sum exp(l - M) of the latest tick
     double s_prev;       // sum exp(l - m_prev) of the carried weights
     double off;          // raw log-weight offset (reference log_lik semantics)
@@ -93,7 +96,7 @@ struct ResArgs {
     const float* lw;      // stored log-weights, or null: recompute from x
     Partials part;
     const double* gptil;
-    const Ctl* ctl;
+    Ctl* ctl;             // tile_next is the kernel's work counter
     const float* y_ptr;
     float y;
     ObsModel obs;

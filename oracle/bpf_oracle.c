@@ -361,12 +361,27 @@ static double c_obs_f64(double nu) {
     return lgamma((nu + 1.0) / 2.0) - lgamma(nu / 2.0) - 0.5 * log(nu * 3.14159265358979323846);
 }
 
+/* Normalised CDF of the weights: an 80-bit running sum divided by its own
+ * total, rounded to FP64 once.  This is the exact CDF to within an FP64
+ * rounding, so targets resolve as they do on the exact values unless one
+ * lies within ~1e-16 of a CDF step (the device's three-level FP64 CDF carries
+ * errors of that size). */
+static void build_cdf(const double* w, int n, double* cdf) {
+    long double tot = 0.0L;
+    for (int i = 0; i < n; i++) tot += (long double)w[i];
+    long double c = 0.0L;
+    for (int i = 0; i < n; i++) {
+        c += (long double)w[i];
+        cdf[i] = (double)(c / tot);
+    }
+}
+
 /* Weights as the GPU forms them: w_i = e_i * f_tile * f_group with
  * e_i = exp(l_i - m_tile), f_tile = exp(m_tile - m_group),
  * f_group = exp(m_group - M) (each an FP32 orc_expf; the two scales multiply
- * exactly in FP64), tiles of `tile` particles, groups of 64 tiles.
+ * exactly in FP64), tiles of `tile` particles, groups of 256 tiles.
  * Returns M.  tmax/tf (n_tiles) receive the tile maxima and FP64 scales. */
-#define ORC_GROUP 64
+#define ORC_GROUP 256
 static float tiled_weights(const float* l, int n, int tile, float* tmax, double* tf, double* w,
                            int n_threads) {
     const int n_tiles = (n + tile - 1) / tile;
@@ -561,8 +576,7 @@ int orc_bpf_run(const OrcParams* p, int mode, const float* y, const float* mu_t,
         if (resample) {
             uint32_t hr = orc_pcg_host_next(&hrng);
             float u = ((float)(hr >> 9) * (1.0f / 8388608.0f)) / (float)n;
-            double c = 0.0;
-            for (int i = 0; i < n; i++) { c += w[i]; cdf[i] = c / S32; }
+            build_cdf(w, n, cdf);
             if (n_threads <= 1) {
                 orc_resample_ancestors(cdf, n, u, anc);
             } else {
@@ -806,8 +820,7 @@ int orc_bpf_score(const OrcParams* p, const float* y, int n_ticks, int x0_repara
             ll32 += (double)gm + log(S32 / (double)n);
             uint32_t hr = orc_pcg_host_next(&hrng);
             float u = ((float)(hr >> 9) * (1.0f / 8388608.0f)) / (float)n;
-            double c = 0.0;
-            for (int i = 0; i < n; i++) { c += w[i]; cdf[i] = c / S32; }
+            build_cdf(w, n, cdf);
             orc_resample_ancestors(cdf, n, u, anc);
             for (int i = 0; i < n; i++) xt[i] = x[anc[i]];
             float* sw = x; x = xt; xt = sw;

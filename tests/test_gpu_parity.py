@@ -237,3 +237,19 @@ def test_degenerate_weights(orc, ydata, items, spike):
     assert_stats_close(st, ref)
     np.testing.assert_array_equal(x, xo)
     assert st["ess"][5] < n / 20
+
+
+@pytest.mark.parametrize("items,n", [(1, 700_000), (4, 1_100_000)])
+def test_many_reduction_groups(orc, ydata, items, n):
+    """Several 256-tile reduction groups: the finalise pass's group records,
+    its last-block reduction and the FP64 group prefix of the CDF.  (Sizes
+    keep the chance that a target falls within the ~1e-16 by which the two
+    CDF constructions differ well below 1e-3 over the run.)"""
+    _, y = ydata
+    T = 25
+    say("groups", items, n, T)
+    _, st, x, tile, _ = gpu_run(n, y[:T], 0.0, items=items)
+    assert (n + tile - 1) // tile > 4 * 256       # more than 4 groups
+    ref, _, xo, _ = orc.run_filter(y[:T], n, ess_threshold=0.0, tile=tile, want_x=True, threads=16)
+    assert_stats_close(st, ref)
+    np.testing.assert_array_equal(x, xo)
