@@ -1,0 +1,424 @@
+// bpf_batch.hip -- batched inner bootstrap filters (the SMC^2 inner layer)
+// for gfx950.
+//
+// One launch per tick advances every inner filter: workgroup g runs the whole
+// tick of slot order[g] (propagate, weights, FP64 statistics and
+// log-likelihood, the resampling decision, FP64 CDF and systematic
+// resampling), so filters of uneven size need no grid-wide step.  Each wave
+// takes whole 256-particle tiles, four particles per lane.  The arithmetic
+// restates the single filter's kernels at one particle per lane -- 64-particle
+// group q of a tile plays the single filter's wave q -- so an inner filter
+// with the same parameters, particle count and seed produces the single
+// filter's particles, ancestors and statistics.
+//
+// Noise: the PCG32 streams of the single filter (per slot seed), optionally
+// stored for later replays, or read back from the store, or proposed by a
+// Crank-Nicolson step around the store (correlated pseudo-marginal moves).
+#include <hip/hip_runtime.h>
+
+#include "bpf_common.hpp"
+
+namespace bpf {
+
+// Orders the wave's own LDS writes before its later LDS reads.
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Standard normals z[q] of particles base + 64 q + lane for the draw with
+// jump constants (jm, jg) of stream `seed`.  Tail uniforms (4.85%) are
+// compacted through the wave's LDS queue and evaluated densely.
+__device__ inline void wave_normals(uint64_t pa, uint64_t pg, int base, int lane, float* s_q,
+                                    float (&z)[4]) {
+    bool tail[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = base + 64 * q + lane;
+        const uint64_t st = pa + (uint64_t)(2 * i + 1) * pg;
+        const float u = pcg_uniform(pcg_output(st));
+        const bool cen = icdf_is_central(u);
+        z[q] = cen ? icdf_central(u - 0.5f) : u;
+        tail[q] = !cen;
+    }
+    int pos[4];
+    int total = 0;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint64_t bal = __ballot(tail[q]);
+        pos[q] = total + __popcll(bal & below);
+        total += __popcll(bal);
+    }
+    if (total == 0) return;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (tail[q]) s_q[pos[q]] = z[q];
+    wave_lds_sync();
+    for (int e = lane; e < total; e += kWave) s_q[e] = icdf_tail(s_q[e]);
+    wave_lds_sync();
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (tail[q]) z[q] = s_q[pos[q]];
+    wave_lds_sync();
+}
+
+// Jump constants of draw k (the single filter's tick-k draw): particle
+// i's PCG32 state is seed * A + (2i + 1) * G with A = M^(k+2) and
+// G = sum_{j < k+2} M^j (mod 2^64), by binary powering of the LCG step.
+__device__ inline void pcg_draw_constants(uint32_t k, uint64_t& A, uint64_t& G) {
+    uint64_t am = 1, ap = 0, cm = kPcgMult, cpl = 1;
+    for (uint32_t d = k + 2; d > 0; d >>= 1) {
+        if (d & 1u) {
+            am *= cm;
+            ap = ap * cm + cpl;
+        }
+        cpl = (cm + 1) * cpl;
+        cm *= cm;
+    }
+    A = am;
+    G = ap;
+}
+
+// The normals of one tick of slot m under the batch's noise source.
+__device__ inline void batch_normals(const BatchArgs& a, int m, uint64_t pa, uint64_t jm,
+                                     uint64_t jg, int base, int lane, int zc, size_t zoff,
+                                     float* s_q, float (&z)[4]) {
+    if (a.noise == kNoiseStored || a.noise == kNoisePropose) {
+        const float* zs = a.zpool[zc] + zoff + base;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) z[q] = zs[64 * q + lane];
+        if (a.noise == kNoisePropose) {
+            float xi[4];
+            const uint64_t pp = (uint64_t)(int64_t)a.prop_seed[m] * jm;
+            wave_normals(pp, jg, base, lane, s_q, xi);
+            float* zd = a.zpool[1 - zc] + zoff + base;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                z[q] = __builtin_fmaf(a.cn_s, xi[q], a.cn_rho * z[q]);
+                zd[64 * q + lane] = z[q];
+            }
+        }
+        return;
+    }
+    wave_normals(pa, jg, base, lane, s_q, z);
+    if (a.noise == kNoisePcgStore) {
+        float* zd = a.zpool[zc] + zoff + base;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) zd[64 * q + lane] = z[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Initialisation: x_0 = mu + sd z with z from draw 0 (as k_init), padding
+// (i >= n) set to mu.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_batch_init(BatchInitArgs a) {
+    const int m = blockIdx.x;
+    const int n = a.n[m];
+    const float mu = a.mu[m], phi = a.phi[m], sig = a.sig[m];
+    const float sd = __builtin_sqrtf((sig * sig) / fmaxf(1.0f - phi * phi, 1e-6f));
+    const uint64_t pa = (uint64_t)(int64_t)a.seed[m] * a.jm;
+    const int zc = (a.noise >= kNoisePcgStore) ? a.zcur[m] : 0;
+    const size_t off = (size_t)m * a.cap;     // tick 0 of the pools
+    float* x = a.x0 + off;
+    for (int i = threadIdx.x; i < a.cap; i += kBlock) {
+        float v = mu;
+        if (i < n) {
+            float z;
+            if (a.noise == kNoiseStored || a.noise == kNoisePropose) {
+                z = a.zpool[zc][off + i];
+                if (a.noise == kNoisePropose) {
+                    const uint64_t pp = (uint64_t)(int64_t)a.prop_seed[m] * a.jm;
+                    const float xi = icdf(pcg_uniform(pcg_output(pp + (uint64_t)(2 * i + 1) * a.jg)));
+                    z = __builtin_fmaf(a.cn_s, xi, a.cn_rho * z);
+                    a.zpool[1 - zc][off + i] = z;
+                }
+            } else {
+                z = icdf(pcg_uniform(pcg_output(pa + (uint64_t)(2 * i + 1) * a.jg)));
+                if (a.noise == kNoisePcgStore) a.zpool[zc][off + i] = z;
+            }
+            v = __builtin_fmaf(sd, z, mu);
+        }
+        x[i] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One tick of every inner filter.
+// ---------------------------------------------------------------------------
+
+template <int BLOCK, bool STUDENT>
+__global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
+    constexpr int NW = BLOCK / kWave;
+    constexpr int TP = kBatchTile + kBatchTile / 16;       // padded tile
+    constexpr float NEG_INF = -__builtin_inff();
+    __shared__ float s_q[NW][kBatchTile];     // tail queue of each wave
+    __shared__ double s_c[NW][TP];            // in-tile CDF of each wave
+    __shared__ float s_x[NW][TP];             // tile particles of each wave
+    __shared__ float s_tmax[kBatchMaxTiles];
+    __shared__ double s_ts[4][kBatchMaxTiles];
+    __shared__ double s_tf[kBatchMaxTiles];
+    __shared__ double s_tp[kBatchMaxTiles];
+    __shared__ int s_flag, s_W;
+    __shared__ float s_u, s_M;
+    __shared__ double s_S;
+
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const int m = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
+    const int n = a.n[m];
+    const int nt = (n + kBatchTile - 1) / kBatchTile;
+    Ctl* cp = a.ctl + m;
+    const int cur = cp->cur, carry = cp->carry;
+    const float moff = carry ? cp->m_prev : 0.0f;
+    const size_t off = (size_t)m * a.cap;
+    float* __restrict__ xb = (cur ? a.x1 : a.x0) + off;
+    float* __restrict__ xo = (cur ? a.x0 : a.x1) + off;
+    float* __restrict__ lb = a.lw + off;
+    const float mu = a.mu[m], phi = a.phi[m], sig = a.sig[m];
+    const int tick = cp->tick;
+    uint64_t jm, jg;
+    pcg_draw_constants((uint32_t)tick, jm, jg);
+    const uint64_t pa = (uint64_t)(int64_t)a.seed[m] * jm;
+    ObsModel obs = a.obs;
+    {
+        const float yv = a.y[tick];
+        obs.ysq = yv * yv;
+    }
+    const int zc = (a.noise >= kNoisePcgStore) ? a.zcur[m] : 0;
+    const size_t zoff = ((size_t)tick * a.n_slots + m) * a.cap;
+
+    // Propagate, log-weights (carry as the single filter), tile maxima and
+    // the tile sums of the single filter's propagate pass: one wave sum per
+    // 64-particle group, the four groups added in order.
+    for (int b = wid; b < nt; b += NW) {
+        const int base = b * kBatchTile;
+        float x[4], l[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = base + 64 * q + lane;
+            x[q] = xb[j];
+            l[q] = carry ? lb[j] : 0.0f;
+        }
+        if (tick > 0) {      // tick 0 weights the stationary draw
+            float z[4];
+            batch_normals(a, m, pa, jm, jg, base, lane, zc, zoff, s_q[wid], z);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                x[q] = transition(x[q], mu, phi, sig, z[q]);
+                xb[base + 64 * q + lane] = x[q];
+            }
+        }
+        float mloc = NEG_INF;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = base + 64 * q + lane;
+            const float lw = STUDENT ? log_weight_t(x[q], obs) : log_weight_gauss(x[q], obs.ysq);
+            const float lk = lw + (l[q] - moff);
+            l[q] = (j < n) ? lk : NEG_INF;
+            mloc = fmaxf(mloc, l[q]);
+            lb[j] = l[q];
+        }
+        const float mt = wave_max(mloc);
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        if (mt != NEG_INF) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int j = base + 64 * q + lane;
+                const double de = (double)expf_cw_nonpos(l[q] - mt);
+                const double dx = (j < n) ? (double)x[q] : 0.0;
+                const double t = de * dx;
+                const double ws = wave_sum(0.0 + de);
+                const double wx = wave_sum(0.0 + t);
+                const double wxx = wave_sum(fma(t, dx, 0.0));
+                const double wq = wave_sum(fma(de, de, 0.0));
+                if (q == 0) {
+                    acc[0] = ws; acc[1] = wx; acc[2] = wxx; acc[3] = wq;
+                } else {
+                    acc[0] += ws; acc[1] += wx; acc[2] += wxx; acc[3] += wq;
+                }
+            }
+        } else {
+            // the single filter's zero sums, added over its four waves
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[k] = ((0.0 + 0.0) + 0.0) + 0.0;
+        }
+        if (lane == 0) {
+            s_tmax[b] = mt;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) s_ts[k][b] = acc[k];
+        }
+    }
+    __syncthreads();
+
+    // Fold of the filter's tiles (its one reduction group), statistics,
+    // log-likelihood and decision -- the single filter's finalise pass with
+    // one group: its empty waves and lanes contribute exact zeros.
+    if (wid == 0) {
+        const bool in = lane < nt;
+        const float tm = in ? s_tmax[lane] : NEG_INF;
+        const double t0 = in ? s_ts[0][lane] : 0.0;
+        const double t1 = in ? s_ts[1][lane] : 0.0;
+        const double t2 = in ? s_ts[2][lane] : 0.0;
+        const double t3 = in ? s_ts[3][lane] : 0.0;
+        const float gmax = wave_max(tm);
+        const double f = (tm == NEG_INF) ? 0.0 : (double)expf_cw_nonpos(tm - gmax);
+        double vs = f * t0, vx = f * t1, vxx = f * t2, vq = (f * f) * t3;
+        const double pin = wave_incl_max(wave_incl_scan(vs, lane), lane);
+        if (in) {
+            s_tf[lane] = f;
+            s_tp[lane] = 0.0 + pin;
+        }
+        vs = wave_sum(vs);
+        vx = wave_sum(vx);
+        vxx = wave_sum(vxx);
+        vq = wave_sum(vq);
+        if (lane == 0) {
+            const double gs = ((vs + 0.0) + 0.0) + 0.0;
+            const double gsx = ((vx + 0.0) + 0.0) + 0.0;
+            const double gsxx = ((vxx + 0.0) + 0.0) + 0.0;
+            const double gsq = ((vq + 0.0) + 0.0) + 0.0;
+            const double fg = (gmax == NEG_INF) ? 0.0 : (double)expf_cw_nonpos(gmax - gmax);
+            const double v0 = fg * gs;
+            const double S = (((((((0.0 + v0) + 0.0) + 0.0) + 0.0) + 0.0) + 0.0) + 0.0) + 0.0;
+            const double SX = (((((((0.0 + fg * gsx) + 0.0) + 0.0) + 0.0) + 0.0) + 0.0) + 0.0) + 0.0;
+            const double SXX = (((((((0.0 + fg * gsxx) + 0.0) + 0.0) + 0.0) + 0.0) + 0.0) + 0.0) + 0.0;
+            const double Q = (((((((0.0 + (fg * fg) * gsq) + 0.0) + 0.0) + 0.0) + 0.0) + 0.0) + 0.0) + 0.0;
+            Ctl c = *cp;
+            const size_t ui = (size_t)tick * a.n_slots + m;
+            float ur_in = -1.0f;
+            if (a.noise == kNoiseStored) {
+                ur_in = a.upool[zc][ui];
+            } else if (a.noise == kNoisePropose) {
+                const uint64_t pp = (uint64_t)(int64_t)a.prop_seed[m] * jm;
+                const float v = pcg_uniform(pcg_output(pp + (uint64_t)(2 * kBatchMaxN + 1) * jg));
+                float w = a.upool[zc][ui] + a.u_step * (2.0f * v - 1.0f);
+                w = w - floorf(w);
+                ur_in = (w < 1.0f) ? w : 0.0f;
+            }
+            finalize_tick(c, n, a.threshold, gmax, S, SX, SXX, Q, ur_in);
+            if (c.resample) {
+                if (a.noise == kNoisePcgStore) a.upool[zc][ui] = c.u_raw;
+                if (a.noise == kNoisePropose) a.upool[1 - zc][ui] = c.u_raw;
+            }
+            *cp = c;
+            if (a.ring) a.ring[(size_t)(c.last.tick % a.ring_cap) * a.n_slots + m] = c.last;
+            s_flag = c.resample;
+            s_S = c.S;
+            s_u = c.u;
+            s_W = c.wrap_start;
+            s_M = gmax;
+        }
+    }
+    __syncthreads();
+    if (!s_flag) return;
+
+    // Systematic resampling into the other buffer.  Tile b owns the CDF
+    // interval (b0, b1]; its CDF is the single filter's tile CDF (one
+    // particle per lane, 64-particle group q as wave q), and every target
+    // the tile owns takes the first particle whose CDF value reaches it.
+    const double rS = 1.0 / s_S;
+    const float u = s_u;
+    const int W = s_W;
+    const float M = s_M;
+    const double fg = (M == NEG_INF) ? 0.0 : (double)expf_cw(M - M);
+    double* cw = s_c[wid];
+    float* xw = s_x[wid];
+    for (int b = wid; b < nt; b += NW) {
+        const int base = b * kBatchTile;
+        const float mb = s_tmax[b];
+        const double pex = (b == 0) ? 0.0 : s_tp[b - 1];
+        const double pin = s_tp[b];
+        const double ft = (s_tf[b] * fg) * rS;
+        const double b0 = fmin(fma(fg * pex, rS, 0.0), 1.0);
+        const double b1 = (b == kGroup - 1 || b == nt - 1) ? 1.0 : fmin(fma(fg * pin, rS, 0.0), 1.0);
+        const int last = (b == nt - 1) ? (n - 1 - base) : (kBatchTile - 1);
+        float xv[4], e[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = base + 64 * q + lane;
+            xv[q] = xb[j];
+            const float lk = lb[j];
+            e[q] = (mb == NEG_INF || j >= n) ? 0.0f : expf_cw(lk - mb);
+        }
+        double cv[4], tot4[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double tot = (double)e[q];
+            const double incl = wave_incl_scan(tot, lane);
+            tot4[q] = __shfl(incl, kWave - 1, kWave);
+            double excl = incl - tot;
+            for (int r = 0; r < q; ++r) excl += tot4[r];
+            excl = (q == 0 && lane == 0) ? 0.0 : excl;
+            double L = excl;
+            L += (double)e[q];
+            cv[q] = fmin(fma(ft, L, b0), b1);
+        }
+        double pm4[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double pm = wave_incl_max(cv[q], lane);
+            double prev = __shfl_up(pm, 1, kWave);
+            prev = (lane == 0) ? b0 : prev;
+            pm4[q] = __shfl(pm, kWave - 1, kWave);
+            for (int r = 0; r < q; ++r) prev = fmax(prev, pm4[r]);
+            const int jl = 64 * q + lane;
+            double c = fmax(cv[q], prev);
+            c = (jl == last) ? b1 : c;
+            c = (jl > last) ? __builtin_inf() : c;
+            cw[pad16(jl)] = c;
+            xw[pad16(jl)] = xv[q];
+        }
+        int r = 0;
+        if (lane == 0) r = (b == 0) ? 0 : count_le_fast(b0, u, 0, W, n);
+        else if (lane == 1) r = (b == nt - 1) ? W : count_le_fast(b1, u, 0, W, n);
+        else if (lane == 2) r = (b == 0) ? W : count_le(b0, u, W, n, n, true);
+        else if (lane == 3) r = (b == nt - 1) ? n : count_le(b1, u, W, n, n, true);
+        const int ilo = __shfl(r, 0, kWave), ihi = __shfl(r, 1, kWave);
+        const int jlo = __shfl(r, 2, kWave), jhi = __shfl(r, 3, kWave);
+        wave_lds_sync();
+        for (int i = ilo + lane; i < ihi; i += kWave) {
+            const double t = (double)target_unwrapped(u, i, n);
+            xo[i] = xw[pad16(lds_lower_bound(cw, last, t))];
+        }
+        for (int i = jlo + lane; i < jhi; i += kWave) {
+            const double t = (double)(target_unwrapped(u, i, n) - 1.0f);
+            xo[i] = xw[pad16(lds_lower_bound(cw, last, t))];
+        }
+        wave_lds_sync();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+
+hipError_t launch_batch_init(const BatchInitArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_batch_init, dim3(a.n_slots), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int BLOCK>
+static hipError_t launch_batch_tick_b(const BatchArgs& a, int n_blocks, bool student, hipStream_t st) {
+    if (student)
+        hipLaunchKernelGGL((k_batch_tick<BLOCK, true>), dim3(n_blocks), dim3(BLOCK), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_batch_tick<BLOCK, false>), dim3(n_blocks), dim3(BLOCK), 0, st, a);
+    return hipGetLastError();
+}
+
+// block: 64, 128, 256 or 512 threads per inner filter
+hipError_t launch_batch_tick(const BatchArgs& a, int n_blocks, int block, hipStream_t st) {
+    const bool student = a.obs.nu > 0.0f;
+    switch (block) {
+        case 64: return launch_batch_tick_b<64>(a, n_blocks, student, st);
+        case 128: return launch_batch_tick_b<128>(a, n_blocks, student, st);
+        case 256: return launch_batch_tick_b<256>(a, n_blocks, student, st);
+        case 512: return launch_batch_tick_b<512>(a, n_blocks, student, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace bpf

@@ -15,104 +15,9 @@
 // (gpu_bpf/gpu_bpf_ptx_full.cu:591-745).
 #include <hip/hip_runtime.h>
 
-#include "bpf_types.hpp"
+#include "bpf_common.hpp"
 
 namespace bpf {
-
-// ---------------------------------------------------------------------------
-// Wavefront / block helpers (wave64)
-// ---------------------------------------------------------------------------
-
-__device__ inline float wave_max(float v) {
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, kWave));
-    return v;
-}
-
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-    return v;
-}
-
-__device__ inline int wave_incl_scan(int v, int lane) {
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        int o = __shfl_up(v, off, kWave);
-        v += (lane >= off) ? o : 0;
-    }
-    return v;
-}
-
-__device__ inline double wave_incl_scan(double v, int lane) {
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        double o = __shfl_up(v, off, kWave);
-        v += (lane >= off) ? o : 0.0;
-    }
-    return v;
-}
-
-__device__ inline double wave_incl_max(double v, int lane) {
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        double o = __shfl_up(v, off, kWave);
-        v = (lane >= off) ? fmax(v, o) : v;
-    }
-    return v;
-}
-
-// LDS index padded by one element per 16, so lane-contiguous runs of 16
-// (stride 16 elements between lanes) fall into distinct banks.
-__device__ inline int pad16(int j) { return j + (j >> 4); }
-
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-// Streaming (non-temporal) variants for particle arrays far larger than the
-// caches: each element is read once and written once per tick.
-template <int VEC>
-__device__ inline void load_vec_nt(const float* p, float* v) {
-    if constexpr (VEC == 4) {
-        v4f t = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(p));
-        v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
-    } else {
-        for (int c = 0; c < VEC; ++c) v[c] = __builtin_nontemporal_load(p + c);
-    }
-}
-
-template <int VEC>
-__device__ inline void store_vec_nt(float* p, const float* v) {
-    if constexpr (VEC == 4) {
-        v4f t = {v[0], v[1], v[2], v[3]};
-        __builtin_nontemporal_store(t, reinterpret_cast<v4f*>(p));
-    } else {
-        for (int c = 0; c < VEC; ++c) __builtin_nontemporal_store(v[c], p + c);
-    }
-}
-
-template <int VEC>
-__device__ inline void load_vec(const float* p, float* v) {
-    if constexpr (VEC == 4) {
-        float4 t = *reinterpret_cast<const float4*>(p);
-        v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
-    } else if constexpr (VEC == 2) {
-        float2 t = *reinterpret_cast<const float2*>(p);
-        v[0] = t.x; v[1] = t.y;
-    } else {
-        v[0] = *p;
-    }
-}
-
-template <int VEC>
-__device__ inline void store_vec(float* p, const float* v) {
-    if constexpr (VEC == 4) {
-        *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
-    } else if constexpr (VEC == 2) {
-        *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
-    } else {
-        *p = v[0];
-    }
-}
 
 // ---------------------------------------------------------------------------
 // Initialisation: x_0 = mu + sd * z, z from draw 0 of each particle's stream
@@ -459,78 +364,6 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
 constexpr int kFinBlock = kGroup;
 constexpr int kFinPer = kMaxGroups / kFinBlock;   // groups per thread of the last block
 
-__device__ inline float host_pcg_float(unsigned long long* st) {
-    unsigned long long old = *st;
-    *st = old * kPcgMult + kPcgHostInc;
-    return (float)(pcg_output(old) >> 9) * (1.0f / 8388608.0f);
-}
-
-// W = first i in [0, n] with u + i/n >= 1 (FP32, gpu_bpf/gpu_bpf.cu:300-302);
-// the targets are non-decreasing in i.  Starts from i ~ (1 - u) n and
-// searches a window around it, widened to [0, n] if it does not bracket W.
-__device__ inline int wrap_start(float u, int n) {
-    long long g = (long long)ceil((1.0 - (double)u) * (double)n);
-    int lo = (int)max(0LL, min((long long)n, g - 512));
-    int hi = (int)max(0LL, min((long long)n, g + 512));
-    if (lo > 0 && target_unwrapped(u, lo - 1, n) >= 1.0f) lo = 0;
-    if (hi < n && target_unwrapped(u, hi, n) < 1.0f) hi = n;
-    while (lo < hi) {
-        int mid = (int)(((unsigned)lo + (unsigned)hi) >> 1);
-        if (target_unwrapped(u, mid, n) >= 1.0f) hi = mid; else lo = mid + 1;
-    }
-    return lo;
-}
-
-// The tick's statistics and decisions from the global sums (one thread, on
-// a register copy of the control block).
-__device__ inline void finalize_tick(Ctl& c, const FinArgs& a, float M, double S, double SX,
-                                     double SXX, double Q) {
-    const int carry = c.carry;
-    const double n = (double)a.n;
-    const double Md = (double)M;
-    BpfTickStats r;
-    r.tick = c.tick;
-    const bool ok = isfinite(S) && S > 0.0 && isfinite(Md);
-    if (ok) {
-        r.mean = SX / S;
-        r.var = SXX / S - r.mean * r.mean;
-        r.ess = (S * S) / Q;
-        r.ll_ref = c.off + Md + log(fmax(S / n, (double)bits_f32(0x0D6BF94Eu)));
-        r.ll_incr = carry ? (Md + log(S)) - log(c.s_prev) : Md + log(S / n);
-    } else {
-        const double qn = __builtin_nan("");
-        r.mean = r.var = r.ess = r.ll_ref = r.ll_incr = qn;
-    }
-    c.ll_cum += r.ll_incr;
-    r.ll_cum = c.ll_cum;
-    r.h_mean = (float)r.mean;
-    r.log_lik = (float)r.ll_ref;
-    const int resample = ok && (a.threshold <= 0.0f || (r.ess / n) < (double)a.threshold);
-    r.resampled = resample;
-    if (resample) {
-        const float u = host_pcg_float(&c.host_rng) / (float)a.n;
-        c.u = u;
-        c.wrap_start = wrap_start(u, a.n);
-        c.src = c.cur;
-        c.dst = 1 - c.cur;
-        c.cur = c.dst;
-        c.carry = 0;
-        c.off = 0.0;
-        c.resample_count += 1;
-        c.tile_next = 0;
-    } else {
-        c.carry = 1;
-        c.off += Md;
-        c.s_prev = S;
-    }
-    c.m_prev = M;
-    c.M = M;
-    c.S = S;
-    c.resample = resample;
-    c.last = r;
-    c.tick += 1;
-}
-
 __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
     constexpr int NW = kFinBlock / kWave;
     __shared__ float s_f[NW];
@@ -616,7 +449,7 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
             for (int w = 1; w < NW; ++w) acc += s_d[q][w];
             t4[q] = acc;
         }
-        finalize_tick(c, a, M, t4[0], t4[1], t4[2], t4[3]);
+        finalize_tick(c, a.n, a.threshold, M, t4[0], t4[1], t4[2], t4[3]);
         *a.ctl = c;
         a.ring[c.last.tick % a.ring_cap] = c.last;
         if (c.resample) {
@@ -677,42 +510,6 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
 // (gpu_bpf/gpu_bpf.cu:297-310, clamp to N-1 as ptx:816-830) on this CDF.
 // ---------------------------------------------------------------------------
 
-__device__ inline int count_le(double v, float u, int lo, int hi, int n, bool wrapped) {
-    while (lo < hi) {
-        int mid = (int)(((unsigned)lo + (unsigned)hi) >> 1);
-        float t = target_unwrapped(u, mid, n);
-        if (wrapped) t -= 1.0f;
-        if ((double)t <= v) lo = mid + 1; else hi = mid;
-    }
-    return lo;
-}
-
-// Same count for unwrapped targets, from the estimate i ~ (v - u) N and an
-// exact correction with the float target rule; falls back to the binary
-// search if the estimate is off by more than a few (it is off by at most
-// about N * 2^-24 positions, so this does not happen below ~2^28 particles).
-__device__ inline int count_le_fast(double v, float u, int lo, int hi, int n) {
-    if (lo >= hi) return lo;
-    double est = floor((v - (double)u) * (double)n) + 1.0;
-    est = fmin(fmax(est, (double)lo), (double)hi);
-    int g = (int)est;
-    int steps = 0;
-    while (g > lo && (double)target_unwrapped(u, g - 1, n) > v) {
-        --g;
-        if (++steps > 8) return count_le(v, u, lo, hi, n, false);
-    }
-    while (g < hi && (double)target_unwrapped(u, g, n) <= v) {
-        ++g;
-        if (++steps > 16) return count_le(v, u, lo, hi, n, false);
-    }
-    return g;
-}
-
-// Builds the tile's CDF in s_c (TILE doubles) from the lane's weights e[K]
-// (coalesced layout) and returns the tile's output ranges in s_rng:
-// [0,1) unwrapped targets, [2,3) wrapped targets.  Returns the local index of
-// the tile's last valid particle.  s_c doubles as the staging area of e.
-// Contains block barriers; every thread of the block must call it.
 template <int BLOCK, int K>
 __device__ inline int tile_cdf(const float (&e)[K], int b, int n, int n_tiles, double ft,
                                double Bb, double Bb1, float u, int W, double* s_c,
@@ -817,24 +614,6 @@ __device__ inline TileScale tile_scale(const Partials& p, const double* gptil, f
 }
 
 // lower_bound over the padded in-tile CDF c[pad16(0..last)].
-__device__ inline int lds_lower_bound(const double* c, int last, double t) {
-    int lo = 0, hi = last;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (c[pad16(mid)] < t) lo = mid + 1; else hi = mid;
-    }
-    return lo;
-}
-
-__device__ inline int wave_incl_max_int(int v, int lane) {
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        int o = __shfl_up(v, off, kWave);
-        v = (lane >= off) ? max(v, o) : v;
-    }
-    return v;
-}
-
 // Children runs longer than this are written by the whole block, so that a
 // heavy parent (degenerate weights) cannot hold one lane for the tile; up to
 // kLongRuns of them per tile (further ones are written by their lanes).

@@ -32,6 +32,7 @@ struct Ctl {
     float m_prev;        // max of the log-weights carried into the next tick
     float u;             // resampling offset u / N of the latest resample
     float M;             // global max log-weight of the latest tick
+    float u_raw;         // uniform behind u (u = u_raw / N)
     double S;            // sum exp(l - M) of the latest tick
     double s_prev;       // sum exp(l - m_prev) of the carried weights
     double off;          // raw log-weight offset (reference log_lik semantics)
@@ -133,6 +134,71 @@ struct InitArgs {
     uint64_t pcg_a, pcg_g;
     float mu, sd;
     int n, n_pad;
+};
+
+// ---------------------------------------------------------------------------
+// Batched inner filters: one workgroup runs one filter's whole tick.  Filters
+// hold at most kBatchMaxN particles, i.e. at most 64 tiles of 256 -- the
+// single filter's own tiles at these sizes (one particle per lane), so each
+// inner filter is one reduction group and reproduces the single filter.
+// ---------------------------------------------------------------------------
+constexpr int kBatchTile = 256;
+constexpr int kBatchMaxTiles = 64;
+constexpr int kBatchMaxN = kBatchTile * kBatchMaxTiles;
+
+// Noise source of a batched tick: the PCG32 streams, the streams with the
+// normals and the resampling uniform stored for later replays, the stored
+// noise, or a Crank-Nicolson proposal around the stored noise (written to
+// the slot's other noise pool).
+enum BatchNoise { kNoisePcg = 0, kNoisePcgStore = 1, kNoiseStored = 2, kNoisePropose = 3 };
+
+struct BatchArgs {
+    const int* order;        // slot run by each workgroup (largest filters first)
+    const int* n;            // particles of each slot's filter
+    const int* seed;         // PCG32 seed of each slot (device streams, host stream)
+    const float* mu;
+    const float* phi;
+    const float* sig;
+    Ctl* ctl;                // one control block per slot
+    float* x0;               // particles of slot m at m * cap, ping-pong by ctl[m].cur
+    float* x1;
+    float* lw;               // log-weights of slot m at m * cap
+    BpfTickStats* ring;      // ring[(tick % ring_cap) * n_slots + m], or null
+    int ring_cap;
+    int cap;
+    int n_slots;
+    const float* y;          // observation series in device memory, indexed by tick
+    ObsModel obs;
+    float threshold;
+    // The tick t is the slot's ctl[m].tick: draw t of the streams (jump
+    // constants computed on the device), y[t], noise-pool row t.  Nothing in
+    // the arguments changes from tick to tick, so a captured launch replays.
+    // noise pools (kNoisePcgStore / kNoiseStored / kNoisePropose)
+    int noise;
+    float* zpool[2];         // normals of slot m, tick t: (t * n_slots + m) * cap + i
+    float* upool[2];         // resampling uniform of slot m, tick t: t * n_slots + m
+    const int* zcur;         // per slot: pool holding its current noise
+    float cn_rho;            // proposal: z' = rho z + sqrt(1 - rho^2) xi
+    float cn_s;              // sqrt(1 - rho^2)
+    float u_step;            // proposal: u' = frac(u + u_step (2 v - 1))
+    const int* prop_seed;    // per slot: seed of the proposal streams xi, v
+};
+
+struct BatchInitArgs {
+    const int* n;
+    const int* seed;
+    const float* mu;
+    const float* phi;
+    const float* sig;
+    float* x0;
+    int cap;
+    int n_slots;
+    uint64_t jm, jg;         // draw 0
+    int noise;
+    float* zpool[2];
+    const int* zcur;
+    float cn_rho, cn_s;
+    const int* prop_seed;
 };
 
 }  // namespace bpf

@@ -1,0 +1,412 @@
+// gpu_bpf_batch_api.hip -- host runtime behind include/gpu_bpf_batch.h.
+//
+// A batch holds M inner filters in slots of `cap` particles (cap = the
+// largest particle count rounded up to a tile), one control block per slot,
+// and the observation history in device memory.  A tick is one launch of
+// k_batch_tick whose arguments do not depend on the tick (each workgroup
+// reads its slot's tick from the control block), so runs of ticks replay one
+// instantiated HIP graph of kGraphTicks launches.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "../../include/gpu_bpf_batch.h"
+#include "api_util.hpp"
+#include "bpf_types.hpp"
+
+namespace bpf {
+hipError_t launch_batch_init(const BatchInitArgs& a, hipStream_t st);
+hipError_t launch_batch_tick(const BatchArgs& a, int n_blocks, int block, hipStream_t st);
+}  // namespace bpf
+
+using namespace bpf;
+
+static_assert(GPU_BPF_BATCH_MAX_N == kBatchMaxN, "header and kernels disagree on the largest filter");
+
+namespace {
+
+constexpr int kGraphTicks = 32;        // launches per captured graph
+constexpr int kBlocks[4] = {64, 128, 256, 512};
+
+double c_obs_student(double nu) {
+    return lgamma((nu + 1.0) / 2.0) - lgamma(nu / 2.0) - 0.5 * log(nu * 3.14159265358979323846);
+}
+
+}  // namespace
+
+struct GpuBpfBatch {
+    int M = 0, cap = 0, block = 0;
+    float threshold = 0.0f;
+    ObsModel obs{};
+    hipStream_t stream = nullptr;
+    std::vector<int> n, seed, order;
+    std::vector<float> mu, phi, sig;
+    int* d_order = nullptr;
+    int* d_n = nullptr;
+    int* d_seed = nullptr;
+    float* d_mu = nullptr;
+    float* d_phi = nullptr;
+    float* d_sig = nullptr;
+    Ctl* d_ctl = nullptr;
+    float* d_x[2] = {nullptr, nullptr};
+    float* d_l = nullptr;
+    BpfTickStats* d_ring = nullptr;
+    int ring_cap = 1024;
+    float* d_y = nullptr;              // observation history, indexed by tick
+    int y_cap = 0;
+    int timestep = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_block = 0;
+};
+
+static void free_graph(GpuBpfBatch* b) {
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+}
+
+static void destroy_batch(GpuBpfBatch* b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    free_graph(b);
+    void* ptrs[] = {b->d_order, b->d_n, b->d_seed, b->d_mu, b->d_phi, b->d_sig, b->d_ctl,
+                    b->d_x[0], b->d_x[1], b->d_l, b->d_ring, b->d_y};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+static BatchArgs tick_args(const GpuBpfBatch* b) {
+    BatchArgs a;
+    memset((void*)&a, 0, sizeof a);
+    a.order = b->d_order;
+    a.n = b->d_n;
+    a.seed = b->d_seed;
+    a.mu = b->d_mu;
+    a.phi = b->d_phi;
+    a.sig = b->d_sig;
+    a.ctl = b->d_ctl;
+    a.x0 = b->d_x[0];
+    a.x1 = b->d_x[1];
+    a.lw = b->d_l;
+    a.ring = b->d_ring;
+    a.ring_cap = b->ring_cap;
+    a.cap = b->cap;
+    a.n_slots = b->M;
+    a.y = b->d_y;
+    a.obs = b->obs;
+    a.threshold = b->threshold;
+    a.noise = kNoisePcg;
+    return a;
+}
+
+static int create_impl(GpuBpfBatch* b) {
+    HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    const int M = b->M;
+    const size_t slots = (size_t)M * b->cap;
+    HIP_TRY(hipMalloc(&b->d_order, M * sizeof(int)));
+    HIP_TRY(hipMalloc(&b->d_n, M * sizeof(int)));
+    HIP_TRY(hipMalloc(&b->d_seed, M * sizeof(int)));
+    HIP_TRY(hipMalloc(&b->d_mu, M * sizeof(float)));
+    HIP_TRY(hipMalloc(&b->d_phi, M * sizeof(float)));
+    HIP_TRY(hipMalloc(&b->d_sig, M * sizeof(float)));
+    HIP_TRY(hipMalloc(&b->d_ctl, M * sizeof(Ctl)));
+    HIP_TRY(hipMalloc(&b->d_x[0], slots * sizeof(float)));
+    HIP_TRY(hipMalloc(&b->d_x[1], slots * sizeof(float)));
+    HIP_TRY(hipMalloc(&b->d_l, slots * sizeof(float)));
+    HIP_TRY(hipMalloc(&b->d_ring, (size_t)b->ring_cap * M * sizeof(BpfTickStats)));
+    HIP_TRY(hipMemsetAsync(b->d_x[1], 0, slots * sizeof(float), b->stream));
+    HIP_TRY(hipMemsetAsync(b->d_l, 0, slots * sizeof(float), b->stream));
+    HIP_TRY(hipMemcpyAsync(b->d_order, b->order.data(), M * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(b->d_n, b->n.data(), M * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(b->d_seed, b->seed.data(), M * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(b->d_mu, b->mu.data(), M * sizeof(float), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(b->d_phi, b->phi.data(), M * sizeof(float), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(b->d_sig, b->sig.data(), M * sizeof(float), hipMemcpyHostToDevice, b->stream));
+    std::vector<Ctl> c(M);
+    memset((void*)c.data(), 0, M * sizeof(Ctl));
+    for (int m = 0; m < M; ++m) {
+        c[m].host_rng = (unsigned long long)(long long)b->seed[m] * 67890ULL + 12345ULL;
+        c[m].s_prev = 1.0;
+    }
+    HIP_TRY(hipMemcpyAsync(b->d_ctl, c.data(), M * sizeof(Ctl), hipMemcpyHostToDevice, b->stream));
+    BatchInitArgs ia;
+    memset((void*)&ia, 0, sizeof ia);
+    ia.n = b->d_n;
+    ia.seed = b->d_seed;
+    ia.mu = b->d_mu;
+    ia.phi = b->d_phi;
+    ia.sig = b->d_sig;
+    ia.x0 = b->d_x[0];
+    ia.cap = b->cap;
+    ia.n_slots = M;
+    ia.jm = kPcgMult * kPcgMult;       // draw 0
+    ia.jg = 1u + kPcgMult;
+    ia.noise = kNoisePcg;
+    HIP_TRY(launch_batch_init(ia, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+// Appends n observations to the device history (grows by doubling).
+static int append_y(GpuBpfBatch* b, const float* y, int n_ticks, hipMemcpyKind kind) {
+    const int need = b->timestep + n_ticks;
+    if (need > b->y_cap) {
+        int cap = std::max(need, std::max(1024, 2 * b->y_cap));
+        float* ny = nullptr;
+        HIP_TRY(hipMalloc(&ny, (size_t)cap * sizeof(float)));
+        if (b->d_y) {
+            HIP_TRY(hipMemcpyAsync(ny, b->d_y, (size_t)b->timestep * sizeof(float),
+                                   hipMemcpyDeviceToDevice, b->stream));
+            HIP_TRY(hipStreamSynchronize(b->stream));
+            (void)hipFree(b->d_y);
+        }
+        b->d_y = ny;
+        b->y_cap = cap;
+        free_graph(b);                  // the captured launches hold the old pointer
+    }
+    HIP_TRY(hipMemcpyAsync(b->d_y + b->timestep, y, (size_t)n_ticks * sizeof(float), kind, b->stream));
+    return 0;
+}
+
+static int capture_graph(GpuBpfBatch* b) {
+    free_graph(b);
+    const BatchArgs a = tick_args(b);
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < kGraphTicks && e == hipSuccess; ++k) e = launch_batch_tick(a, b->M, b->block, b->stream);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        set_err("batch graph capture failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+        return -1;
+    }
+    b->graph = g;
+    HIP_TRY(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+    b->graph_block = b->block;
+    return 0;
+}
+
+static int run_ticks(GpuBpfBatch* b, int n_ticks) {
+    if (b->block == 0) b->block = 256;
+    if (n_ticks >= kGraphTicks && (!b->graph_exec || b->graph_block != b->block)) {
+        if (capture_graph(b)) return -1;
+    }
+    int done = 0;
+    while (n_ticks - done >= kGraphTicks) {
+        HIP_TRY(hipGraphLaunch(b->graph_exec, b->stream));
+        done += kGraphTicks;
+    }
+    const BatchArgs a = tick_args(b);
+    for (; done < n_ticks; ++done) HIP_TRY(launch_batch_tick(a, b->M, b->block, b->stream));
+    b->timestep += n_ticks;
+    return 0;
+}
+
+static GpuBpfBatch* new_batch(int M, const int* n, const float* mu, const float* phi, const float* sig,
+                              const int* seeds, float nu_obs, float thr) {
+    GpuBpfBatch* b = new GpuBpfBatch();
+    b->M = M;
+    b->n.assign(n, n + M);
+    b->mu.assign(mu, mu + M);
+    b->phi.assign(phi, phi + M);
+    b->sig.assign(sig, sig + M);
+    b->seed.assign(seeds, seeds + M);
+    b->threshold = thr;
+    const int nmax = *std::max_element(b->n.begin(), b->n.end());
+    b->cap = (nmax + kBatchTile - 1) / kBatchTile * kBatchTile;
+    b->order.resize(M);
+    std::iota(b->order.begin(), b->order.end(), 0);
+    std::stable_sort(b->order.begin(), b->order.end(), [&](int i, int j) { return b->n[i] > b->n[j]; });
+    const float nu = nu_obs > 0.0f ? nu_obs : 0.0f;
+    b->obs.nu = nu;
+    b->obs.c_obs = nu > 0.0f ? (float)c_obs_student((double)nu) : 0.0f;
+    b->obs.nph = (nu + 1.0f) * 0.5f;
+    return b;
+}
+
+extern "C" {
+
+GpuBpfBatch* gpu_bpf_batch_create(int n_filters, const int* n_particles, const float* mu,
+                                  const float* phi, const float* sigma_z, const int* seeds,
+                                  float nu_obs, float ess_threshold) {
+    clear_err();
+    if (n_filters <= 0 || !n_particles || !mu || !phi || !sigma_z || !seeds) {
+        set_err("gpu_bpf_batch_create: n_filters must be positive and every array given");
+        return nullptr;
+    }
+    for (int m = 0; m < n_filters; ++m) {
+        if (n_particles[m] <= 0 || n_particles[m] > kBatchMaxN) {
+            set_err("gpu_bpf_batch_create: filter %d has %d particles (1..%d)", m, n_particles[m],
+                    kBatchMaxN);
+            return nullptr;
+        }
+    }
+    GpuBpfBatch* b = new_batch(n_filters, n_particles, mu, phi, sigma_z, seeds, nu_obs, ess_threshold);
+    if (create_impl(b)) {
+        destroy_batch(b);
+        return nullptr;
+    }
+    return b;
+}
+
+void gpu_bpf_batch_destroy(GpuBpfBatch* b) { destroy_batch(b); }
+
+int gpu_bpf_batch_set_block(GpuBpfBatch* b, int threads) {
+    if (!b) {
+        set_err("gpu_bpf_batch_set_block: null batch");
+        return -1;
+    }
+    if (threads != 0 && threads != 64 && threads != 128 && threads != 256 && threads != 512) {
+        set_err("gpu_bpf_batch_set_block: %d is not 0, 64, 128, 256 or 512", threads);
+        return -1;
+    }
+    b->block = threads;
+    return 0;
+}
+
+int gpu_bpf_batch_get_block(GpuBpfBatch* b) { return b ? b->block : -1; }
+
+int gpu_bpf_batch_autotune(GpuBpfBatch* b, const float* y, int ticks) {
+    if (!b || !y || ticks <= 0) {
+        set_err("gpu_bpf_batch_autotune: bad arguments");
+        return -1;
+    }
+    float best_ms = 0.0f;
+    int best = 0;
+    for (int blk : kBlocks) {
+        GpuBpfBatch* s = new_batch(b->M, b->n.data(), b->mu.data(), b->phi.data(), b->sig.data(),
+                                   b->seed.data(), b->obs.nu, b->threshold);
+        if (create_impl(s)) {
+            destroy_batch(s);
+            return -1;
+        }
+        s->block = blk;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        float ms = 0.0f;
+        int rc = -1;
+        if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
+            append_y(s, y, ticks, hipMemcpyHostToDevice) == 0 && run_ticks(s, 2) == 0 &&
+            hipEventRecord(e0, s->stream) == hipSuccess && run_ticks(s, ticks - 2 > 0 ? ticks - 2 : 1) == 0 &&
+            hipEventRecord(e1, s->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+            hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+            rc = 0;
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        destroy_batch(s);
+        if (rc) {
+            set_err("gpu_bpf_batch_autotune: timing run failed");
+            return -1;
+        }
+        if (best == 0 || ms < best_ms) {
+            best_ms = ms;
+            best = blk;
+        }
+    }
+    b->block = best;
+    return best;
+}
+
+int gpu_bpf_batch_run(GpuBpfBatch* b, const float* y, int n_ticks) {
+    if (!b || (!y && n_ticks > 0) || n_ticks < 0) {
+        set_err("gpu_bpf_batch_run: bad arguments");
+        return -1;
+    }
+    if (n_ticks == 0) return 0;
+    if (append_y(b, y, n_ticks, hipMemcpyHostToDevice)) return -1;
+    return run_ticks(b, n_ticks);
+}
+
+int gpu_bpf_batch_run_device(GpuBpfBatch* b, const float* y_dev, int n_ticks) {
+    if (!b || (!y_dev && n_ticks > 0) || n_ticks < 0) {
+        set_err("gpu_bpf_batch_run_device: bad arguments");
+        return -1;
+    }
+    if (n_ticks == 0) return 0;
+    if (append_y(b, y_dev, n_ticks, hipMemcpyDeviceToDevice)) return -1;
+    return run_ticks(b, n_ticks);
+}
+
+int gpu_bpf_batch_synchronize(GpuBpfBatch* b) {
+    if (!b) {
+        set_err("gpu_bpf_batch_synchronize: null batch");
+        return -1;
+    }
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int gpu_bpf_batch_get_timestep(GpuBpfBatch* b) { return b ? b->timestep : -1; }
+
+int gpu_bpf_batch_copy_stats(GpuBpfBatch* b, BpfTickStats* out, int n_ticks) {
+    if (!b || !out || n_ticks < 0) {
+        set_err("gpu_bpf_batch_copy_stats: bad arguments");
+        return -1;
+    }
+    if (n_ticks > b->timestep || n_ticks > b->ring_cap) {
+        set_err("gpu_bpf_batch_copy_stats: %d ticks requested, %d available", n_ticks,
+                std::min(b->timestep, b->ring_cap));
+        return -1;
+    }
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    const size_t row = (size_t)b->M * sizeof(BpfTickStats);
+    for (int k = 0; k < n_ticks; ++k) {
+        const int t = b->timestep - n_ticks + k;
+        HIP_TRY(hipMemcpy((char*)out + (size_t)k * row, (const char*)b->d_ring + (size_t)(t % b->ring_cap) * row,
+                          row, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+static int copy_ctl(GpuBpfBatch* b, std::vector<Ctl>& c) {
+    c.resize(b->M);
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipMemcpy(c.data(), b->d_ctl, (size_t)b->M * sizeof(Ctl), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int gpu_bpf_batch_get_loglik(GpuBpfBatch* b, double* ll) {
+    if (!b || !ll) {
+        set_err("gpu_bpf_batch_get_loglik: bad arguments");
+        return -1;
+    }
+    std::vector<Ctl> c;
+    if (copy_ctl(b, c)) return -1;
+    for (int m = 0; m < b->M; ++m) ll[m] = c[m].ll_cum;
+    return 0;
+}
+
+int gpu_bpf_batch_get_resample_counts(GpuBpfBatch* b, int* counts) {
+    if (!b || !counts) {
+        set_err("gpu_bpf_batch_get_resample_counts: bad arguments");
+        return -1;
+    }
+    std::vector<Ctl> c;
+    if (copy_ctl(b, c)) return -1;
+    for (int m = 0; m < b->M; ++m) counts[m] = c[m].resample_count;
+    return 0;
+}
+
+int gpu_bpf_batch_copy_particles(GpuBpfBatch* b, int filter, float* x) {
+    if (!b || !x || filter < 0 || filter >= b->M) {
+        set_err("gpu_bpf_batch_copy_particles: bad arguments");
+        return -1;
+    }
+    std::vector<Ctl> c;
+    if (copy_ctl(b, c)) return -1;
+    const float* src = b->d_x[c[filter].cur] + (size_t)filter * b->cap;
+    HIP_TRY(hipMemcpy(x, src, (size_t)b->n[filter] * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+}  // extern "C"

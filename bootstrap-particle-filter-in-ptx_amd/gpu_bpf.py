@@ -48,10 +48,10 @@ _lib = None
 
 
 def header_symbols(path=HEADER_PATH):
-    """Function names declared in include/gpu_bpf.h."""
+    """Function names declared in a boundary header (default include/gpu_bpf.h)."""
     text = open(path).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(gpu_bpf_[a-z_0-9]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b((?:gpu_bpf|gpu_smc2)_[a-z_0-9]+)\s*\(", text)))
 
 
 def load(path=LIB_PATH):

@@ -23,16 +23,39 @@ def test_header_declares_reference_surface():
     assert ref <= syms
 
 
-def test_library_exports_every_declared_symbol():
+HEADERS = ("gpu_bpf.h", "gpu_bpf_batch.h")
+
+
+@pytest.mark.parametrize("header", HEADERS)
+def test_library_exports_every_declared_symbol(header):
     if not os.path.exists(gpu_bpf.LIB_PATH):
         pytest.fail(f"{gpu_bpf.LIB_PATH} not built (run __graft_entry__.build())")
+    path = os.path.join(gpu_bpf.REPO_DIR, "include", header)
+    syms = gpu_bpf.header_symbols(path)
+    assert syms
     lib = ctypes.CDLL(gpu_bpf.LIB_PATH)
-    missing = [s for s in gpu_bpf.header_symbols() if not hasattr(lib, s)]
+    missing = [s for s in syms if not hasattr(lib, s)]
     assert not missing, missing
     out = subprocess.run(["nm", "-D", "--defined-only", gpu_bpf.LIB_PATH],
                          capture_output=True, text=True, check=True).stdout
     exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
-    assert set(gpu_bpf.header_symbols()) <= exported
+    assert set(syms) <= exported
+
+
+def test_batch_header_is_plain_c():
+    src = '#include "gpu_bpf_batch.h"\nint main(void){GpuBpfBatch* b = 0; (void)b; return 0;}\n'
+    inc = os.path.join(gpu_bpf.REPO_DIR, "include")
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", inc, "-x", "c", "-",
+                        "-o", "/dev/null"], input=src, text=True, capture_output=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_batch_mirror_declares_every_symbol():
+    import gpu_bpf_batch
+    path = os.path.join(gpu_bpf.REPO_DIR, "include", "gpu_bpf_batch.h")
+    src = open(gpu_bpf_batch.__file__).read()
+    missing = [s for s in gpu_bpf.header_symbols(path) if f'"{s}"' not in src]
+    assert not missing, missing
 
 
 def test_header_is_plain_c():

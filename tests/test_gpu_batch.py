@@ -1,0 +1,143 @@
+"""Batched inner filters against the plain filter, through the C ABI.
+
+Inner filter m of a batch must be the plain filter created with the same
+(N_m, phi_m, sigma_m, mu_m, seed_m) and ESS threshold: particles bit-exact,
+ancestors therefore equal, FP64 statistics and log-likelihood to 1e-12
+relative (the same arithmetic in the same order; only the FP64 group sums
+differ by additions of exact zeros).  M = 1 reduces to the plain filter.
+"""
+import numpy as np
+import pytest
+
+import gpu_bpf
+import gpu_bpf_batch
+
+pytestmark = pytest.mark.gpu
+
+STAT_FIELDS = ("mean", "var", "ess", "ll_incr", "ll_cum", "ll_ref")
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ydata(orc):
+    _, y = orc.dgp(400, 0.97, 0.15, -1.0, seed=42)
+    return y.astype(np.float32)
+
+
+def filters(M, rng, nmin=256, nmax=16384):
+    n = np.exp(rng.uniform(np.log(nmin), np.log(nmax), M)).round().astype(np.int32)
+    mu = rng.uniform(-2.0, 0.0, M).astype(np.float32)
+    phi = rng.uniform(0.90, 0.995, M).astype(np.float32)
+    sig = rng.uniform(0.05, 0.40, M).astype(np.float32)
+    seeds = rng.integers(1, 1 << 30, M).astype(np.int32)
+    return n, mu, phi, sig, seeds
+
+
+def plain(n, mu, phi, sig, seed, y, thr, nu=0.0):
+    f = gpu_bpf.GpuBpf(int(n), float(phi), float(sig), float(mu), nu_obs=nu, seed=int(seed),
+                       ess_threshold=thr)
+    _, st = f.run(y)
+    x = f.particles()
+    cnt = f.resample_count()
+    f.close()
+    return st, x, cnt
+
+
+def assert_same(st_b, st_p, rtol=1e-12):
+    np.testing.assert_array_equal(st_b["resampled"], st_p["resampled"])
+    for k in STAT_FIELDS:
+        a, b = st_b[k], st_p[k]
+        fin = np.isfinite(b)
+        np.testing.assert_array_equal(np.isfinite(a), fin, err_msg=k)
+        err = np.max(np.abs(a[fin] - b[fin]) / np.maximum(np.abs(b[fin]), 1.0)) if fin.any() else 0
+        assert err <= rtol, (k, err)
+    np.testing.assert_array_equal(st_b["h_mean"], st_p["h_mean"])
+
+
+@pytest.mark.parametrize("thr", [0.0, 0.5])
+def test_batch_equals_looped_plain_filters(ydata, thr):
+    rng = np.random.default_rng(3)
+    n, mu, phi, sig, seeds = filters(12, rng)
+    n[:3] = [256, 257, 16384]          # one tile, a padded second tile, the largest filter
+    T = 70                             # two captured 32-tick graphs and eager ticks
+    y = ydata[:T]
+    say("batch vs plain", thr, n.tolist())
+    b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, ess_threshold=thr, block=256)
+    st = b.run(y)
+    counts = b.resample_counts()
+    ll = b.loglik()
+    for m in range(len(n)):
+        stp, xp, cp = plain(n[m], mu[m], phi[m], sig[m], seeds[m], y, thr)
+        assert_same(st[:, m], stp)
+        np.testing.assert_array_equal(b.particles(m), xp)
+        assert counts[m] == cp
+        assert ll[m] == st["ll_cum"][-1, m]
+    b.close()
+
+
+@pytest.mark.parametrize("block", [64, 128, 512])
+def test_block_sizes_give_identical_results(ydata, block):
+    rng = np.random.default_rng(5)
+    n, mu, phi, sig, seeds = filters(9, rng)
+    y = ydata[:40]
+    say("block", block)
+    ref = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, ess_threshold=0.5, block=256)
+    sr = ref.run(y)
+    b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, ess_threshold=0.5, block=block)
+    sb = b.run(y)
+    for k in STAT_FIELDS + ("resampled",):
+        np.testing.assert_array_equal(sb[k], sr[k], err_msg=k)
+    for m in range(len(n)):
+        np.testing.assert_array_equal(b.particles(m), ref.particles(m))
+    b.close()
+    ref.close()
+
+
+def test_student_t_batch(ydata):
+    rng = np.random.default_rng(7)
+    n, mu, phi, sig, seeds = filters(5, rng, 300, 5000)
+    y = ydata[:50]
+    say("student-t batch")
+    b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=5.0, ess_threshold=0.0)
+    st = b.run(y)
+    for m in range(len(n)):
+        stp, xp, _ = plain(n[m], mu[m], phi[m], sig[m], seeds[m], y, 0.0, nu=5.0)
+        assert_same(st[:, m], stp)
+        np.testing.assert_array_equal(b.particles(m), xp)
+    b.close()
+
+
+def test_single_filter_batch_is_the_plain_filter(ydata):
+    y = ydata[:100]
+    say("M=1")
+    b = gpu_bpf_batch.GpuBpfBatch([4096], [-1.0], [0.97], [0.15], [42], ess_threshold=0.5)
+    st = b.run(y[:37])
+    st2 = b.run(y[37:])
+    stp, xp, cp = plain(4096, -1.0, 0.97, 0.15, 42, y, 0.5)
+    assert_same(np.concatenate([st[:, 0], st2[:, 0]]), stp)
+    np.testing.assert_array_equal(b.particles(0), xp)
+    assert b.resample_counts()[0] == cp
+    assert b.timestep() == 100
+    b.close()
+
+
+def test_autotune_picks_a_block(ydata):
+    rng = np.random.default_rng(9)
+    n, mu, phi, sig, seeds = filters(64, rng)
+    say("autotune")
+    b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds)
+    blk = b.autotune(ydata, 40)
+    assert blk in (64, 128, 256, 512) and b.block == blk
+    b.run(ydata[:10], stats=False)
+    assert b.timestep() == 10
+    b.close()
+
+
+def test_batch_errors():
+    with pytest.raises(gpu_bpf.GpuBpfError, match="particles"):
+        gpu_bpf_batch.GpuBpfBatch([100, 20000], 0.0, 0.9, 0.1, [1, 2])
+    with pytest.raises(gpu_bpf.GpuBpfError, match="set_block"):
+        gpu_bpf_batch.GpuBpfBatch([100], 0.0, 0.9, 0.1, [1], block=96)
