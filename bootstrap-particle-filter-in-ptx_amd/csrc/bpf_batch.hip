@@ -166,6 +166,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
     __shared__ float s_u, s_M;
     __shared__ double s_S;
 
+    if (a.halt && *a.halt) return;       // the outer layer stopped the run of ticks
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const int m = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
     const int n = a.n[m];
@@ -305,6 +306,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
             }
             *cp = c;
             if (a.ring) a.ring[(size_t)(c.last.tick % a.ring_cap) * a.n_slots + m] = c.last;
+            if (a.outer_logw) a.outer_logw[m] += c.last.ll_incr;
             s_flag = c.resample;
             s_S = c.S;
             s_u = c.u;
@@ -392,8 +394,134 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// SMC^2 outer layer
+// ---------------------------------------------------------------------------
+
+// Outer ESS of the slots' log-weights after a tick (one block of 1024):
+// ESS = (sum w)^2 / sum w^2, w = exp(logw - max), in FP64.  Below
+// threshold * M (or non-finite) it raises the halt flag and records the tick.
+constexpr int kOuterBlock = 1024;
+
+__global__ __launch_bounds__(kOuterBlock) void k_outer_check(const double* logw, const Ctl* ctl, int M,
+                                                             float threshold, int* halt, double* ess_out) {
+    constexpr int NW = kOuterBlock / kWave;
+    __shared__ double s_a[NW], s_b[NW];
+    if (*halt) return;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    double mx = -__builtin_inf();
+    for (int m = tid; m < M; m += kOuterBlock) mx = fmax(mx, logw[m]);
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, kWave));
+    if (lane == 0) s_a[wid] = mx;
+    __syncthreads();
+    mx = s_a[0];
+    for (int w = 1; w < NW; ++w) mx = fmax(mx, s_a[w]);
+    __syncthreads();
+    double S = 0.0, Q = 0.0;
+    for (int m = tid; m < M; m += kOuterBlock) {
+        const double w = exp(logw[m] - mx);
+        S += w;
+        Q += w * w;
+    }
+    S = wave_sum(S);
+    Q = wave_sum(Q);
+    if (lane == 0) {
+        s_a[wid] = S;
+        s_b[wid] = Q;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double St = s_a[0], Qt = s_b[0];
+        for (int w = 1; w < NW; ++w) {
+            St += s_a[w];
+            Qt += s_b[w];
+        }
+        const double ess = (St * St) / Qt;
+        *ess_out = ess;
+        if (!(ess >= (double)threshold * (double)M)) {
+            halt[0] = 1;
+            halt[1] = ctl[0].tick;
+        }
+    }
+}
+
+// Slot m receives slot anc[m]'s whole inner filter (GatherArgs).
+__global__ __launch_bounds__(kBlock) void k_smc2_gather(GatherArgs a) {
+    const int m = blockIdx.x, src = a.anc[m];
+    const int n = a.n_src[src];
+    const Ctl c = a.ctl_src[src];
+    if (threadIdx.x == 0) {
+        a.n_dst[m] = n;
+        a.mu_dst[m] = a.mu_src[src];
+        a.phi_dst[m] = a.phi_src[src];
+        a.sig_dst[m] = a.sig_src[src];
+        Ctl d = c;
+        d.cur = 0;
+        a.ctl_dst[m] = d;
+    }
+    const float* xs = a.x_src[c.cur] + (size_t)src * a.cap;
+    const float* ls = a.l_src + (size_t)src * a.cap;
+    float* xd = a.x_dst + (size_t)m * a.cap;
+    float* ld = a.l_dst + (size_t)m * a.cap;
+    for (int i = threadIdx.x; i < a.cap; i += kBlock) {
+        xd[i] = xs[i];
+        ld[i] = ls[i];
+    }
+    if (a.rows > 0) {
+        const float* zs = a.zpool[a.zcur[src]];
+        float* zd = a.zpool[1 - a.zcur[m]];
+        for (int t = 0; t < a.rows; ++t) {
+            const size_t rs = ((size_t)t * a.n_slots + src) * a.cap, rd = ((size_t)t * a.n_slots + m) * a.cap;
+            for (int i = threadIdx.x; i < n; i += kBlock) zd[rd + i] = zs[rs + i];
+        }
+        const float* us = a.upool[a.zcur[src]];
+        float* ud = a.upool[1 - a.zcur[m]];
+        for (int t = threadIdx.x; t < a.rows; t += kBlock)
+            ud[(size_t)t * a.n_slots + m] = us[(size_t)t * a.n_slots + src];
+    }
+}
+
+// Accepted moves: slot m takes the replayed filter where acc[m] is set.
+__global__ __launch_bounds__(kBlock) void k_smc2_accept(AcceptArgs a) {
+    const int m = blockIdx.x;
+    if (!a.acc[m]) return;
+    const Ctl c = a.ctl_src[m];
+    if (threadIdx.x == 0) {
+        a.mu_dst[m] = a.mu_src[m];
+        a.phi_dst[m] = a.phi_src[m];
+        a.sig_dst[m] = a.sig_src[m];
+        a.ctl_dst[m] = c;
+    }
+    const float* xs = a.x_src[c.cur] + (size_t)m * a.cap;
+    float* xd = a.x_dst[c.cur] + (size_t)m * a.cap;
+    const float* ls = a.l_src + (size_t)m * a.cap;
+    float* ld = a.l_dst + (size_t)m * a.cap;
+    for (int i = threadIdx.x; i < a.cap; i += kBlock) {
+        xd[i] = xs[i];
+        ld[i] = ls[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
+
+hipError_t launch_outer_check(const double* logw, const Ctl* ctl, int M, float threshold, int* halt,
+                              double* ess_out, hipStream_t st) {
+    hipLaunchKernelGGL(k_outer_check, dim3(1), dim3(kOuterBlock), 0, st, logw, ctl, M, threshold, halt,
+                       ess_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_smc2_gather(const GatherArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_smc2_gather, dim3(a.n_slots), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_smc2_accept(const AcceptArgs& a, int n_slots, hipStream_t st) {
+    hipLaunchKernelGGL(k_smc2_accept, dim3(n_slots), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
 
 hipError_t launch_batch_init(const BatchInitArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_batch_init, dim3(a.n_slots), dim3(kBlock), 0, st, a);

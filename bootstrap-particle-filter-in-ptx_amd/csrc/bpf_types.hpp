@@ -182,6 +182,9 @@ struct BatchArgs {
     float cn_s;              // sqrt(1 - rho^2)
     float u_step;            // proposal: u' = frac(u + u_step (2 v - 1))
     const int* prop_seed;    // per slot: seed of the proposal streams xi, v
+    // SMC^2 outer layer
+    double* outer_logw;      // per slot: += the tick's log-likelihood increment, or null
+    const int* halt;         // non-zero: every workgroup returns at once, or null
 };
 
 struct BatchInitArgs {
@@ -199,6 +202,33 @@ struct BatchInitArgs {
     const int* zcur;
     float cn_rho, cn_s;
     const int* prop_seed;
+};
+
+// Slot m receives slot anc[m]'s whole inner filter: parameters, particle
+// count, control block, particles (into buffer 0), log-weights, and rows
+// 0..rows-1 of its stored noise (into the pool that slot m does not use).
+struct GatherArgs {
+    const int* anc;
+    const int* n_src; const float* mu_src; const float* phi_src; const float* sig_src;
+    int* n_dst; float* mu_dst; float* phi_dst; float* sig_dst;
+    const Ctl* ctl_src; Ctl* ctl_dst;
+    const float* x_src[2]; float* x_dst;
+    const float* l_src; float* l_dst;
+    float* zpool[2]; float* upool[2]; const int* zcur;
+    int rows;              // ticks of stored noise to move (0: none)
+    int cap, n_slots;
+};
+
+// Accepted moves: slot m takes the replayed filter (parameters, control
+// block, particles, log-weights) where acc[m] is set.
+struct AcceptArgs {
+    const int* acc;
+    const float* mu_src; const float* phi_src; const float* sig_src;
+    float* mu_dst; float* phi_dst; float* sig_dst;
+    const Ctl* ctl_src; Ctl* ctl_dst;
+    const float* x_src[2]; float* x_dst[2];
+    const float* l_src; float* l_dst;
+    int cap;
 };
 
 }  // namespace bpf

@@ -16,14 +16,8 @@
 #include <numeric>
 #include <vector>
 
-#include "../../include/gpu_bpf_batch.h"
 #include "api_util.hpp"
-#include "bpf_types.hpp"
-
-namespace bpf {
-hipError_t launch_batch_init(const BatchInitArgs& a, hipStream_t st);
-hipError_t launch_batch_tick(const BatchArgs& a, int n_blocks, int block, hipStream_t st);
-}  // namespace bpf
+#include "batch_host.hpp"
 
 using namespace bpf;
 
@@ -40,40 +34,16 @@ double c_obs_student(double nu) {
 
 }  // namespace
 
-struct GpuBpfBatch {
-    int M = 0, cap = 0, block = 0;
-    float threshold = 0.0f;
-    ObsModel obs{};
-    hipStream_t stream = nullptr;
-    std::vector<int> n, seed, order;
-    std::vector<float> mu, phi, sig;
-    int* d_order = nullptr;
-    int* d_n = nullptr;
-    int* d_seed = nullptr;
-    float* d_mu = nullptr;
-    float* d_phi = nullptr;
-    float* d_sig = nullptr;
-    Ctl* d_ctl = nullptr;
-    float* d_x[2] = {nullptr, nullptr};
-    float* d_l = nullptr;
-    BpfTickStats* d_ring = nullptr;
-    int ring_cap = 1024;
-    float* d_y = nullptr;              // observation history, indexed by tick
-    int y_cap = 0;
-    int timestep = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    int graph_block = 0;
-};
+namespace bpf {
 
-static void free_graph(GpuBpfBatch* b) {
+void free_graph(GpuBpfBatch* b) {
     if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->graph_exec = nullptr;
     b->graph = nullptr;
 }
 
-static void destroy_batch(GpuBpfBatch* b) {
+void destroy_batch(GpuBpfBatch* b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     free_graph(b);
@@ -85,7 +55,7 @@ static void destroy_batch(GpuBpfBatch* b) {
     delete b;
 }
 
-static BatchArgs tick_args(const GpuBpfBatch* b) {
+BatchArgs tick_args(const GpuBpfBatch* b) {
     BatchArgs a;
     memset((void*)&a, 0, sizeof a);
     a.order = b->d_order;
@@ -109,7 +79,7 @@ static BatchArgs tick_args(const GpuBpfBatch* b) {
     return a;
 }
 
-static int create_impl(GpuBpfBatch* b) {
+int create_impl(GpuBpfBatch* b) {
     HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     const int M = b->M;
     const size_t slots = (size_t)M * b->cap;
@@ -158,7 +128,7 @@ static int create_impl(GpuBpfBatch* b) {
 }
 
 // Appends n observations to the device history (grows by doubling).
-static int append_y(GpuBpfBatch* b, const float* y, int n_ticks, hipMemcpyKind kind) {
+int append_y(GpuBpfBatch* b, const float* y, int n_ticks, hipMemcpyKind kind) {
     const int need = b->timestep + n_ticks;
     if (need > b->y_cap) {
         int cap = std::max(need, std::max(1024, 2 * b->y_cap));
@@ -197,7 +167,7 @@ static int capture_graph(GpuBpfBatch* b) {
     return 0;
 }
 
-static int run_ticks(GpuBpfBatch* b, int n_ticks) {
+int run_ticks(GpuBpfBatch* b, int n_ticks) {
     if (b->block == 0) b->block = 256;
     if (n_ticks >= kGraphTicks && (!b->graph_exec || b->graph_block != b->block)) {
         if (capture_graph(b)) return -1;
@@ -213,7 +183,7 @@ static int run_ticks(GpuBpfBatch* b, int n_ticks) {
     return 0;
 }
 
-static GpuBpfBatch* new_batch(int M, const int* n, const float* mu, const float* phi, const float* sig,
+GpuBpfBatch* new_batch(int M, const int* n, const float* mu, const float* phi, const float* sig,
                               const int* seeds, float nu_obs, float thr) {
     GpuBpfBatch* b = new GpuBpfBatch();
     b->M = M;
@@ -234,6 +204,8 @@ static GpuBpfBatch* new_batch(int M, const int* n, const float* mu, const float*
     b->obs.nph = (nu + 1.0f) * 0.5f;
     return b;
 }
+
+}  // namespace bpf
 
 extern "C" {
 

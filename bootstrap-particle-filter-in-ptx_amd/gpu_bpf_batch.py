@@ -126,3 +126,117 @@ class GpuBpfBatch:
         x = np.empty(int(self.n[m]), np.float32)
         self._check(self.L.gpu_bpf_batch_copy_particles(self.h, int(m), _ptr(x)))
         return x
+
+
+_smc2_declared = False
+
+
+def _smc2_lib():
+    L = _lib()
+    global _smc2_declared
+    if not _smc2_declared:
+        P, F, I = ctypes.c_void_p, ctypes.c_float, ctypes.c_int
+        sig = {
+            "gpu_smc2_create": (P, [I, I, I, F, I]),
+            "gpu_smc2_destroy": (None, [P]),
+            "gpu_smc2_set_prior": (I, [P, P, P]),
+            "gpu_smc2_set_rejuvenation": (I, [P, I, I]),
+            "gpu_smc2_set_cpmmh_rho": (I, [P, F]),
+            "gpu_smc2_set_proposal_scale": (I, [P, F, F]),
+            "gpu_smc2_set_outer_threshold": (I, [P, F]),
+            "gpu_smc2_set_capacity": (I, [P, I]),
+            "gpu_smc2_init_from_prior": (I, [P]),
+            "gpu_smc2_update": (F, [P, F]),
+            "gpu_smc2_run": (I, [P, P, I]),
+            "gpu_smc2_get_outer_ess": (F, [P]),
+            "gpu_smc2_get_theta_mean": (I, [P, P]),
+            "gpu_smc2_get_theta_std": (I, [P, P]),
+            "gpu_smc2_copy_theta": (I, [P, P, P, P]),
+            "gpu_smc2_get_loglik": (I, [P, P]),
+            "gpu_smc2_get_counters": (I, [P, P]),
+            "gpu_smc2_set_block": (I, [P, I]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _smc2_declared = True
+    return L
+
+
+COUNTER_NAMES = ("ticks", "outer_resamples", "proposed", "accepted", "replay_ticks",
+                 "pt_forward", "pt_replay")
+
+
+class GpuSmc2:
+    """SMC^2 over batched inner SV filters (mirror of the gpu_smc2_* C ABI)."""
+
+    def __init__(self, n_theta, n_min, n_max, nu_obs=0.0, seed=42, moves=5, cpmmh_rho=0.99,
+                 outer_threshold=0.5, capacity=2048, prior=None, theta_scale=1.0, u_step=0.05,
+                 block=None):
+        self.L = _smc2_lib()
+        self.M = int(n_theta)
+        self.h = self.L.gpu_smc2_create(self.M, int(n_min), int(n_max), float(nu_obs), int(seed))
+        if not self.h:
+            raise gpu_bpf.GpuBpfError(gpu_bpf.last_error())
+        if prior is not None:
+            lo = np.ascontiguousarray(prior[0], np.float32)
+            hi = np.ascontiguousarray(prior[1], np.float32)
+            self._check(self.L.gpu_smc2_set_prior(self.h, _ptr(lo), _ptr(hi)))
+        self._check(self.L.gpu_smc2_set_rejuvenation(self.h, 0, int(moves)))
+        self._check(self.L.gpu_smc2_set_cpmmh_rho(self.h, float(cpmmh_rho)))
+        self._check(self.L.gpu_smc2_set_proposal_scale(self.h, float(theta_scale), float(u_step)))
+        self._check(self.L.gpu_smc2_set_outer_threshold(self.h, float(outer_threshold)))
+        self._check(self.L.gpu_smc2_set_capacity(self.h, int(capacity)))
+        self._check(self.L.gpu_smc2_init_from_prior(self.h))
+        if block is not None:
+            self._check(self.L.gpu_smc2_set_block(self.h, int(block)))
+
+    def _check(self, rc):
+        if rc < 0:
+            raise gpu_bpf.GpuBpfError(gpu_bpf.last_error())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.gpu_smc2_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def run(self, y):
+        y = np.ascontiguousarray(y, np.float32)
+        self._check(self.L.gpu_smc2_run(self.h, _ptr(y), y.size))
+
+    def update(self, y):
+        ess = self.L.gpu_smc2_update(self.h, float(y))
+        if ess < 0:
+            raise gpu_bpf.GpuBpfError(gpu_bpf.last_error())
+        return ess
+
+    def outer_ess(self):
+        return self.L.gpu_smc2_get_outer_ess(self.h)
+
+    def theta(self):
+        th = np.zeros((self.M, 3), np.float32)
+        lw = np.zeros(self.M, np.float64)
+        n = np.zeros(self.M, np.int32)
+        self._check(self.L.gpu_smc2_copy_theta(self.h, _ptr(th), _ptr(lw), _ptr(n)))
+        return th, lw, n
+
+    def theta_mean_std(self):
+        m = np.zeros(3, np.float32)
+        s = np.zeros(3, np.float32)
+        self._check(self.L.gpu_smc2_get_theta_mean(self.h, _ptr(m)))
+        self._check(self.L.gpu_smc2_get_theta_std(self.h, _ptr(s)))
+        return m, s
+
+    def loglik(self):
+        ll = np.zeros(self.M, np.float64)
+        self._check(self.L.gpu_smc2_get_loglik(self.h, _ptr(ll)))
+        return ll
+
+    def counters(self):
+        c = np.zeros(7, np.int64)
+        self._check(self.L.gpu_smc2_get_counters(self.h, _ptr(c)))
+        return dict(zip(COUNTER_NAMES, c.tolist()))

@@ -56,6 +56,46 @@ int gpu_bpf_batch_get_resample_counts(GpuBpfBatch* batch, int* counts);
 /* The current particles of filter m (n_particles[m] floats). */
 int gpu_bpf_batch_copy_particles(GpuBpfBatch* batch, int filter, float* x);
 
+/* ---- SMC^2 over the batched inner filters ---------------------------------
+ * Mirrors the reference's smc2_cuda_* host API (smc2/smc2_rbpf_batch.cuh:
+ * 463-486) for the SV bootstrap inner model: theta = (mu, phi, sigma_eta)
+ * with a uniform box prior, each theta-particle driving an inner filter whose
+ * particle count is drawn log-uniformly in [n_min, n_max] and travels with it
+ * on resampling.  Outer resampling at outer ESS < threshold * n_theta
+ * (default 0.5), then `moves` correlated pseudo-marginal MH moves (CPMMH:
+ * adaptive Gaussian random walk on theta, Crank-Nicolson noise proposal with
+ * rho = 0.99; smc2_rbpf_batch.cu:693-981,1246-1294).  The inner noise of the
+ * moves is stored on the device (capacity in ticks, default 2048). */
+typedef struct GpuSmc2 GpuSmc2;
+
+GpuSmc2* gpu_smc2_create(int n_theta, int n_min, int n_max, float nu_obs, int seed);
+void gpu_smc2_destroy(GpuSmc2* smc2);
+/* lo/hi: 3 floats each (mu, phi, sigma_eta); default [-2,0]x[0.90,0.995]x[0.05,0.40]. */
+int gpu_smc2_set_prior(GpuSmc2* smc2, const float* lo, const float* hi);
+/* kind 0 = CPMMH; moves per rejuvenation (default 5). */
+int gpu_smc2_set_rejuvenation(GpuSmc2* smc2, int kind, int moves);
+int gpu_smc2_set_cpmmh_rho(GpuSmc2* smc2, float rho);
+/* Scale of the adaptive theta random walk (default 1) and half-width of the
+ * resampling-uniform random walk on the circle (default 0.05). */
+int gpu_smc2_set_proposal_scale(GpuSmc2* smc2, float theta_scale, float u_step);
+int gpu_smc2_set_outer_threshold(GpuSmc2* smc2, float fraction);
+int gpu_smc2_set_capacity(GpuSmc2* smc2, int max_ticks);
+int gpu_smc2_init_from_prior(GpuSmc2* smc2);
+/* One observation; returns the outer ESS (as smc2_cuda_update), -1 on error. */
+float gpu_smc2_update(GpuSmc2* smc2, float y_obs);
+int gpu_smc2_run(GpuSmc2* smc2, const float* y, int n_ticks);
+float gpu_smc2_get_outer_ess(GpuSmc2* smc2);
+int gpu_smc2_get_theta_mean(GpuSmc2* smc2, float* theta_mean);   /* 3 floats */
+int gpu_smc2_get_theta_std(GpuSmc2* smc2, float* theta_std);     /* 3 floats */
+/* theta [n_theta x 3], outer log-weights, particle counts (any may be NULL). */
+int gpu_smc2_copy_theta(GpuSmc2* smc2, float* theta, double* logw, int* n_particles);
+/* FP64 cumulative log-likelihood estimate of each theta-particle's filter. */
+int gpu_smc2_get_loglik(GpuSmc2* smc2, double* ll);
+/* [ticks, outer resamples, moves proposed, moves accepted, replayed ticks,
+ *  forward particle-ticks, replayed particle-ticks] */
+int gpu_smc2_get_counters(GpuSmc2* smc2, long long* counters);
+int gpu_smc2_set_block(GpuSmc2* smc2, int threads);
+
 #ifdef __cplusplus
 }
 #endif
