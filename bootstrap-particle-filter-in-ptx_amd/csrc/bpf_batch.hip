@@ -120,6 +120,10 @@ __global__ __launch_bounds__(kBlock) void k_batch_init(BatchInitArgs a) {
     const int n = a.n[m];
     const float mu = a.mu[m], phi = a.phi[m], sig = a.sig[m];
     const float sd = __builtin_sqrtf((sig * sig) / fmaxf(1.0f - phi * phi, 1e-6f));
+    // derivatives of sd = sigma / sqrt(max(1 - phi^2, 1e-6)) (FP64)
+    const double om = 1.0 - (double)phi * (double)phi;
+    const float dsd_dsig = (float)(om > 1e-6 ? 1.0 / sqrt(om) : 1.0 / sqrt(1e-6));
+    const float dsd_dphi = (float)(om > 1e-6 ? (double)sig * (double)phi / (om * sqrt(om)) : 0.0);
     const uint64_t pa = (uint64_t)(int64_t)a.seed[m] * a.jm;
     const int zc = (a.noise >= kNoisePcgStore) ? a.zcur[m] : 0;
     const size_t off = (size_t)m * a.cap;     // tick 0 of the pools
@@ -141,6 +145,11 @@ __global__ __launch_bounds__(kBlock) void k_batch_init(BatchInitArgs a) {
                 if (a.noise == kNoisePcgStore) a.zpool[zc][off + i] = z;
             }
             v = __builtin_fmaf(sd, z, mu);
+            if (a.dx0[0]) {
+                a.dx0[0][off + i] = 1.0f;
+                a.dx0[1][off + i] = z * dsd_dphi;
+                a.dx0[2][off + i] = z * dsd_dsig;
+            }
         }
         x[i] = v;
     }
@@ -150,7 +159,22 @@ __global__ __launch_bounds__(kBlock) void k_batch_init(BatchInitArgs a) {
 // One tick of every inner filter.
 // ---------------------------------------------------------------------------
 
-template <int BLOCK, bool STUDENT>
+// d log g(y | x) / dx of the observation density (Gaussian or Student-t).
+template <bool STUDENT>
+__device__ inline float dlw_dx(float x, const ObsModel& o) {
+    const float eta2 = o.ysq * expf_cw(-x);
+    if (STUDENT) {
+        const float r = eta2 / o.nu;
+        return o.nph * (r / (1.0f + r)) - 0.5f;
+    }
+    return __builtin_fmaf(0.5f, eta2, -0.5f);
+}
+
+// TAN: also carry the tangents dx/d(mu, phi, sigma) through propagation and
+// resampling and accumulate the pathwise score of each filter's
+// log-likelihood, sum_t sum_i w_ti g_ti dx_ti with g = d lw / dx (forward
+// mode of the single filter's reverse-mode score; ancestors held fixed).
+template <int BLOCK, bool STUDENT, bool TAN>
 __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
     constexpr int NW = BLOCK / kWave;
     constexpr int TP = kBatchTile + kBatchTile / 16;       // padded tile
@@ -159,12 +183,14 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
     __shared__ double s_c[NW][TP];            // in-tile CDF of each wave
     __shared__ float s_x[NW][TP];             // tile particles of each wave
     __shared__ float s_tmax[kBatchMaxTiles];
-    __shared__ double s_ts[4][kBatchMaxTiles];
+    __shared__ double s_ts[TAN ? 7 : 4][kBatchMaxTiles];
+    __shared__ float s_dx[TAN ? NW : 1][TAN ? 3 : 1][TP];   // tile tangents of each wave
     __shared__ double s_tf[kBatchMaxTiles];
     __shared__ double s_tp[kBatchMaxTiles];
     __shared__ int s_flag, s_W;
     __shared__ float s_u, s_M;
     __shared__ double s_S;
+    __shared__ double s_sc[3];
 
     if (a.halt && *a.halt) return;       // the outer layer stopped the run of ticks
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
@@ -178,6 +204,15 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
     float* __restrict__ xb = (cur ? a.x1 : a.x0) + off;
     float* __restrict__ xo = (cur ? a.x0 : a.x1) + off;
     float* __restrict__ lb = a.lw + off;
+    float* dxb[3];
+    float* dxo[3];
+    if (TAN) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            dxb[k] = (cur ? a.dx1[k] : a.dx0[k]) + off;
+            dxo[k] = (cur ? a.dx0[k] : a.dx1[k]) + off;
+        }
+    }
     const float mu = a.mu[m], phi = a.phi[m], sig = a.sig[m];
     const int tick = cp->tick;
     uint64_t jm, jg;
@@ -203,11 +238,25 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
             x[q] = xb[j];
             l[q] = carry ? lb[j] : 0.0f;
         }
+        float d[TAN ? 3 : 1][4];
+        if (TAN) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) d[k][q] = dxb[k][base + 64 * q + lane];
+        }
         if (tick > 0) {      // tick 0 weights the stationary draw
             float z[4];
             batch_normals(a, m, pa, jm, jg, base, lane, zc, zoff, s_q[wid], z);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
+                if (TAN) {    // x' = mu + phi (x - mu) + sigma z
+                    d[0][q] = __builtin_fmaf(phi, d[0][q], 1.0f - phi);
+                    d[1][q] = __builtin_fmaf(phi, d[1][q], x[q] - mu);
+                    d[2][q] = __builtin_fmaf(phi, d[2][q], z[q]);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) dxb[k][base + 64 * q + lane] = d[k][q];
+                }
                 x[q] = transition(x[q], mu, phi, sig, z[q]);
                 xb[base + 64 * q + lane] = x[q];
             }
@@ -223,7 +272,10 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
             lb[j] = l[q];
         }
         const float mt = wave_max(mloc);
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        constexpr int NS = TAN ? 7 : 4;
+        double acc[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) acc[k] = 0.0;
         if (mt != NEG_INF) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -240,6 +292,11 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
                 } else {
                     acc[0] += ws; acc[1] += wx; acc[2] += wxx; acc[3] += wq;
                 }
+                if (TAN) {
+                    const double eg = (j < n) ? de * (double)dlw_dx<STUDENT>(x[q], obs) : 0.0;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) acc[4 + k] += wave_sum(eg * (double)d[k][q]);
+                }
             }
         } else {
             // the single filter's zero sums, added over its four waves
@@ -249,7 +306,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
         if (lane == 0) {
             s_tmax[b] = mt;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) s_ts[k][b] = acc[k];
+            for (int k = 0; k < NS; ++k) s_ts[k][b] = acc[k];
         }
     }
     __syncthreads();
@@ -276,6 +333,13 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
         vx = wave_sum(vx);
         vxx = wave_sum(vxx);
         vq = wave_sum(vq);
+        if (TAN) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double v = wave_sum(f * (in ? s_ts[4 + k][lane] : 0.0));
+                if (lane == 0) s_sc[k] = v;
+            }
+        }
         if (lane == 0) {
             const double gs = ((vs + 0.0) + 0.0) + 0.0;
             const double gsx = ((vx + 0.0) + 0.0) + 0.0;
@@ -300,6 +364,10 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
                 ur_in = (w < 1.0f) ? w : 0.0f;
             }
             finalize_tick(c, n, a.threshold, gmax, S, SX, SXX, Q, ur_in);
+            if (TAN && c.last.mean == c.last.mean) {     // finite statistics
+#pragma unroll
+                for (int k = 0; k < 3; ++k) a.score[3 * m + k] += s_sc[k] / S;
+            }
             if (c.resample) {
                 if (a.noise == kNoisePcgStore) a.upool[zc][ui] = c.u_raw;
                 if (a.noise == kNoisePropose) a.upool[1 - zc][ui] = c.u_raw;
@@ -372,6 +440,10 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
             c = (jl > last) ? __builtin_inf() : c;
             cw[pad16(jl)] = c;
             xw[pad16(jl)] = xv[q];
+            if (TAN) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) s_dx[wid][k][pad16(jl)] = dxb[k][base + jl];
+            }
         }
         int r = 0;
         if (lane == 0) r = (b == 0) ? 0 : count_le_fast(b0, u, 0, W, n);
@@ -383,11 +455,21 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
         wave_lds_sync();
         for (int i = ilo + lane; i < ihi; i += kWave) {
             const double t = (double)target_unwrapped(u, i, n);
-            xo[i] = xw[pad16(lds_lower_bound(cw, last, t))];
+            const int p = pad16(lds_lower_bound(cw, last, t));
+            xo[i] = xw[p];
+            if (TAN) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) dxo[k][i] = s_dx[wid][k][p];
+            }
         }
         for (int i = jlo + lane; i < jhi; i += kWave) {
             const double t = (double)(target_unwrapped(u, i, n) - 1.0f);
-            xo[i] = xw[pad16(lds_lower_bound(cw, last, t))];
+            const int p = pad16(lds_lower_bound(cw, last, t));
+            xo[i] = xw[p];
+            if (TAN) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) dxo[k][i] = s_dx[wid][k][p];
+            }
         }
         wave_lds_sync();
     }
@@ -530,10 +612,15 @@ hipError_t launch_batch_init(const BatchInitArgs& a, hipStream_t st) {
 
 template <int BLOCK>
 static hipError_t launch_batch_tick_b(const BatchArgs& a, int n_blocks, bool student, hipStream_t st) {
-    if (student)
-        hipLaunchKernelGGL((k_batch_tick<BLOCK, true>), dim3(n_blocks), dim3(BLOCK), 0, st, a);
+    const bool tan = a.score != nullptr;
+    if (student && tan)
+        hipLaunchKernelGGL((k_batch_tick<BLOCK, true, true>), dim3(n_blocks), dim3(BLOCK), 0, st, a);
+    else if (student)
+        hipLaunchKernelGGL((k_batch_tick<BLOCK, true, false>), dim3(n_blocks), dim3(BLOCK), 0, st, a);
+    else if (tan)
+        hipLaunchKernelGGL((k_batch_tick<BLOCK, false, true>), dim3(n_blocks), dim3(BLOCK), 0, st, a);
     else
-        hipLaunchKernelGGL((k_batch_tick<BLOCK, false>), dim3(n_blocks), dim3(BLOCK), 0, st, a);
+        hipLaunchKernelGGL((k_batch_tick<BLOCK, false, false>), dim3(n_blocks), dim3(BLOCK), 0, st, a);
     return hipGetLastError();
 }
 

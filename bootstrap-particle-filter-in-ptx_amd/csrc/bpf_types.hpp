@@ -185,6 +185,12 @@ struct BatchArgs {
     // SMC^2 outer layer
     double* outer_logw;      // per slot: += the tick's log-likelihood increment, or null
     const int* halt;         // non-zero: every workgroup returns at once, or null
+    // forward-mode pathwise score (resampling every tick): tangents
+    // dx/d(mu, phi, sigma) of every particle, ping-pong with x, and per slot
+    // the FP64 accumulated d(log-likelihood)/d(mu, phi, sigma)
+    float* dx0[3];
+    float* dx1[3];
+    double* score;           // 3 per slot, or null: no tangents
 };
 
 struct BatchInitArgs {
@@ -202,6 +208,7 @@ struct BatchInitArgs {
     const int* zcur;
     float cn_rho, cn_s;
     const int* prop_seed;
+    float* dx0[3];           // tangents of x_0 (x_0 = mu + sd(phi, sigma) z), or null
 };
 
 // Slot m receives slot anc[m]'s whole inner filter: parameters, particle
@@ -217,6 +224,8 @@ struct GatherArgs {
     float* zpool[2]; float* upool[2]; const int* zcur;
     int rows;              // ticks of stored noise to move (0: none)
     int cap, n_slots;
+    const float* dx_src[2][3]; float* dx_dst[3];   // tangents (null: none)
+    const double* score_src; double* score_dst;
 };
 
 // Accepted moves: slot m takes the replayed filter (parameters, control
@@ -229,6 +238,9 @@ struct AcceptArgs {
     const float* x_src[2]; float* x_dst[2];
     const float* l_src; float* l_dst;
     int cap;
+    const float* dx_src[2][3]; float* dx_dst[2][3];   // tangents (null: none)
+    const double* score_src; double* score_dst;
+    const int* seed_src; int* seed_dst;               // null: seeds kept
 };
 
 }  // namespace bpf

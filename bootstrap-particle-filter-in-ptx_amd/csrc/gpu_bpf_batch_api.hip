@@ -48,9 +48,12 @@ void destroy_batch(GpuBpfBatch* b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     free_graph(b);
     void* ptrs[] = {b->d_order, b->d_n, b->d_seed, b->d_mu, b->d_phi, b->d_sig, b->d_ctl,
-                    b->d_x[0], b->d_x[1], b->d_l, b->d_ring, b->d_y};
+                    b->d_x[0], b->d_x[1], b->d_l, b->d_ring, b->d_y, b->d_score};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    for (int h = 0; h < 2; ++h)
+        for (int k = 0; k < 3; ++k)
+            if (b->d_dx[h][k]) (void)hipFree(b->d_dx[h][k]);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -76,7 +79,45 @@ BatchArgs tick_args(const GpuBpfBatch* b) {
     a.obs = b->obs;
     a.threshold = b->threshold;
     a.noise = kNoisePcg;
+    if (b->d_score) {
+        for (int k = 0; k < 3; ++k) {
+            a.dx0[k] = b->d_dx[0][k];
+            a.dx1[k] = b->d_dx[1][k];
+        }
+        a.score = b->d_score;
+    }
     return a;
+}
+
+// Tangents and score accumulators; x_0 is drawn again with its tangents.
+int enable_score(GpuBpfBatch* b) {
+    if (b->d_score) return 0;
+    const size_t slots = (size_t)b->M * b->cap;
+    for (int h = 0; h < 2; ++h)
+        for (int k = 0; k < 3; ++k) {
+            HIP_TRY(hipMalloc(&b->d_dx[h][k], slots * sizeof(float)));
+            HIP_TRY(hipMemsetAsync(b->d_dx[h][k], 0, slots * sizeof(float), b->stream));
+        }
+    HIP_TRY(hipMalloc(&b->d_score, (size_t)b->M * 3 * sizeof(double)));
+    HIP_TRY(hipMemsetAsync(b->d_score, 0, (size_t)b->M * 3 * sizeof(double), b->stream));
+    BatchInitArgs ia;
+    memset((void*)&ia, 0, sizeof ia);
+    ia.n = b->d_n;
+    ia.seed = b->d_seed;
+    ia.mu = b->d_mu;
+    ia.phi = b->d_phi;
+    ia.sig = b->d_sig;
+    ia.x0 = b->d_x[0];
+    ia.cap = b->cap;
+    ia.n_slots = b->M;
+    ia.jm = kPcgMult * kPcgMult;
+    ia.jg = 1u + kPcgMult;
+    ia.noise = kNoisePcg;
+    for (int k = 0; k < 3; ++k) ia.dx0[k] = b->d_dx[0][k];
+    HIP_TRY(launch_batch_init(ia, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    free_graph(b);
+    return 0;
 }
 
 int create_impl(GpuBpfBatch* b) {
@@ -366,6 +407,25 @@ int gpu_bpf_batch_get_resample_counts(GpuBpfBatch* b, int* counts) {
     std::vector<Ctl> c;
     if (copy_ctl(b, c)) return -1;
     for (int m = 0; m < b->M; ++m) counts[m] = c[m].resample_count;
+    return 0;
+}
+
+int gpu_bpf_batch_enable_score(GpuBpfBatch* b) {
+    if (!b || b->timestep != 0 || b->threshold > 0.0f) {
+        set_err("gpu_bpf_batch_enable_score: before the first tick, with resampling every tick "
+                "(ess_threshold <= 0)");
+        return -1;
+    }
+    return enable_score(b);
+}
+
+int gpu_bpf_batch_get_score(GpuBpfBatch* b, double* score) {
+    if (!b || !score || !b->d_score) {
+        set_err("gpu_bpf_batch_get_score: score not enabled");
+        return -1;
+    }
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipMemcpy(score, b->d_score, (size_t)b->M * 3 * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -37,6 +37,8 @@ def _lib():
             "gpu_bpf_batch_get_loglik": (I, [P, P]),
             "gpu_bpf_batch_get_resample_counts": (I, [P, P]),
             "gpu_bpf_batch_copy_particles": (I, [P, I, P]),
+            "gpu_bpf_batch_enable_score": (I, [P]),
+            "gpu_bpf_batch_get_score": (I, [P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -121,6 +123,14 @@ class GpuBpfBatch:
         c = np.zeros(self.M, np.int32)
         self._check(self.L.gpu_bpf_batch_get_resample_counts(self.h, _ptr(c)))
         return c
+
+    def enable_score(self):
+        self._check(self.L.gpu_bpf_batch_enable_score(self.h))
+
+    def score(self):
+        sc = np.zeros((self.M, 3), np.float64)
+        self._check(self.L.gpu_bpf_batch_get_score(self.h, _ptr(sc)))
+        return sc
 
     def particles(self, m):
         x = np.empty(int(self.n[m]), np.float32)

@@ -141,3 +141,47 @@ def test_batch_errors():
         gpu_bpf_batch.GpuBpfBatch([100, 20000], 0.0, 0.9, 0.1, [1, 2])
     with pytest.raises(gpu_bpf.GpuBpfError, match="set_block"):
         gpu_bpf_batch.GpuBpfBatch([100], 0.0, 0.9, 0.1, [1], block=96)
+
+
+def test_batch_score_matches_adjoint_and_oracle(orc, ydata):
+    """Forward-mode tangents of the batch give the single filter's reverse-mode
+    pathwise score (x_0 differentiated) and the FP64 tangent oracle's, filter
+    by filter, within 1e-4 relative (FP32 states and tangents)."""
+    rng = np.random.default_rng(13)
+    n, mu, phi, sig, seeds = filters(6, rng, 256, 6000)
+    T = 120
+    y = ydata[:T]
+    say("batch score", n.tolist())
+    b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, ess_threshold=0.0)
+    b.enable_score()
+    b.run(y, stats=False)
+    sc = b.score()
+    ll = b.loglik()
+    for m in range(len(n)):
+        f = gpu_bpf.GpuBpf(int(n[m]), float(phi[m]), float(sig[m]), float(mu[m]), seed=int(seeds[m]))
+        llp = f.score_forward(y, 32)
+        gp = f.score_backward(True)
+        f.close()
+        assert llp == ll[m]
+        np.testing.assert_allclose(sc[m], gp, rtol=1e-4, atol=1e-4 * np.max(np.abs(gp)))
+        if m < 2:
+            og = orc.score(y, int(n[m]), rho=float(phi[m]), sigma_z=float(sig[m]), mu=float(mu[m]),
+                           seed=int(seeds[m]), tile=256, x0_reparam=True)[1]
+            np.testing.assert_allclose(sc[m], og, rtol=1e-4, atol=1e-4 * np.max(np.abs(og)))
+    b.close()
+
+
+def test_batch_score_student_t(orc, ydata):
+    rng = np.random.default_rng(17)
+    n, mu, phi, sig, seeds = filters(3, rng, 300, 3000)
+    y = ydata[:80]
+    say("batch score student-t")
+    b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=5.0)
+    b.enable_score()
+    b.run(y, stats=False)
+    sc = b.score()
+    for m in range(len(n)):
+        og = orc.score(y, int(n[m]), rho=float(phi[m]), sigma_z=float(sig[m]), mu=float(mu[m]),
+                       nu_obs=5.0, seed=int(seeds[m]), tile=256, x0_reparam=True)[1]
+        np.testing.assert_allclose(sc[m], og, rtol=1e-4, atol=1e-4 * np.max(np.abs(og)))
+    b.close()
