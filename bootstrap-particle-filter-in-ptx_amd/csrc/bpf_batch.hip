@@ -549,6 +549,14 @@ __global__ __launch_bounds__(kBlock) void k_smc2_gather(GatherArgs a) {
         xd[i] = xs[i];
         ld[i] = ls[i];
     }
+    if (a.score_src) {
+        for (int k = 0; k < 3; ++k) {
+            const float* ds = a.dx_src[c.cur][k] + (size_t)src * a.cap;
+            float* dd = a.dx_dst[k] + (size_t)m * a.cap;
+            for (int i = threadIdx.x; i < a.cap; i += kBlock) dd[i] = ds[i];
+        }
+        if (threadIdx.x < 3) a.score_dst[3 * m + threadIdx.x] = a.score_src[3 * src + threadIdx.x];
+    }
     if (a.rows > 0) {
         const float* zs = a.zpool[a.zcur[src]];
         float* zd = a.zpool[1 - a.zcur[m]];
@@ -582,6 +590,15 @@ __global__ __launch_bounds__(kBlock) void k_smc2_accept(AcceptArgs a) {
         xd[i] = xs[i];
         ld[i] = ls[i];
     }
+    if (a.score_src) {
+        for (int k = 0; k < 3; ++k) {
+            const float* ds = a.dx_src[c.cur][k] + (size_t)m * a.cap;
+            float* dd = a.dx_dst[c.cur][k] + (size_t)m * a.cap;
+            for (int i = threadIdx.x; i < a.cap; i += kBlock) dd[i] = ds[i];
+        }
+        if (threadIdx.x < 3) a.score_dst[3 * m + threadIdx.x] = a.score_src[3 * m + threadIdx.x];
+    }
+    if (a.seed_src && threadIdx.x == 0) a.seed_dst[m] = a.seed_src[m];
 }
 
 // ---------------------------------------------------------------------------

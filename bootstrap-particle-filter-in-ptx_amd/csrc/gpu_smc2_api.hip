@@ -77,11 +77,12 @@ struct TickGraph {
 struct GpuSmc2 {
     int M = 0, nmin = 0, nmax = 0, seed = 0;
     float nu_obs = 0.0f;
-    int kind = 0;                      // 0 = CPMMH
+    int kind = 0;                      // 0 = CPMMH, 1 = MALA
     int moves = 5;
     float cn_rho = 0.99f;
     float u_step = 0.05f;
     float prop_scale = 1.0f;
+    float mala_h = 0.5f;
     float outer_thr = 0.5f;
     float lo[3] = {-2.0f, 0.90f, 0.05f};
     float hi[3] = {0.0f, 0.995f, 0.40f};
@@ -147,7 +148,7 @@ static BatchArgs main_args(const GpuSmc2* s) {
 static BatchArgs rep_args(const GpuSmc2* s) {
     BatchArgs a = tick_args(s->rep);
     a.y = s->main->d_y;
-    a.noise = kNoisePropose;
+    a.noise = stores_noise(s) ? kNoisePropose : kNoisePcg;
     a.zpool[0] = s->d_zpool[0];
     a.zpool[1] = s->d_zpool[1];
     a.upool[0] = s->d_upool[0];
@@ -252,19 +253,28 @@ static void theta_moments(const GpuSmc2* s, double mean[3], double cov[9]) {
     for (int k = 0; k < 9; ++k) cov[k] /= (M > 1 ? M - 1 : 1);
 }
 
-// Lower Cholesky factor of cov + eps I (diagonal fallback).
-static void cholesky3(const double cov[9], double L[9]) {
-    double A[9];
-    for (int k = 0; k < 9; ++k) A[k] = cov[k];
-    for (int i = 0; i < 3; ++i) A[4 * i] += 1e-10;
+// ---------------------------------------------------------------------------
+// Proposals.  A = scale2 * Cov(theta) (+ a tiny ridge) with Cholesky factor
+// L; random walk: theta' = theta + L xi; preconditioned MALA:
+// theta' = theta + (h/2) A g + sqrt(h) L xi, log q = -|L^-1 (theta' - mean)|^2 / 2h.
+// ---------------------------------------------------------------------------
+struct Precond {
+    double A[9], L[9];
+};
+
+static void make_precond(const double cov[9], double scale2, Precond& P) {
+    // factor Cov + ridge, then scale (scale2 = 0 gives exactly theta' = theta)
+    double C[9], L[9];
+    for (int k = 0; k < 9; ++k) C[k] = cov[k];
+    for (int i = 0; i < 3; ++i) C[4 * i] += 1e-12 + 1e-9 * C[4 * i];
     for (int k = 0; k < 9; ++k) L[k] = 0.0;
     bool ok = true;
-    for (int i = 0; i < 3 && ok; ++i) {
+    for (int i = 0; i < 3 && ok; ++i)
         for (int j = 0; j <= i; ++j) {
-            double sum = A[3 * i + j];
+            double sum = C[3 * i + j];
             for (int k = 0; k < j; ++k) sum -= L[3 * i + k] * L[3 * j + k];
             if (i == j) {
-                if (sum <= 0.0) {
+                if (!(sum > 0.0)) {
                     ok = false;
                     break;
                 }
@@ -273,11 +283,67 @@ static void cholesky3(const double cov[9], double L[9]) {
                 L[3 * i + j] = sum / L[3 * j + j];
             }
         }
+    if (!ok) {      // diagonal fallback
+        for (int k = 0; k < 9; ++k) {
+            L[k] = 0.0;
+            if (k % 4 != 0) C[k] = 0.0;
+        }
+        for (int i = 0; i < 3; ++i) L[4 * i] = sqrt(std::max(C[4 * i], 1e-12));
     }
-    if (!ok) {
-        for (int k = 0; k < 9; ++k) L[k] = 0.0;
-        for (int i = 0; i < 3; ++i) L[4 * i] = sqrt(std::max(A[4 * i], 1e-12));
+    const double sc = sqrt(scale2);
+    for (int k = 0; k < 9; ++k) {
+        P.L[k] = sc * L[k];
+        P.A[k] = scale2 * C[k];
     }
+}
+
+static void mala_mean(const Precond& P, double h, const double th[3], const double g[3], double out[3]) {
+    for (int i = 0; i < 3; ++i) {
+        double ag = 0.0;
+        for (int k = 0; k < 3; ++k) ag += P.A[3 * i + k] * g[k];
+        out[i] = th[i] + 0.5 * h * ag;
+    }
+}
+
+// log q(to | from) up to a constant (the same constant both ways).
+static double mala_logq(const Precond& P, double h, const double from[3], const double g_from[3],
+                        const double to[3]) {
+    double mean[3], d[3], r[3];
+    mala_mean(P, h, from, g_from, mean);
+    for (int i = 0; i < 3; ++i) d[i] = to[i] - mean[i];
+    for (int i = 0; i < 3; ++i) {        // forward substitution L r = d
+        double v = d[i];
+        for (int k = 0; k < i; ++k) v -= P.L[3 * i + k] * r[k];
+        r[i] = v / P.L[3 * i + i];
+    }
+    return -(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) / (2.0 * h);
+}
+
+static void propose(int kind, const Precond& P, double h, const double th[3], const double g[3],
+                    const double xi[3], double out[3]) {
+    double base[3];
+    if (kind == 1) {
+        mala_mean(P, h, th, g, base);
+    } else {
+        for (int i = 0; i < 3; ++i) base[i] = th[i];
+    }
+    const double sc = (kind == 1) ? sqrt(h) : 1.0;
+    for (int i = 0; i < 3; ++i) {
+        double d = 0.0;
+        for (int k = 0; k <= i; ++k) d += P.L[3 * i + k] * xi[k];
+        out[i] = base[i] + sc * d;
+    }
+}
+
+// log acceptance ratio of a move from (theta, ll, g) to (theta', ll', g'):
+// the pseudo-marginal ratio, with the MALA proposal correction (omitted
+// when `drop_q` -- a planted fault for the self test).
+static double log_alpha(int kind, const Precond& P, double h, const double th[3], double ll,
+                        const double g[3], const double tp[3], double llp, const double gp[3],
+                        bool drop_q = false) {
+    double la = llp - ll;
+    if (kind == 1 && !drop_q) la += mala_logq(P, h, tp, gp, th) - mala_logq(P, h, th, g, tp);
+    return la;
 }
 
 // Outer systematic resampling after `rows` ticks: slot m takes slot anc[m]'s
@@ -320,7 +386,21 @@ static int outer_resample(GpuSmc2* s, int rows) {
     g.rows = stores_noise(s) ? rows : 0;
     g.cap = mb->cap;
     g.n_slots = M;
+    if (mb->d_score) {
+        for (int h = 0; h < 2; ++h)
+            for (int k = 0; k < 3; ++k) g.dx_src[h][k] = mb->d_dx[h][k];
+        for (int k = 0; k < 3; ++k) g.dx_dst[k] = rb->d_dx[0][k];
+        g.score_src = mb->d_score;
+        g.score_dst = rb->d_score;
+    }
     HIP_TRY(launch_smc2_gather(g, st));
+    if (mb->d_score) {
+        for (int k = 0; k < 3; ++k)
+            HIP_TRY(hipMemcpyAsync(mb->d_dx[0][k], rb->d_dx[0][k], (size_t)M * mb->cap * sizeof(float),
+                                   hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(mb->d_score, rb->d_score, (size_t)M * 3 * sizeof(double),
+                               hipMemcpyDeviceToDevice, st));
+    }
     const size_t slots = (size_t)M * mb->cap;
     HIP_TRY(hipMemcpyAsync(mb->d_n, rb->d_n, M * sizeof(int), hipMemcpyDeviceToDevice, st));
     HIP_TRY(hipMemcpyAsync(mb->d_mu, rb->d_mu, M * sizeof(float), hipMemcpyDeviceToDevice, st));
@@ -358,34 +438,47 @@ static int outer_resample(GpuSmc2* s, int rows) {
 
 // One correlated pseudo-marginal MH move of every slot, replaying ticks
 // 0..rows-1 with proposed parameters and noise.
-static int cpmmh_move(GpuSmc2* s, int rows) {
+static int read_score(GpuBpfBatch* b, int M, std::vector<double>& g) {
+    g.assign((size_t)M * 3, 0.0);
+    if (!b->d_score) return 0;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipMemcpy(g.data(), b->d_score, (size_t)M * 3 * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// One pseudo-marginal MH move of every slot, replaying ticks 0..rows-1 with
+// proposed parameters: CPMMH (random walk, Crank-Nicolson noise around the
+// stored noise) or MALA (score-driven drift, fresh noise, replay with
+// tangents for the reverse proposal density).
+static int move(GpuSmc2* s, int rows) {
     const int M = s->M;
+    const int kind = s->kind;
     GpuBpfBatch* mb = s->main;
     GpuBpfBatch* rb = s->rep;
-    double mean[3], cov[9], L[9];
+    double mean[3], cov[9];
     theta_moments(s, mean, cov);
-    cholesky3(cov, L);
-    const double scale = s->prop_scale * 2.38 / sqrt(3.0);
+    Precond P;
+    make_precond(cov, kind == 1 ? 1.0 : (double)s->prop_scale * s->prop_scale * 2.38 * 2.38 / 3.0, P);
+    const double h = s->mala_h;
+    std::vector<double> g;
+    if (read_score(mb, M, g)) return -1;
     std::vector<int> pseed(M);
     for (int m = 0; m < M; ++m) {
         const double xi[3] = {s->mrng.normal(), s->mrng.normal(), s->mrng.normal()};
         const double th[3] = {mb->mu[m], mb->phi[m], mb->sig[m]};
-        float tp[3];
-        for (int i = 0; i < 3; ++i) {
-            double d = 0.0;
-            for (int k = 0; k <= i; ++k) d += L[3 * i + k] * xi[k];
-            tp[i] = (float)(th[i] + scale * d);
-        }
-        rb->mu[m] = tp[0];
-        rb->phi[m] = tp[1];
-        rb->sig[m] = tp[2];
+        double tp[3];
+        propose(kind, P, h, th, &g[3 * m], xi, tp);
+        rb->mu[m] = (float)tp[0];
+        rb->phi[m] = (float)tp[1];
+        rb->sig[m] = (float)tp[2];
         rb->n[m] = mb->n[m];
-        rb->seed[m] = mb->seed[m];
         pseed[m] = (int)(s->mrng.next() & 0x7fffffffu);
+        rb->seed[m] = (kind == 1) ? pseed[m] : mb->seed[m];
     }
     if (upload_slots(s, rb)) return -1;
     HIP_TRY(hipMemcpyAsync(s->d_prop_seed, pseed.data(), M * sizeof(int), hipMemcpyHostToDevice, rb->stream));
     if (fresh_ctl(s, rb)) return -1;
+    if (rb->d_score) HIP_TRY(hipMemsetAsync(rb->d_score, 0, (size_t)M * 3 * sizeof(double), rb->stream));
     BatchInitArgs ia;
     memset((void*)&ia, 0, sizeof ia);
     ia.n = rb->d_n; ia.seed = rb->d_seed; ia.mu = rb->d_mu; ia.phi = rb->d_phi; ia.sig = rb->d_sig;
@@ -394,29 +487,36 @@ static int cpmmh_move(GpuSmc2* s, int rows) {
     ia.n_slots = M;
     ia.jm = kPcgMult * kPcgMult;
     ia.jg = 1u + kPcgMult;
-    ia.noise = kNoisePropose;
+    ia.noise = (kind == 1) ? kNoisePcg : kNoisePropose;
     ia.zpool[0] = s->d_zpool[0]; ia.zpool[1] = s->d_zpool[1];
     ia.zcur = s->d_zcur;
     ia.cn_rho = s->cn_rho;
     ia.cn_s = sqrtf(fmaxf(1.0f - s->cn_rho * s->cn_rho, 0.0f));
     ia.prop_seed = s->d_prop_seed;
+    if (rb->d_score)
+        for (int k = 0; k < 3; ++k) ia.dx0[k] = rb->d_dx[0][k];
     HIP_TRY(launch_batch_init(ia, rb->stream));
     if (enqueue_run(s, s->rep_graph, rep_args(s), false, rows, rb->stream)) return -1;
-    std::vector<double> llp, llc;
-    if (read_ll(rb, M, llp) || read_ll(mb, M, llc)) return -1;
+    std::vector<double> llp, llc, gp;
+    if (read_ll(rb, M, llp) || read_ll(mb, M, llc) || read_score(rb, M, gp)) return -1;
+    std::vector<Ctl> cm(M);
+    HIP_TRY(hipMemcpy(cm.data(), mb->d_ctl, M * sizeof(Ctl), hipMemcpyDeviceToHost));
     std::vector<int> acc(M, 0);
     int n_acc = 0;
     for (int m = 0; m < M; ++m) {
-        const float tp[3] = {rb->mu[m], rb->phi[m], rb->sig[m]};
-        const double la = llp[m] - llc[m];
+        const double th[3] = {mb->mu[m], mb->phi[m], mb->sig[m]};
+        const double tp[3] = {rb->mu[m], rb->phi[m], rb->sig[m]};
+        const float tpf[3] = {rb->mu[m], rb->phi[m], rb->sig[m]};
+        const double la = log_alpha(kind, P, h, th, llc[m], &g[3 * m], tp, llp[m], &gp[3 * m]);
         const double lu = log(s->mrng.uniform());
-        if (in_prior(s, tp) && std::isfinite(llp[m]) && (lu < la || !std::isfinite(llc[m]))) {
+        if (in_prior(s, tpf) && std::isfinite(llp[m]) && (lu < la || !std::isfinite(llc[m]))) {
             acc[m] = 1;
             ++n_acc;
-            mb->mu[m] = tp[0];
-            mb->phi[m] = tp[1];
-            mb->sig[m] = tp[2];
-            s->zcur[m] = 1 - s->zcur[m];
+            mb->mu[m] = tpf[0];
+            mb->phi[m] = tpf[1];
+            mb->sig[m] = tpf[2];
+            if (kind == 1) mb->seed[m] = rb->seed[m];
+            else s->zcur[m] = 1 - s->zcur[m];
         }
     }
     hipStream_t st = mb->stream;
@@ -431,15 +531,30 @@ static int cpmmh_move(GpuSmc2* s, int rows) {
     a.x_dst[0] = mb->d_x[0]; a.x_dst[1] = mb->d_x[1];
     a.l_src = rb->d_l; a.l_dst = mb->d_l;
     a.cap = mb->cap;
+    if (mb->d_score) {
+        for (int h2 = 0; h2 < 2; ++h2)
+            for (int k = 0; k < 3; ++k) {
+                a.dx_src[h2][k] = rb->d_dx[h2][k];
+                a.dx_dst[h2][k] = mb->d_dx[h2][k];
+            }
+        a.score_src = rb->d_score;
+        a.score_dst = mb->d_score;
+    }
+    if (kind == 1) {
+        a.seed_src = rb->d_seed;
+        a.seed_dst = mb->d_seed;
+    }
     HIP_TRY(launch_smc2_accept(a, M, st));
     HIP_TRY(hipMemcpyAsync(s->d_zcur, s->zcur.data(), M * sizeof(int), hipMemcpyHostToDevice, st));
     HIP_TRY(hipStreamSynchronize(st));
-    // accepted slots keep drawing their future ticks from their own streams
-    std::vector<Ctl> c(M);
-    HIP_TRY(hipMemcpy(c.data(), mb->d_ctl, M * sizeof(Ctl), hipMemcpyDeviceToHost));
-    for (int m = 0; m < M; ++m)
-        if (acc[m]) c[m].host_rng = (unsigned long long)(long long)mb->seed[m] * 67890ULL + 12345ULL;
-    HIP_TRY(hipMemcpy(mb->d_ctl, c.data(), M * sizeof(Ctl), hipMemcpyHostToDevice));
+    if (kind == 0) {
+        // accepted slots continue their own host streams for future ticks
+        std::vector<Ctl> c(M);
+        HIP_TRY(hipMemcpy(c.data(), mb->d_ctl, M * sizeof(Ctl), hipMemcpyDeviceToHost));
+        for (int m = 0; m < M; ++m)
+            if (acc[m]) c[m].host_rng = cm[m].host_rng;
+        HIP_TRY(hipMemcpy(mb->d_ctl, c.data(), M * sizeof(Ctl), hipMemcpyHostToDevice));
+    }
     s->proposed += M;
     s->accepted += n_acc;
     s->replay_ticks += rows;
@@ -452,7 +567,7 @@ static int cpmmh_move(GpuSmc2* s, int rows) {
 static int rejuvenate(GpuSmc2* s, int rows) {
     if (outer_resample(s, rows)) return -1;
     for (int k = 0; k < s->moves; ++k)
-        if (cpmmh_move(s, rows)) return -1;
+        if (move(s, rows)) return -1;
     return 0;
 }
 
@@ -529,8 +644,9 @@ int gpu_smc2_set_prior(GpuSmc2* s, const float* lo, const float* hi) {
 }
 
 int gpu_smc2_set_rejuvenation(GpuSmc2* s, int kind, int moves) {
-    if (!s || kind != 0 || moves < 0) {
-        set_err("gpu_smc2_set_rejuvenation: kind 0 (CPMMH) and moves >= 0");
+    if (!s || s->ready || (kind != 0 && kind != 1) || moves < 0) {
+        set_err("gpu_smc2_set_rejuvenation: kind 0 (CPMMH) or 1 (MALA), moves >= 0, before "
+                "gpu_smc2_init_from_prior");
         return -1;
     }
     s->kind = kind;
@@ -554,6 +670,15 @@ int gpu_smc2_set_proposal_scale(GpuSmc2* s, float theta_scale, float u_step) {
     }
     s->prop_scale = theta_scale;
     s->u_step = u_step;
+    return 0;
+}
+
+int gpu_smc2_set_mala_step(GpuSmc2* s, float h) {
+    if (!s || !(h > 0.0f)) {
+        set_err("gpu_smc2_set_mala_step: h > 0");
+        return -1;
+    }
+    s->mala_h = h;
     return 0;
 }
 
@@ -620,6 +745,7 @@ int gpu_smc2_init_from_prior(GpuSmc2* s) {
     HIP_TRY(hipMemsetAsync(s->d_halt, 0, 2 * sizeof(int), st));
     HIP_TRY(hipMemsetAsync(s->d_zcur, 0, M * sizeof(int), st));
     if (upload_slots(s, s->main) || fresh_ctl(s, s->main)) return -1;
+    if (!stores_noise(s) && (enable_score(s->main) || enable_score(s->rep))) return -1;
     // x_0 of every slot (stored as tick 0 of the noise when moves replay it)
     BatchInitArgs ia;
     memset((void*)&ia, 0, sizeof ia);
@@ -633,6 +759,10 @@ int gpu_smc2_init_from_prior(GpuSmc2* s) {
     ia.noise = stores_noise(s) ? kNoisePcgStore : kNoisePcg;
     ia.zpool[0] = s->d_zpool[0]; ia.zpool[1] = s->d_zpool[1];
     ia.zcur = s->d_zcur;
+    if (s->main->d_score) {
+        HIP_TRY(hipMemsetAsync(s->main->d_score, 0, (size_t)M * 3 * sizeof(double), st));
+        for (int k = 0; k < 3; ++k) ia.dx0[k] = s->main->d_dx[0][k];
+    }
     HIP_TRY(launch_batch_init(ia, st));
     HIP_TRY(hipStreamSynchronize(st));
     s->last_ess = (double)M;
@@ -723,6 +853,76 @@ int gpu_smc2_get_counters(GpuSmc2* s, long long* out) {
     const long long v[7] = {s->ticks, s->outer_resamples, s->proposed, s->accepted,
                             s->replay_ticks, s->pt_forward, s->pt_replay};
     memcpy(out, v, sizeof v);
+    return 0;
+}
+
+// The move code on a closed-form target: N(m, C) in three dimensions with
+// the exact log-density as "log-likelihood" and its gradient as "score",
+// preconditioner A = C.  Writes [mean(3), cov(9), acceptance rate].  fault 1
+// drops the MALA proposal correction (a planted mistake the check must see).
+int gpu_smc2_selftest_mcmc(int kind, int iters, int seed, int fault, double* out) {
+    if ((kind != 0 && kind != 1) || iters <= 0 || !out) {
+        set_err("gpu_smc2_selftest_mcmc: bad arguments");
+        return -1;
+    }
+    const double mu0[3] = {-1.0, 0.95, 0.2};
+    const double C[9] = {0.04, 0.002, -0.003, 0.002, 0.0004, 0.0001, -0.003, 0.0001, 0.0025};
+    // C^-1 by cofactors
+    const double det = C[0] * (C[4] * C[8] - C[5] * C[7]) - C[1] * (C[3] * C[8] - C[5] * C[6]) +
+                       C[2] * (C[3] * C[7] - C[4] * C[6]);
+    double Ci[9];
+    Ci[0] = (C[4] * C[8] - C[5] * C[7]) / det;
+    Ci[1] = (C[2] * C[7] - C[1] * C[8]) / det;
+    Ci[2] = (C[1] * C[5] - C[2] * C[4]) / det;
+    Ci[3] = (C[5] * C[6] - C[3] * C[8]) / det;
+    Ci[4] = (C[0] * C[8] - C[2] * C[6]) / det;
+    Ci[5] = (C[2] * C[3] - C[0] * C[5]) / det;
+    Ci[6] = (C[3] * C[7] - C[4] * C[6]) / det;
+    Ci[7] = (C[1] * C[6] - C[0] * C[7]) / det;
+    Ci[8] = (C[0] * C[4] - C[1] * C[3]) / det;
+    auto logp = [&](const double* t, double* g) {
+        double d[3] = {t[0] - mu0[0], t[1] - mu0[1], t[2] - mu0[2]};
+        double q = 0.0;
+        for (int i = 0; i < 3; ++i) {
+            double v = 0.0;
+            for (int k = 0; k < 3; ++k) v += Ci[3 * i + k] * d[k];
+            g[i] = -v;
+            q += d[i] * v;
+        }
+        return -0.5 * q;
+    };
+    Precond P;
+    make_precond(C, kind == 1 ? 1.0 : 2.38 * 2.38 / 3.0, P);
+    const double h = 0.8;
+    HostRng r;
+    r.seed((uint64_t)(int64_t)seed, 0x7e57ull);
+    double th[3] = {mu0[0], mu0[1], mu0[2]}, g[3];
+    double lp = logp(th, g);
+    double s1[3] = {0, 0, 0}, s2[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    long long acc = 0;
+    for (int it = 0; it < iters; ++it) {
+        const double xi[3] = {r.normal(), r.normal(), r.normal()};
+        double tp[3], gp[3];
+        propose(kind, P, h, th, g, xi, tp);
+        const double lpp = logp(tp, gp);
+        const double la = log_alpha(kind, P, h, th, lp, g, tp, lpp, gp, fault == 1);
+        if (log(r.uniform()) < la) {
+            for (int k = 0; k < 3; ++k) {
+                th[k] = tp[k];
+                g[k] = gp[k];
+            }
+            lp = lpp;
+            ++acc;
+        }
+        for (int i = 0; i < 3; ++i) {
+            s1[i] += th[i];
+            for (int j = 0; j < 3; ++j) s2[3 * i + j] += th[i] * th[j];
+        }
+    }
+    for (int i = 0; i < 3; ++i) out[i] = s1[i] / iters;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) out[3 + 3 * i + j] = s2[3 * i + j] / iters - out[i] * out[j];
+    out[12] = (double)acc / iters;
     return 0;
 }
 

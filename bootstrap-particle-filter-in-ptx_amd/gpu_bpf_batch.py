@@ -165,6 +165,8 @@ def _smc2_lib():
             "gpu_smc2_get_loglik": (I, [P, P]),
             "gpu_smc2_get_counters": (I, [P, P]),
             "gpu_smc2_set_block": (I, [P, I]),
+            "gpu_smc2_set_mala_step": (I, [P, F]),
+            "gpu_smc2_selftest_mcmc": (I, [I, I, I, I, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -183,7 +185,7 @@ class GpuSmc2:
 
     def __init__(self, n_theta, n_min, n_max, nu_obs=0.0, seed=42, moves=5, cpmmh_rho=0.99,
                  outer_threshold=0.5, capacity=2048, prior=None, theta_scale=1.0, u_step=0.05,
-                 block=None):
+                 block=None, kind="cpmmh", mala_step=0.5):
         self.L = _smc2_lib()
         self.M = int(n_theta)
         self.h = self.L.gpu_smc2_create(self.M, int(n_min), int(n_max), float(nu_obs), int(seed))
@@ -193,7 +195,9 @@ class GpuSmc2:
             lo = np.ascontiguousarray(prior[0], np.float32)
             hi = np.ascontiguousarray(prior[1], np.float32)
             self._check(self.L.gpu_smc2_set_prior(self.h, _ptr(lo), _ptr(hi)))
-        self._check(self.L.gpu_smc2_set_rejuvenation(self.h, 0, int(moves)))
+        self._check(self.L.gpu_smc2_set_rejuvenation(self.h, 1 if kind == "mala" else 0,
+                                                     int(moves)))
+        self._check(self.L.gpu_smc2_set_mala_step(self.h, float(mala_step)))
         self._check(self.L.gpu_smc2_set_cpmmh_rho(self.h, float(cpmmh_rho)))
         self._check(self.L.gpu_smc2_set_proposal_scale(self.h, float(theta_scale), float(u_step)))
         self._check(self.L.gpu_smc2_set_outer_threshold(self.h, float(outer_threshold)))
@@ -250,3 +254,14 @@ class GpuSmc2:
         c = np.zeros(7, np.int64)
         self._check(self.L.gpu_smc2_get_counters(self.h, _ptr(c)))
         return dict(zip(COUNTER_NAMES, c.tolist()))
+
+
+def selftest_mcmc(kind, iters, seed=1, fault=0):
+    """The rejuvenation moves on a closed-form 3-d Gaussian target (host code
+    of the library, no GPU): returns (mean[3], cov[3, 3], acceptance)."""
+    out = np.zeros(13, np.float64)
+    rc = _smc2_lib().gpu_smc2_selftest_mcmc(1 if kind == "mala" else 0, int(iters), int(seed),
+                                             int(fault), _ptr(out))
+    if rc < 0:
+        raise gpu_bpf.GpuBpfError(gpu_bpf.last_error())
+    return out[:3], out[3:12].reshape(3, 3), out[12]

@@ -81,8 +81,14 @@ GpuSmc2* gpu_smc2_create(int n_theta, int n_min, int n_max, float nu_obs, int se
 void gpu_smc2_destroy(GpuSmc2* smc2);
 /* lo/hi: 3 floats each (mu, phi, sigma_eta); default [-2,0]x[0.90,0.995]x[0.05,0.40]. */
 int gpu_smc2_set_prior(GpuSmc2* smc2, const float* lo, const float* hi);
-/* kind 0 = CPMMH; moves per rejuvenation (default 5). */
+/* kind 0 = CPMMH (stored noise, correlated proposals), 1 = MALA
+ * (preconditioned Langevin proposals from the per-filter pathwise scores,
+ * fresh noise per proposal); moves per rejuvenation (default 5).  Before
+ * gpu_smc2_init_from_prior. */
 int gpu_smc2_set_rejuvenation(GpuSmc2* smc2, int kind, int moves);
+/* MALA step h: theta' = theta + (h/2) A g + sqrt(h) A^(1/2) xi, A = Cov(theta)
+ * of the theta-particles (default 0.5). */
+int gpu_smc2_set_mala_step(GpuSmc2* smc2, float h);
 int gpu_smc2_set_cpmmh_rho(GpuSmc2* smc2, float rho);
 /* Scale of the adaptive theta random walk (default 1) and half-width of the
  * resampling-uniform random walk on the circle (default 0.05). */
@@ -104,6 +110,11 @@ int gpu_smc2_get_loglik(GpuSmc2* smc2, double* ll);
  *  forward particle-ticks, replayed particle-ticks] */
 int gpu_smc2_get_counters(GpuSmc2* smc2, long long* counters);
 int gpu_smc2_set_block(GpuSmc2* smc2, int threads);
+/* Verification aid (host only, no GPU): the rejuvenation moves' proposal and
+ * acceptance code on a closed-form 3-d Gaussian target; kind as above;
+ * fault 1 drops the MALA proposal correction.  out: mean[3], cov[9],
+ * acceptance rate. */
+int gpu_smc2_selftest_mcmc(int kind, int iters, int seed, int fault, double* out);
 
 #ifdef __cplusplus
 }

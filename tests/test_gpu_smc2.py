@@ -83,3 +83,20 @@ def test_update_returns_outer_ess(ydata):
         assert 0.0 < ess <= 32.0 + 1e-6
     assert s.counters()["ticks"] == 20
     s.close()
+
+
+def test_mala_rejuvenation_concentrates(ydata):
+    """Score-driven (MALA) rejuvenation from the batch's per-filter pathwise
+    scores, Student-t observations as the heavy-tailed configuration."""
+    say("mala")
+    s = gpu_bpf_batch.GpuSmc2(128, 512, 2048, nu_obs=5.0, seed=9, kind="mala", moves=3,
+                              mala_step=0.5)
+    s.run(ydata[:300])
+    mean, sd = s.theta_mean_std()
+    c = s.counters()
+    assert c["outer_resamples"] >= 2
+    assert 0.05 < c["accepted"] / c["proposed"] < 0.99
+    assert np.all(np.isfinite(mean)) and sd[0] < 0.5
+    th, lw, n = s.theta()
+    assert np.all((n >= 512) & (n <= 2048))
+    s.close()
