@@ -11,7 +11,12 @@ timed region starts.
 
 Prints one JSON line: particle-ticks/s, the HBM roofline of the dominant
 kernel (propagate+weight), and the CPU oracle's throughput on a bounded
-sample of the same workload.
+sample of the same workload.  Under "workloads" the same line carries the
+other configurations (unless --no-extra): the always-resample N=65,536
+filter with its log-likelihood error against the FP64 oracle, the
+differentiable N=2^20 filter (forward + adjoint score), and the batched SMC^2
+runs (CPMMH, and MALA with Student-t observations) next to the plain filter
+looped over the same parameter sets.
 """
 import argparse
 import json
@@ -65,6 +70,181 @@ def cpu_baseline(y, mu_t, n, ticks, threads):
     return n * ticks / dt, dt
 
 
+def log_uniform_counts(rng, M, lo, hi):
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), M)).round().astype(np.int32)
+
+
+def prior_draws(rng, M):
+    mu = rng.uniform(-2.0, 0.0, M).astype(np.float32)
+    phi = rng.uniform(0.90, 0.995, M).astype(np.float32)
+    sig = rng.uniform(0.05, 0.40, M).astype(np.float32)
+    seeds = rng.integers(1, 1 << 30, M).astype(np.int32)
+    return mu, phi, sig, seeds
+
+
+def looped_plain(gpu_bpf, n, mu, phi, sig, seeds, y, ticks, nu=0.0, score=False):
+    """The plain filter looped over parameter sets (the baseline the batch
+    replaces): particle-ticks/s over `ticks` ticks of each filter."""
+    tot = 0.0
+    pt = 0
+    for m in range(len(n)):
+        f = gpu_bpf.GpuBpf(int(n[m]), float(phi[m]), float(sig[m]), float(mu[m]), nu_obs=nu,
+                           seed=int(seeds[m]), ess_threshold=0.0)
+        if score:
+            t0 = time.perf_counter()
+            f.score_forward(y[:ticks], 64)
+            f.score_backward(True)
+            tot += time.perf_counter() - t0
+        else:
+            f.run(y[:2])
+            f.synchronize()
+            t0 = time.perf_counter()
+            f.run(y[2:2 + ticks])
+            f.synchronize()
+            tot += time.perf_counter() - t0
+        f.close()
+        pt += int(n[m]) * ticks
+    return pt / tot
+
+
+def extra_workloads(args, gpu_bpf, oracle):
+    import gpu_bpf_batch
+    out = {}
+    # always-resample N = 65,536, T = 1,000 (the reference's CPU-runnable case)
+    _, y0 = oracle.dgp(1000, 0.97, 0.15, -1.0, seed=42)
+    y0 = y0.astype(np.float32)
+    f = gpu_bpf.GpuBpf(65536, 0.97, 0.15, -1.0, seed=42, ess_threshold=0.0)
+    tile, _ = f.layout()
+    t0 = time.perf_counter()
+    _, st = f.run(y0)
+    wall = time.perf_counter() - t0
+    f.close()
+    ref = oracle.run_filter(y0, 65536, ess_threshold=0.0, tile=tile, mode=oracle.MODE_F64_COUPLED)[0]
+    t0 = time.perf_counter()
+    oracle.run_filter(y0[:200], 65536, ess_threshold=0.0, tile=tile, threads=args.cpu_threads)
+    cpu = 65536 * 200 / (time.perf_counter() - t0)
+    out["plain_n65536_t1000"] = {
+        "particle_ticks_per_s": 65536 * 1000 / wall,
+        "ms_per_tick": wall,
+        "ll_fp64": float(st["ll_cum"][-1]),
+        "ll_rel_err_vs_fp64_oracle": abs(st["ll_cum"][-1] - ref["ll_cum"][-1]) / abs(ref["ll_cum"][-1]),
+        "cpu_oracle_particle_ticks_per_s": cpu,
+        "cpu_threads": args.cpu_threads,
+    }
+    log("C0", out["plain_n65536_t1000"])
+    # differentiable filter: N = 2^20, T = 2,000, checkpoint every 64 ticks
+    hs = np.array(oracle.host_stream(42, 3), np.uint64)
+    u = (hs >> np.uint64(9)).astype(np.float64) / 8388608.0
+    th = (-2.0 + 2.0 * u[0], 0.90 + 0.095 * u[1], 0.05 + 0.35 * u[2])
+    _, y2 = oracle.dgp(2000, th[1], th[2], th[0], seed=42)
+    y2 = y2.astype(np.float32)
+    n2 = 1 << 20
+    f = gpu_bpf.GpuBpf(n2, th[1], th[2], th[0], seed=42)
+    t0 = time.perf_counter()
+    ll2 = f.score_forward(y2, 64)
+    t1 = time.perf_counter()
+    g2 = f.score_backward(True)
+    t2 = time.perf_counter()
+    f.close()
+    out["adjoint_n1M_t2000"] = {
+        "theta": [float(v) for v in th],
+        "loglik": ll2,
+        "score": [float(v) for v in g2],
+        "forward_s": t1 - t0,
+        "backward_s": t2 - t1,
+        "particle_ticks_per_s": n2 * 2000 / (t2 - t0),
+        "checkpoint_every": 64,
+    }
+    log("C2", out["adjoint_n1M_t2000"])
+    # batched SMC^2: CPMMH, M = 512, N log-uniform in [256, 8192], T = 2,000
+    T3 = 2000
+    mu3 = mu_schedule(T3)
+    _, y3 = oracle.dgp_regimes(mu3.astype(np.float64), rho=0.97, sigma_z=0.15, seed=42)
+    y3 = y3.astype(np.float32)
+    rng = np.random.default_rng(42)
+    M = 512
+    n3 = log_uniform_counts(rng, M, 256, 8192)
+    mu, phi, sig, seeds = prior_draws(rng, M)
+    b = gpu_bpf_batch.GpuBpfBatch(n3, mu, phi, sig, seeds)
+    blk = b.autotune(y3, 64)
+    b.close()
+    b = gpu_bpf_batch.GpuBpfBatch(n3, mu, phi, sig, seeds, block=blk)
+    b.run(y3[:32], stats=False)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.run(y3[32:T3], stats=False)
+    b.synchronize()
+    batch_pt = float(n3.sum()) * (T3 - 32) / (time.perf_counter() - t0)
+    b.close()
+    loop_pt = looped_plain(gpu_bpf, n3[:64], mu, phi, sig, seeds, y3, 40)
+    t0 = time.perf_counter()
+    s = gpu_bpf_batch.GpuSmc2(M, 256, 8192, seed=42, capacity=T3, block=blk)
+    s.run(y3)
+    wall = time.perf_counter() - t0
+    c = s.counters()
+    tm, tsd = s.theta_mean_std()
+    s.close()
+    out["smc2_cpmmh_m512_t2000"] = {
+        "inner_batch_particle_ticks_per_s": batch_pt,
+        "looped_plain_particle_ticks_per_s": loop_pt,
+        "batch_over_looped": batch_pt / loop_pt,
+        "block": blk,
+        "sum_n": int(n3.sum()),
+        "smc2_wall_s": wall,
+        "smc2_useful_particle_ticks_per_s": c["pt_forward"] / wall,
+        "smc2_total_particle_ticks_per_s": (c["pt_forward"] + c["pt_replay"]) / wall,
+        "counters": c,
+        "theta_mean": [float(v) for v in tm],
+        "theta_sd": [float(v) for v in tsd],
+    }
+    log("C3", out["smc2_cpmmh_m512_t2000"])
+    # batched SMC^2: MALA from per-filter scores, M = 1024, N log-uniform in
+    # [512, 16384], Student-t (nu = 5) observations, T = 5,000
+    T4 = 5000
+    mu4 = mu_schedule(T4)
+    _, y4 = oracle.dgp_regimes(mu4.astype(np.float64), rho=0.97, sigma_z=0.15, nu_obs=5.0, seed=42)
+    y4 = y4.astype(np.float32)
+    M = 1024
+    n4 = log_uniform_counts(rng, M, 512, 16384)
+    mu, phi, sig, seeds = prior_draws(rng, M)
+    b = gpu_bpf_batch.GpuBpfBatch(n4, mu, phi, sig, seeds, nu_obs=5.0)
+    blk4 = b.autotune(y4, 64)
+    b.close()
+    b = gpu_bpf_batch.GpuBpfBatch(n4, mu, phi, sig, seeds, nu_obs=5.0, block=blk4)
+    b.enable_score()
+    b.run(y4[:32], stats=False)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.run(y4[32:1056], stats=False)
+    b.synchronize()
+    batch4 = float(n4.sum()) * 1024 / (time.perf_counter() - t0)
+    b.close()
+    loop4 = looped_plain(gpu_bpf, n4[:16], mu, phi, sig, seeds, y4, 100, nu=5.0, score=True)
+    t0 = time.perf_counter()
+    s = gpu_bpf_batch.GpuSmc2(M, 512, 16384, nu_obs=5.0, seed=42, kind="mala", moves=3,
+                              mala_step=0.5, block=blk4)
+    s.run(y4)
+    wall = time.perf_counter() - t0
+    c = s.counters()
+    tm, tsd = s.theta_mean_std()
+    s.close()
+    out["smc2_mala_m1024_t5000_student_t"] = {
+        "inner_batch_with_score_particle_ticks_per_s": batch4,
+        "looped_plain_score_particle_ticks_per_s": loop4,
+        "batch_over_looped": batch4 / loop4,
+        "block": blk4,
+        "sum_n": int(n4.sum()),
+        "smc2_wall_s": wall,
+        "smc2_useful_particle_ticks_per_s": c["pt_forward"] / wall,
+        "smc2_total_particle_ticks_per_s": (c["pt_forward"] + c["pt_replay"]) / wall,
+        "counters": c,
+        "theta_mean": [float(v) for v in tm],
+        "theta_sd": [float(v) for v in tsd],
+    }
+    log("C4", out["smc2_mala_m1024_t5000_student_t"])
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,9 +253,10 @@ def main():
     ap.add_argument("--particles", type=int, default=51_200_000)
     ap.add_argument("--ticks", type=int, default=5000, help="length of the observation series")
     ap.add_argument("--items", type=int, default=0, help="particles per lane (0 = auto)")
-    ap.add_argument("--cpu-ticks", type=int, default=12)
+    ap.add_argument("--cpu-ticks", type=int, default=100)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the other configurations")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
@@ -198,6 +379,8 @@ def main():
             "sample": f"oracle FP32 filter (OpenMP), N={n}, first {args.cpu_ticks} ticks of the "
                       f"same workload, {dt:.1f} s",
         }
+    if not args.no_extra:
+        res["workloads"] = extra_workloads(args, gpu_bpf, oracle)
     print(json.dumps(res), flush=True)
 
 
