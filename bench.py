@@ -257,7 +257,20 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other configurations")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run 5 device-to-device copies of BYTES (the reference kernel "
+                         "for scaling HBM traffic counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 5}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
               "(this build runs on a single MI355X)", file=sys.stderr)
@@ -368,6 +381,15 @@ def main():
             "traffic": None,
         },
     }
+    # measured HBM bytes per launch of the same kernel (rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE, calibrated on a copy of known size), recorded for this workload
+    tpath = os.path.join(ROOT, "profiles", "k_propagate_sv51M_traffic.json")
+    if n == 51_200_000 and tile == 4096 and os.path.exists(tpath):
+        tr = json.load(open(tpath))
+        res["roofline"]["traffic"] = tr["traffic_bytes_per_launch_mean"]
+        res["roofline"]["traffic_carry_tick"] = tr["traffic_bytes_per_launch_carry_tick"]
+        res["roofline"]["algorithmic_bytes_per_launch"] = bytes_per_launch
+        res["roofline"]["traffic_source"] = tr["source"]
     if not args.no_cpu:
         log("cpu baseline", args.cpu_ticks, "ticks x", n, "threads", args.cpu_threads)
         v, dt = cpu_baseline(y, mu_t, n, args.cpu_ticks, args.cpu_threads)
