@@ -41,6 +41,7 @@ struct GpuBpfBatch {
     int timestep = 0;
     float* d_dx[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};   // tangents
     double* d_score = nullptr;         // 3 per slot when the score is enabled
+    int fisher = 0;                    // score estimator: 0 pathwise, 1 Fisher identity
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_block = 0;
@@ -52,7 +53,7 @@ void free_graph(GpuBpfBatch* b);
 void destroy_batch(GpuBpfBatch* b);
 BatchArgs tick_args(const GpuBpfBatch* b);
 int create_impl(GpuBpfBatch* b);
-int enable_score(GpuBpfBatch* b);
+int enable_score(GpuBpfBatch* b, int fisher);
 int append_y(GpuBpfBatch* b, const float* y, int n_ticks, hipMemcpyKind kind);
 int run_ticks(GpuBpfBatch* b, int n_ticks);
 GpuBpfBatch* new_batch(int M, const int* n, const float* mu, const float* phi, const float* sig,

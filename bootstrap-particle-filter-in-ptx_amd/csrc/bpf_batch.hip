@@ -145,7 +145,12 @@ __global__ __launch_bounds__(kBlock) void k_batch_init(BatchInitArgs a) {
                 if (a.noise == kNoisePcgStore) a.zpool[zc][off + i] = z;
             }
             v = __builtin_fmaf(sd, z, mu);
-            if (a.dx0[0]) {
+            if (a.dx0[0] && a.fisher) {   // grad log N(x_0; mu, sd^2), sd = sd(phi, sigma)
+                const float z2m1 = __builtin_fmaf(z, z, -1.0f);
+                a.dx0[0][off + i] = z / sd;
+                a.dx0[1][off + i] = z2m1 * (dsd_dphi / sd);
+                a.dx0[2][off + i] = z2m1 * (dsd_dsig / sd);
+            } else if (a.dx0[0]) {
                 a.dx0[0][off + i] = 1.0f;
                 a.dx0[1][off + i] = z * dsd_dphi;
                 a.dx0[2][off + i] = z * dsd_dsig;
@@ -170,11 +175,16 @@ __device__ inline float dlw_dx(float x, const ObsModel& o) {
     return __builtin_fmaf(0.5f, eta2, -0.5f);
 }
 
-// TAN: also carry the tangents dx/d(mu, phi, sigma) through propagation and
-// resampling and accumulate the pathwise score of each filter's
-// log-likelihood, sum_t sum_i w_ti g_ti dx_ti with g = d lw / dx (forward
-// mode of the single filter's reverse-mode score; ancestors held fixed).
-template <int BLOCK, bool STUDENT, bool TAN>
+// TAN: also carry a per-particle 3-vector d through propagation and
+// resampling.  FISHER false: d = dx/d(mu, phi, sigma), the tangents, and the
+// filter accumulates the pathwise score of its log-likelihood,
+// sum_t sum_i w_ti g_ti d_ti with g = d lw / dx (forward mode of the single
+// filter's reverse-mode score; ancestors held fixed).  FISHER true: d is the
+// path statistic alpha = grad_theta log p(x_0:t) along each particle's
+// ancestry (the observation density does not depend on theta), and the
+// filter's score is the Fisher-identity estimate sum_i w_ti alpha_ti of the
+// latest tick.
+template <int BLOCK, bool STUDENT, bool TAN, bool FISHER>
 __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
     constexpr int NW = BLOCK / kWave;
     constexpr int TP = kBatchTile + kBatchTile / 16;       // padded tile
@@ -250,7 +260,14 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
             batch_normals(a, m, pa, jm, jg, base, lane, zc, zoff, s_q[wid], z);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                if (TAN) {    // x' = mu + phi (x - mu) + sigma z
+                if (TAN && FISHER) {   // grad log N(x'; mu + phi (x - mu), sigma^2)
+                    const float zs = z[q] / sig;
+                    d[0][q] = __builtin_fmaf(zs, 1.0f - phi, d[0][q]);
+                    d[1][q] = __builtin_fmaf(zs, x[q] - mu, d[1][q]);
+                    d[2][q] += __builtin_fmaf(z[q], z[q], -1.0f) / sig;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) dxb[k][base + 64 * q + lane] = d[k][q];
+                } else if (TAN) {    // x' = mu + phi (x - mu) + sigma z
                     d[0][q] = __builtin_fmaf(phi, d[0][q], 1.0f - phi);
                     d[1][q] = __builtin_fmaf(phi, d[1][q], x[q] - mu);
                     d[2][q] = __builtin_fmaf(phi, d[2][q], z[q]);
@@ -293,7 +310,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
                     acc[0] += ws; acc[1] += wx; acc[2] += wxx; acc[3] += wq;
                 }
                 if (TAN) {
-                    const double eg = (j < n) ? de * (double)dlw_dx<STUDENT>(x[q], obs) : 0.0;
+                    const double eg = (j < n) ? (FISHER ? de : de * (double)dlw_dx<STUDENT>(x[q], obs)) : 0.0;
 #pragma unroll
                     for (int k = 0; k < 3; ++k) acc[4 + k] += wave_sum(eg * (double)d[k][q]);
                 }
@@ -366,7 +383,10 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
             finalize_tick(c, n, a.threshold, gmax, S, SX, SXX, Q, ur_in);
             if (TAN && c.last.mean == c.last.mean) {     // finite statistics
 #pragma unroll
-                for (int k = 0; k < 3; ++k) a.score[3 * m + k] += s_sc[k] / S;
+                for (int k = 0; k < 3; ++k) {
+                    if (FISHER) a.score[3 * m + k] = s_sc[k] / S;
+                    else a.score[3 * m + k] += s_sc[k] / S;
+                }
             }
             if (c.resample) {
                 if (a.noise == kNoisePcgStore) a.upool[zc][ui] = c.u_raw;
@@ -627,17 +647,20 @@ hipError_t launch_batch_init(const BatchInitArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
+template <int BLOCK, bool STUDENT>
+static void launch_batch_tick_s(const BatchArgs& a, int n_blocks, hipStream_t st) {
+    if (!a.score)
+        hipLaunchKernelGGL((k_batch_tick<BLOCK, STUDENT, false, false>), dim3(n_blocks), dim3(BLOCK), 0, st, a);
+    else if (a.fisher)
+        hipLaunchKernelGGL((k_batch_tick<BLOCK, STUDENT, true, true>), dim3(n_blocks), dim3(BLOCK), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_batch_tick<BLOCK, STUDENT, true, false>), dim3(n_blocks), dim3(BLOCK), 0, st, a);
+}
+
 template <int BLOCK>
 static hipError_t launch_batch_tick_b(const BatchArgs& a, int n_blocks, bool student, hipStream_t st) {
-    const bool tan = a.score != nullptr;
-    if (student && tan)
-        hipLaunchKernelGGL((k_batch_tick<BLOCK, true, true>), dim3(n_blocks), dim3(BLOCK), 0, st, a);
-    else if (student)
-        hipLaunchKernelGGL((k_batch_tick<BLOCK, true, false>), dim3(n_blocks), dim3(BLOCK), 0, st, a);
-    else if (tan)
-        hipLaunchKernelGGL((k_batch_tick<BLOCK, false, true>), dim3(n_blocks), dim3(BLOCK), 0, st, a);
-    else
-        hipLaunchKernelGGL((k_batch_tick<BLOCK, false, false>), dim3(n_blocks), dim3(BLOCK), 0, st, a);
+    if (student) launch_batch_tick_s<BLOCK, true>(a, n_blocks, st);
+    else launch_batch_tick_s<BLOCK, false>(a, n_blocks, st);
     return hipGetLastError();
 }
 

@@ -191,6 +191,7 @@ struct BatchArgs {
     float* dx0[3];
     float* dx1[3];
     double* score;           // 3 per slot, or null: no tangents
+    int fisher;              // the 3-vectors are Fisher-identity path statistics instead
 };
 
 struct BatchInitArgs {
@@ -209,6 +210,7 @@ struct BatchInitArgs {
     float cn_rho, cn_s;
     const int* prop_seed;
     float* dx0[3];           // tangents of x_0 (x_0 = mu + sd(phi, sigma) z), or null
+    int fisher;              // dx0 receives grad log p(x_0) instead
 };
 
 // Slot m receives slot anc[m]'s whole inner filter: parameters, particle

@@ -85,12 +85,14 @@ BatchArgs tick_args(const GpuBpfBatch* b) {
             a.dx1[k] = b->d_dx[1][k];
         }
         a.score = b->d_score;
+        a.fisher = b->fisher;
     }
     return a;
 }
 
 // Tangents and score accumulators; x_0 is drawn again with its tangents.
-int enable_score(GpuBpfBatch* b) {
+int enable_score(GpuBpfBatch* b, int fisher) {
+    b->fisher = fisher;
     if (b->d_score) return 0;
     const size_t slots = (size_t)b->M * b->cap;
     for (int h = 0; h < 2; ++h)
@@ -114,6 +116,7 @@ int enable_score(GpuBpfBatch* b) {
     ia.jg = 1u + kPcgMult;
     ia.noise = kNoisePcg;
     for (int k = 0; k < 3; ++k) ia.dx0[k] = b->d_dx[0][k];
+    ia.fisher = fisher;
     HIP_TRY(launch_batch_init(ia, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
     free_graph(b);
@@ -410,13 +413,14 @@ int gpu_bpf_batch_get_resample_counts(GpuBpfBatch* b, int* counts) {
     return 0;
 }
 
-int gpu_bpf_batch_enable_score(GpuBpfBatch* b) {
-    if (!b || b->timestep != 0 || b->threshold > 0.0f) {
-        set_err("gpu_bpf_batch_enable_score: before the first tick, with resampling every tick "
-                "(ess_threshold <= 0)");
+int gpu_bpf_batch_enable_score(GpuBpfBatch* b, int estimator) {
+    if (!b || b->timestep != 0 || b->threshold > 0.0f || (estimator != 0 && estimator != 1) ||
+        (b->d_score && estimator != b->fisher)) {
+        set_err("gpu_bpf_batch_enable_score: estimator 0 or 1, once, before the first tick, with "
+                "resampling every tick (ess_threshold <= 0)");
         return -1;
     }
-    return enable_score(b);
+    return enable_score(b, estimator);
 }
 
 int gpu_bpf_batch_get_score(GpuBpfBatch* b, double* score) {

@@ -745,7 +745,7 @@ int gpu_smc2_init_from_prior(GpuSmc2* s) {
     HIP_TRY(hipMemsetAsync(s->d_halt, 0, 2 * sizeof(int), st));
     HIP_TRY(hipMemsetAsync(s->d_zcur, 0, M * sizeof(int), st));
     if (upload_slots(s, s->main) || fresh_ctl(s, s->main)) return -1;
-    if (!stores_noise(s) && (enable_score(s->main) || enable_score(s->rep))) return -1;
+    if (!stores_noise(s) && (enable_score(s->main, 0) || enable_score(s->rep, 0))) return -1;
     // x_0 of every slot (stored as tick 0 of the noise when moves replay it)
     BatchInitArgs ia;
     memset((void*)&ia, 0, sizeof ia);

@@ -37,7 +37,7 @@ def _lib():
             "gpu_bpf_batch_get_loglik": (I, [P, P]),
             "gpu_bpf_batch_get_resample_counts": (I, [P, P]),
             "gpu_bpf_batch_copy_particles": (I, [P, I, P]),
-            "gpu_bpf_batch_enable_score": (I, [P]),
+            "gpu_bpf_batch_enable_score": (I, [P, I]),
             "gpu_bpf_batch_get_score": (I, [P, P]),
         }
         for name, (res, args) in sig.items():
@@ -124,8 +124,9 @@ class GpuBpfBatch:
         self._check(self.L.gpu_bpf_batch_get_resample_counts(self.h, _ptr(c)))
         return c
 
-    def enable_score(self):
-        self._check(self.L.gpu_bpf_batch_enable_score(self.h))
+    def enable_score(self, estimator="pathwise"):
+        """estimator: "pathwise" (ancestors fixed) or "fisher" (Fisher identity)."""
+        self._check(self.L.gpu_bpf_batch_enable_score(self.h, 1 if estimator == "fisher" else 0))
 
     def score(self):
         sc = np.zeros((self.M, 3), np.float64)

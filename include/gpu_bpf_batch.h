@@ -56,13 +56,15 @@ int gpu_bpf_batch_get_resample_counts(GpuBpfBatch* batch, int* counts);
 /* The current particles of filter m (n_particles[m] floats). */
 int gpu_bpf_batch_copy_particles(GpuBpfBatch* batch, int filter, float* x);
 
-/* Pathwise score of every filter's log-likelihood with respect to
- * (mu, phi, sigma_eta), ancestors held fixed and x_0 = mu + sd z
- * differentiated (the quantity of gpu_bpf_score_backward with x0_reparam =
- * 1), carried forward as per-particle tangents.  Enable before the first
- * tick, with ess_threshold <= 0; the score accumulates over every tick run
- * since.  get_score writes n_filters x 3 doubles. */
-int gpu_bpf_batch_enable_score(GpuBpfBatch* batch);
+/* Score of every filter's log-likelihood with respect to (mu, phi,
+ * sigma_eta), carried forward as a per-particle 3-vector.  estimator 0:
+ * pathwise -- ancestors held fixed and x_0 = mu + sd z differentiated (the
+ * quantity of gpu_bpf_score_backward with x0_reparam = 1), accumulated over
+ * every tick run since.  estimator 1: Fisher identity -- each particle carries
+ * grad log p(x_0:t) along its ancestry and the score is sum_i w_i alpha_i of
+ * the latest tick.  Enable before the first tick, with ess_threshold <= 0.
+ * get_score writes n_filters x 3 doubles. */
+int gpu_bpf_batch_enable_score(GpuBpfBatch* batch, int estimator);
 int gpu_bpf_batch_get_score(GpuBpfBatch* batch, double* score);
 
 /* ---- SMC^2 over the batched inner filters ---------------------------------

@@ -153,7 +153,7 @@ def test_batch_score_matches_adjoint_and_oracle(orc, ydata):
     y = ydata[:T]
     say("batch score", n.tolist())
     b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, ess_threshold=0.0)
-    b.enable_score()
+    b.enable_score("pathwise")
     b.run(y, stats=False)
     sc = b.score()
     ll = b.loglik()
@@ -185,3 +185,35 @@ def test_batch_score_student_t(orc, ydata):
                        nu_obs=5.0, seed=int(seeds[m]), tile=256, x0_reparam=True)[1]
         np.testing.assert_allclose(sc[m], og, rtol=1e-4, atol=1e-4 * np.max(np.abs(og)))
     b.close()
+
+
+def test_fisher_identity_score_matches_the_exact_score(orc, ydata):
+    """Fisher-identity estimate sum_i w_i alpha_i (alpha = grad log p(x_0:t)
+    along each ancestry) over 16 independent filters, against the exact score
+    of the SV log-likelihood: central differences of the point-mass (grid)
+    filter's log-likelihood, which is converged in the grid size here.  (The
+    pathwise estimator -- ancestors held fixed, as tests/diff_bpf.py detaches
+    them -- is biased for this quantity: on these data its mean is ~50% off
+    in every component, so it is pinned against the oracle's tangent instead.)"""
+    y = ydata[:60]
+    th = np.array([-1.0, 0.97, 0.15])
+    h = np.array([1e-4, 1e-5, 1e-5])
+    exact = np.zeros(3)
+    for k in range(3):
+        tp, tm = th.copy(), th.copy()
+        tp[k] += h[k]
+        tm[k] -= h[k]
+        lp = orc.grid_loglik(y, rho=tp[1], sigma_z=tp[2], mu=tp[0], n_grid=2048)[0]
+        lm = orc.grid_loglik(y, rho=tm[1], sigma_z=tm[2], mu=tm[0], n_grid=2048)[0]
+        exact[k] = (lp - lm) / (2 * h[k])
+    M = 16
+    say("fisher vs exact", exact.tolist())
+    b = gpu_bpf_batch.GpuBpfBatch(np.full(M, 16384), th[0], th[1], th[2], np.arange(1, M + 1))
+    b.enable_score("fisher")
+    b.run(y, stats=False)
+    sc = b.score()
+    b.close()
+    mean, se = sc.mean(0), sc.std(0, ddof=1) / np.sqrt(M)
+    say("fisher", mean.tolist(), se.tolist())
+    assert np.all(np.abs(mean - exact) < 4 * se + 0.02 * np.abs(exact)), (mean, se, exact)
+    assert np.all(se < 0.15 * np.abs(exact))        # the check has power
