@@ -28,12 +28,22 @@ __global__ __launch_bounds__(kBlock) void k_init(InitArgs a) {
     long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
     if (i >= a.n_pad) return;
     float x = a.mu;
+    float al[3] = {0.0f, 0.0f, 0.0f};
     if (i < a.n) {
         uint64_t s = a.pcg_a + (uint64_t)(2 * i + 1) * a.pcg_g;
         float z = icdf(pcg_uniform(pcg_output(s)));
         x = __builtin_fmaf(a.sd, z, a.mu);
+        // grad log N(x_0; mu, sd^2) with sd = sd(phi, sigma)
+        const float z2m1 = __builtin_fmaf(z, z, -1.0f);
+        al[0] = z / a.sd;
+        al[1] = z2m1 * (a.dsd_dphi / a.sd);
+        al[2] = z2m1 * (a.dsd_dsig / a.sd);
     }
-    a.x[i] = x;
+    if (a.x) a.x[i] = x;
+    if (a.alpha[0]) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a.alpha[c][i] = al[c];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -228,7 +238,30 @@ __device__ inline void lane_weight_sums(const float (&x)[K], const float (&l)[K]
     }
 }
 
+// FP64 sums e * alpha_c of the lane's particles (padding contributes zero).
 template <int K>
+__device__ inline void lane_fisher_sums(const float (&l)[K], const float (&al)[3][K], float mb, int tid,
+                                        int rem, double (&F)[3]) {
+    constexpr int VEC = K < 4 ? K : 4;
+    constexpr int NV = K / VEC;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) {
+            const int k = v * VEC + c;
+            const int li = (v * kBlock + tid) * VEC + c;
+            const double de = (li < rem) ? (double)expf_cw_nonpos(l[k] - mb) : 0.0;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) F[q] = fma(de, (double)al[q][k], F[q]);
+        }
+    }
+}
+
+// FISH: also carry the Fisher-identity vector alpha = grad log p(x_0:t) of
+// each particle through the transition (alpha += grad log N(x'; mu + phi
+// (x - mu), sigma^2), with z = (x' - mu - phi (x - mu)) / sigma the drawn
+// normal) and write the per-tile sums of e * alpha.
+template <int K, bool FISH>
 __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
     constexpr int VEC = K < 4 ? K : 4;
     constexpr int NV = K / VEC;
@@ -238,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
     __shared__ float s_q[TILE];
     __shared__ int s_cnt[NW];
     __shared__ float s_m[NW];
-    __shared__ double s_d[4][NW];
+    __shared__ double s_d[FISH ? 7 : 4][NW];
 
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
@@ -283,9 +316,36 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
         obs.ysq = yv * yv;
     }
 
+    float al[FISH ? 3 : 1][K];
+    if constexpr (FISH) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                long long off = base + (long long)(v * kBlock + tid) * VEC;
+                load_vec<VEC>(a.alpha[cur][q] + off, &al[q][v * VEC]);
+            }
+    }
+
     if (a.do_prop) {
         float z[K];
         lane_normals<K>(a.pcg_a, a.pcg_g, base, tid, s_q, s_cnt, z);
+        if constexpr (FISH) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const float zs = z[k] / a.sig;
+                al[0][k] = __builtin_fmaf(zs, 1.0f - a.phi, al[0][k]);
+                al[1][k] = __builtin_fmaf(zs, x[k] - a.mu, al[1][k]);
+                al[2][k] += __builtin_fmaf(z[k], z[k], -1.0f) / a.sig;
+            }
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    long long off = base + (long long)(v * kBlock + tid) * VEC;
+                    store_vec<VEC>(a.alpha[cur][q] + off, &al[q][v * VEC]);
+                }
+        }
 #pragma unroll
         for (int k = 0; k < K; ++k) x[k] = transition(x[k], a.mu, a.phi, a.sig, z[k]);
 #pragma unroll
@@ -337,6 +397,15 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
         s_d[2][wid] = SXX;
         s_d[3][wid] = Q;
     }
+    if constexpr (FISH) {
+        double F[3] = {0.0, 0.0, 0.0};
+        if (mb != -__builtin_inff()) lane_fisher_sums<K>(l, al, mb, tid, rem, F);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            F[q] = wave_sum(F[q]);
+            if (lane == 0) s_d[4 + q][wid] = F[q];
+        }
+    }
     __syncthreads();
     if (tid == 0) {
         double r[4];
@@ -352,6 +421,15 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
         a.part.sx[b] = r[1];
         a.part.sxx[b] = r[2];
         a.part.sq[b] = r[3];
+        if constexpr (FISH) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                double acc = s_d[4 + q][0];
+#pragma unroll
+                for (int w = 1; w < NW; ++w) acc += s_d[4 + q][w];
+                a.fsum[(size_t)q * a.part.n_tiles + b] = acc;
+            }
+        }
     }
 }
 
@@ -620,7 +698,8 @@ __device__ inline TileScale tile_scale(const Partials& p, const double* gptil, f
 constexpr int kLaneRunCap = 48;
 constexpr int kLongRuns = 64;
 
-template <int BLOCK, int K>
+// FISH: the Fisher-identity vectors are gathered with x (staged in LDS).
+template <int BLOCK, int K, bool FISH>
 __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
     Ctl* ctl = a.ctl;
     if (!ctl->resample) return;
@@ -639,11 +718,19 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
     __shared__ int s_qa[kLongRuns];
     __shared__ int s_qb[kLongRuns];
     __shared__ int s_tile;
+    __shared__ float s_al[FISH ? 3 : 1][FISH ? TP : 1];
 
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const int n = a.n;
     const float* __restrict__ xs = ctl->src ? a.x1 : a.x0;
     float* __restrict__ xd = ctl->dst ? a.x1 : a.x0;
+    const float* als[3];
+    float* ald[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        als[q] = FISH ? a.alpha[ctl->src][q] : nullptr;
+        ald[q] = FISH ? a.alpha[ctl->dst][q] : nullptr;
+    }
     const float u = ctl->u;
     const int W = ctl->wrap_start;
     const float Mg = ctl->M;
@@ -676,6 +763,18 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
             const int loc0 = (v * BLOCK + tid) * VEC;
             load_vec<VEC>(xs + base + loc0, &xv[v * VEC]);
             if (a.lw) load_vec<VEC>(a.lw + base + loc0, &lv[v * VEC]);
+        }
+        if constexpr (FISH) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    const int loc0 = (v * BLOCK + tid) * VEC;
+                    float t[VEC];
+                    load_vec<VEC>(als[q] + base + loc0, t);
+#pragma unroll
+                    for (int c = 0; c < VEC; ++c) s_al[q][pad16(loc0 + c)] = t[c];
+                }
         }
         const float mb = a.part.m[b];
         const bool dead = (mb == -__builtin_inff());
@@ -741,6 +840,10 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
 #pragma unroll
                 for (int r = 1; r < K; ++r) v = (q == r) ? xl[r] : v;
                 xd[i] = v;
+                if constexpr (FISH) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) ald[c][i] = s_al[c][pad16(tid * K + q)];
+                }
             }
         }
         __syncthreads();
@@ -754,12 +857,21 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
 #pragma unroll
                 for (int r = 0; r < K - 1; ++r) q += (s_c[pad16(owner * K + r)] < t) ? 1 : 0;
                 xd[i] = s_x[pad16(owner * K + q)];
+                if constexpr (FISH) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) ald[c][i] = s_al[c][pad16(owner * K + q)];
+                }
             }
         }
         // wrapped targets (a few at most)
         for (int i = jlo + tid; i < jhi; i += BLOCK) {
             double t = (double)(target_unwrapped(u, i, n) - 1.0f);
-            xd[i] = s_x[pad16(lds_lower_bound(s_c, last, t))];
+            const int j = pad16(lds_lower_bound(s_c, last, t));
+            xd[i] = s_x[j];
+            if constexpr (FISH) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) ald[c][i] = s_al[c][j];
+            }
         }
         __syncthreads();          // the next tile reuses the LDS arrays
     }
@@ -907,8 +1019,49 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(AdjArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Fisher-identity score of the latest tick, sum_i w_i alpha_i: tile b's
+// normalised weight scale is tf_b * exp(gm_g - M) / S (as in tile_scale),
+// applied to the tile sums of e * alpha.  One block, fixed order.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_fisher_score(FisherArgs a) {
+    constexpr int NW = kBlock / kWave;
+    constexpr float NEG_INF = -__builtin_inff();
+    __shared__ double s_r[3][NW];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const float M = a.ctl->M;
+    const double S = a.ctl->S;
+    const int G = a.part.n_tiles;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int b = tid; b < G; b += kBlock) {
+        const float gm = a.part.gm[b / kGroup];
+        const double fg = (gm == NEG_INF) ? 0.0 : (double)expf_cw(gm - M);
+        const double f = a.part.tf[b] * fg;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[c] = fma(f, a.fsum[(size_t)c * G + b], acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        acc[c] = wave_sum(acc[c]);
+        if (lane == 0) s_r[c][wid] = acc[c];
+    }
+    __syncthreads();
+    if (tid < 3) {
+        double r = s_r[tid][0];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) r += s_r[tid][w];
+        a.out[tid] = r / S;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
+
+hipError_t launch_fisher_score(const FisherArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_fisher_score, dim3(1), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
 
 // Element-wise evaluation of the per-particle arithmetic on given inputs:
 // fn 0 = inverse normal CDF of a uniform (central rational or tail branch, as
@@ -942,11 +1095,11 @@ hipError_t launch_init(const InitArgs& a, hipStream_t st) {
 
 // ev (nullable): four events recorded before propagate, before finalise,
 // before resample and after resample.
-template <int K>
+template <int K, bool FISH>
 static hipError_t launch_tick_k(const PropArgs& p, const FinArgs& f, const ResArgs& r,
                                 hipStream_t st, hipEvent_t* ev) {
     if (ev) (void)hipEventRecord(ev[0], st);
-    hipLaunchKernelGGL(k_propagate<K>, dim3(p.part.n_tiles), dim3(kBlock), 0, st, p);
+    hipLaunchKernelGGL((k_propagate<K, FISH>), dim3(p.part.n_tiles), dim3(kBlock), 0, st, p);
     if (ev) (void)hipEventRecord(ev[1], st);
     hipLaunchKernelGGL(k_finalize, dim3(f.part.n_groups), dim3(kFinBlock), 0, st, f);
     if (ev) (void)hipEventRecord(ev[2], st);
@@ -960,7 +1113,7 @@ static hipError_t launch_tick_k(const PropArgs& p, const FinArgs& f, const ResAr
         hipError_t e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         if (e == hipSuccess)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_resample<RB, TILE / RB>, RB, 0);
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_resample<RB, TILE / RB, FISH>, RB, 0);
         if (e != hipSuccess) return e;
         resident = per_cu > 0 && cus > 0 ? per_cu * cus : 1024;
     }
@@ -968,19 +1121,23 @@ static hipError_t launch_tick_k(const PropArgs& p, const FinArgs& f, const ResAr
     // claims of a block overlap its neighbours' tiles; many more blocks make
     // the counter the bottleneck)
     const int grid = r.part.n_tiles < 3 * resident ? r.part.n_tiles : 3 * resident;
-    hipLaunchKernelGGL((k_resample<RB, TILE / RB>), dim3(grid), dim3(RB), 0, st, r);
+    hipLaunchKernelGGL((k_resample<RB, TILE / RB, FISH>), dim3(grid), dim3(RB), 0, st, r);
     if (ev) (void)hipEventRecord(ev[3], st);
     return hipGetLastError();
 }
 
+// The Fisher-identity variants are launched when PropArgs.alpha is set (the
+// resample pass must then carry the same buffers).
 hipError_t launch_tick(int K, const PropArgs& p, const FinArgs& f, const ResArgs& r,
                        hipStream_t st, hipEvent_t* ev) {
+    const bool fish = p.alpha[0][0] != nullptr;
+    if (fish != (r.alpha[0][0] != nullptr)) return hipErrorInvalidValue;
     switch (K) {
-        case 1: return launch_tick_k<1>(p, f, r, st, ev);
-        case 2: return launch_tick_k<2>(p, f, r, st, ev);
-        case 4: return launch_tick_k<4>(p, f, r, st, ev);
-        case 8: return launch_tick_k<8>(p, f, r, st, ev);
-        case 16: return launch_tick_k<16>(p, f, r, st, ev);
+        case 1: return fish ? launch_tick_k<1, true>(p, f, r, st, ev) : launch_tick_k<1, false>(p, f, r, st, ev);
+        case 2: return fish ? launch_tick_k<2, true>(p, f, r, st, ev) : launch_tick_k<2, false>(p, f, r, st, ev);
+        case 4: return fish ? launch_tick_k<4, true>(p, f, r, st, ev) : launch_tick_k<4, false>(p, f, r, st, ev);
+        case 8: return fish ? launch_tick_k<8, true>(p, f, r, st, ev) : launch_tick_k<8, false>(p, f, r, st, ev);
+        case 16: return fish ? launch_tick_k<16, true>(p, f, r, st, ev) : launch_tick_k<16, false>(p, f, r, st, ev);
         default: return hipErrorInvalidValue;
     }
 }

@@ -77,6 +77,12 @@ struct PropArgs {
     int do_prop;          // 0 on tick 0 (ptx_assembly/bpf_kernels_full.ptx:294)
     int store_l;          // keep log-weights for a possible carry
     float* x_save;        // optional copy of the pre-propagation states
+    // Fisher-identity score (null: off): per particle a 3-vector
+    // alpha = grad log p(x_0:t) (component c of buffer q at alpha[q][c]),
+    // double-buffered like x, and per tile FP64 sums of e * alpha_c
+    // (component c at fsum + c * n_tiles)
+    float* alpha[2][3];
+    double* fsum;
 };
 
 struct FinArgs {
@@ -105,6 +111,7 @@ struct ResArgs {
     float* m_log;         // optional per-tile copies for the score's replay
     double* ft_log;
     double* b_log;        // n_tiles + 1
+    float* alpha[2][3];   // Fisher-identity vectors, gathered with x (null: off)
 };
 
 // One reverse-mode tick of the pathwise score.
@@ -130,10 +137,21 @@ struct AdjArgs {
 };
 
 struct InitArgs {
-    float* x;
+    float* x;             // x_0 (null: not written)
     uint64_t pcg_a, pcg_g;
     float mu, sd;
     int n, n_pad;
+    float* alpha[3];      // optional grad log N(x_0; mu, sd^2), zero on padding
+    float dsd_dphi, dsd_dsig;
+};
+
+// Fisher-identity score of the latest tick: sum_i w_i alpha_i from the tile
+// sums and the tile / group scales of the finalise pass.
+struct FisherArgs {
+    Partials part;
+    const double* fsum;
+    const Ctl* ctl;
+    double* out;          // 3 doubles
 };
 
 // ---------------------------------------------------------------------------

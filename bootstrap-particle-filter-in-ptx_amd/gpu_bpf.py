@@ -98,6 +98,7 @@ def load(path=LIB_PATH):
         "gpu_bpf_score_forward": (I, [P, P, I, I, ctypes.POINTER(D)]),
         "gpu_bpf_kernel_tick_times": (I, [P, P, I]),
         "gpu_bpf_score_backward": (I, [P, I, P]),
+        "gpu_bpf_set_score_estimator": (I, [P, I]),
         "gpu_bpf_kernel_times": (I, [P, P, P, I]),
         "gpu_bpf_eval_device_math": (I, [I, P, P, I]),
     }
@@ -232,8 +233,14 @@ class GpuBpf:
         self._y_keep = y
         self._check(self.L.gpu_bpf_run(self.h, _ptr(y), len(y), None, None))
 
+    def set_score_estimator(self, estimator):
+        """'pathwise' (default) or 'fisher' for the next score_forward."""
+        code = {"pathwise": 0, "fisher": 1}[estimator]
+        self._check(self.L.gpu_bpf_set_score_estimator(self.h, code))
+
     def score_forward(self, y, checkpoint_every=64):
-        """Checkpointed forward pass of the pathwise score; returns LL (FP64)."""
+        """Forward pass of the score (checkpointed for the pathwise
+        estimator, complete for the Fisher-identity one); returns LL (FP64)."""
         y = np.ascontiguousarray(y, np.float32)
         ll = ctypes.c_double()
         self._check(self.L.gpu_bpf_score_forward(self.h, _ptr(y), len(y), int(checkpoint_every),
@@ -241,7 +248,8 @@ class GpuBpf:
         return ll.value
 
     def score_backward(self, x0_reparam=True):
-        """Reverse sweep; returns dLL/d(mu, phi, sigma_z)."""
+        """Reverse sweep (pathwise) or the forward pass's result (Fisher);
+        returns dLL/d(mu, phi, sigma_z)."""
         g = np.zeros(3, np.float64)
         self._check(self.L.gpu_bpf_score_backward(self.h, 1 if x0_reparam else 0, _ptr(g)))
         return g

@@ -173,6 +173,15 @@ int gpu_bpf_score_forward(GpuBpfState* state, const float* y, int n_ticks,
                           int checkpoint_every, double* loglik);
 int gpu_bpf_score_backward(GpuBpfState* state, int x0_reparam, double* score);
 
+/* Estimator of the next gpu_bpf_score_forward: 0 = pathwise (above, the
+ * default), 1 = Fisher identity -- each particle carries
+ * alpha = grad log p(x_0:t | mu, phi, sigma_z) along its ancestry (initial
+ * stationary density included), and the score is sum_i w_i alpha_i of the
+ * last tick.  With 1, score_forward computes the score in its single pass
+ * (checkpoint_every is not used; memory 6 * N floats) and score_backward
+ * returns it (x0_reparam is not used).  No counterpart in the reference. */
+int gpu_bpf_set_score_estimator(GpuBpfState* state, int estimator);
+
 /* Per-kernel timing with HIP events recorded on the state's own stream
  * around each of the tick's three kernels (propagate, finalise, resample).
  * Enable before enqueuing ticks; gpu_bpf_kernel_times synchronises, then

@@ -44,3 +44,22 @@ def grad(theta0, z, anc, y, x0_reparam=True, nu_obs=0.0):
                 torch.as_tensor(y, dtype=torch.float64), x0_reparam, nu_obs)
     ll.backward()
     return ll.item(), th.grad.numpy().copy()
+
+
+def exact_score(orc, y, theta, h=(1e-4, 1e-5, 1e-5), n_grid=2048):
+    """Score of the exact SV log-likelihood log p(y_0:T-1 | mu, phi, sigma)
+    with Gaussian observations: central differences of the oracle's
+    point-mass (grid) filter, which is converged in the grid size for these
+    series.  theta = (mu, phi, sigma)."""
+    import numpy as np
+
+    th = np.asarray(theta, np.float64)
+    out = np.zeros(3)
+    for k in range(3):
+        tp, tm = th.copy(), th.copy()
+        tp[k] += h[k]
+        tm[k] -= h[k]
+        lp = orc.grid_loglik(y, rho=tp[1], sigma_z=tp[2], mu=tp[0], n_grid=n_grid)[0]
+        lm = orc.grid_loglik(y, rho=tm[1], sigma_z=tm[2], mu=tm[0], n_grid=n_grid)[0]
+        out[k] = (lp - lm) / (2 * h[k])
+    return out

@@ -11,6 +11,7 @@ import pytest
 
 import gpu_bpf
 import gpu_bpf_batch
+import score_reference
 
 pytestmark = pytest.mark.gpu
 
@@ -197,15 +198,7 @@ def test_fisher_identity_score_matches_the_exact_score(orc, ydata):
     in every component, so it is pinned against the oracle's tangent instead.)"""
     y = ydata[:60]
     th = np.array([-1.0, 0.97, 0.15])
-    h = np.array([1e-4, 1e-5, 1e-5])
-    exact = np.zeros(3)
-    for k in range(3):
-        tp, tm = th.copy(), th.copy()
-        tp[k] += h[k]
-        tm[k] -= h[k]
-        lp = orc.grid_loglik(y, rho=tp[1], sigma_z=tp[2], mu=tp[0], n_grid=2048)[0]
-        lm = orc.grid_loglik(y, rho=tm[1], sigma_z=tm[2], mu=tm[0], n_grid=2048)[0]
-        exact[k] = (lp - lm) / (2 * h[k])
+    exact = score_reference.exact_score(orc, y, th)
     M = 16
     say("fisher vs exact", exact.tolist())
     b = gpu_bpf_batch.GpuBpfBatch(np.full(M, 16384), th[0], th[1], th[2], np.arange(1, M + 1))

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import gpu_bpf
+import gpu_bpf_batch
 import score_reference
 
 pytestmark = pytest.mark.gpu
@@ -136,3 +137,54 @@ def test_autograd_function_matches_torch_restatement(orc, ydata):
     tl, tg = score_reference.grad(theta0, z, anc, y.astype(np.float64))
     assert abs(ll.item() - tl) <= 1e-5 * abs(tl)
     np.testing.assert_allclose(th.grad.numpy(), tg, rtol=1e-4, atol=1e-4 * np.max(np.abs(tg)))
+
+
+def fisher_score(n, y, nu=0.0, theta=(MU, RHO, SIG), seed=42):
+    mu, rho, sig = theta
+    f = gpu_bpf.GpuBpf(n, rho, sig, mu, nu_obs=nu, seed=seed)
+    f.set_score_estimator("fisher")
+    ll = f.score_forward(y)
+    g = f.score_backward()
+    f.close()
+    return ll, g
+
+
+@pytest.mark.parametrize("n,nu", [(3000, 0.0), (16384, 0.0), (5000, 5.0)])
+def test_fisher_single_filter_equals_batch(ydata, n, nu):
+    """The single filter's Fisher-identity estimate equals the batched
+    filter's with one inner filter: the same particles, ancestors and FP32
+    alpha vectors; only the association of the FP64 sums differs."""
+    y = ydata[:150]
+    theta = (-0.7, 0.95, 0.22)
+    say("fisher single vs batch", n, nu)
+    ll, g = fisher_score(n, y, nu=nu, theta=theta, seed=11)
+    b = gpu_bpf_batch.GpuBpfBatch([n], theta[0], theta[1], theta[2], [11], nu_obs=nu, ess_threshold=0.0)
+    b.enable_score("fisher")
+    b.run(y, stats=False)
+    gb, llb = b.score()[0], b.loglik()[0]
+    b.close()
+    assert abs(ll - llb) <= 1e-12 * max(abs(llb), 1.0)
+    np.testing.assert_allclose(g, gb, rtol=1e-9, atol=1e-12 * np.max(np.abs(gb)))
+
+
+def test_fisher_single_filter_matches_exact_score(orc, ydata):
+    """Fisher-identity estimates of four 2^20-particle filters (independent
+    seeds) against the exact score of the SV log-likelihood (central
+    differences of the point-mass filter)."""
+    y = ydata[:60]
+    th = (MU, RHO, SIG)
+    exact = score_reference.exact_score(orc, y, th)
+    say("fisher single vs exact", exact.tolist())
+    gs = np.array([fisher_score(1 << 20, y, seed=s)[1] for s in (1, 2, 3, 4)])
+    mean, se = gs.mean(0), gs.std(0, ddof=1) / 2.0
+    say("fisher single", mean.tolist(), se.tolist())
+    assert np.all(np.abs(mean - exact) < 4 * se + 0.02 * np.abs(exact)), (mean, se, exact)
+    assert np.all(se < 0.1 * np.abs(exact))        # the check has power
+
+
+def test_fisher_estimator_argument_checked():
+    f = gpu_bpf.GpuBpf(1000, RHO, SIG, MU, seed=1)
+    with pytest.raises(KeyError):
+        f.set_score_estimator("bogus")
+    assert f.L.gpu_bpf_set_score_estimator(f.h, 7) == -1
+    f.close()
