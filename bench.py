@@ -47,6 +47,11 @@ def segments(mu_t):
     return list(zip(cuts[:-1], cuts[1:]))
 
 
+def nanmean_or_none(v):
+    v = v[np.isfinite(v)]
+    return float(v.mean()) if v.size else None
+
+
 def enqueue_run(f, L, y_dev_ptr, mu_t, t0, t1):
     """Enqueue ticks [t0, t1) with the mu schedule; observations already in HBM."""
     for a, b in segments(mu_t[t0:t1]):
@@ -254,6 +259,8 @@ def main():
     ap.add_argument("--ticks", type=int, default=5000, help="length of the observation series")
     ap.add_argument("--items", type=int, default=0, help="particles per lane (0 = auto)")
     ap.add_argument("--cpu-ticks", type=int, default=100)
+    ap.add_argument("--timing-stride", type=int, default=10,
+                    help="HIP-event timing of every n-th tick of the timed region")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other configurations")
@@ -303,7 +310,9 @@ def main():
     f = gpu_bpf.GpuBpf(n, 0.97, 0.15, float(mu_t[0]), seed=42, ess_threshold=0.5,
                        items_per_thread=args.items or None)
     tile, n_tiles = f.layout()
-    f.enable_kernel_timing(True)
+    # HIP events around the three kernels of every timing_stride-th tick
+    # (each event adds ~2 us to the stream, so not every tick is timed)
+    f.enable_kernel_timing(True, stride=args.timing_stride)
     K = args.steps
     log("timed", K, "ticks, N =", n, "tile", tile)
     torch.cuda.synchronize()
@@ -334,8 +343,10 @@ def main():
     last_rs = int(st["resampled"][-1])
     n_carry = (K - 1) - (rcount - last_rs)     # ticks 1..K-1 whose predecessor kept weights
     bytes_prop = n * (8.0 + 12.0 * n_prop + 4.0 * n_carry)
+    # the timed launches are propagating ticks (tick 0 is not sampled): their
+    # mean algorithmic bytes are those of ticks 1..K-1
     avg_prop_s = ms[0] / 1e3 / max(cnt[0], 1)
-    bytes_per_launch = bytes_prop / K
+    bytes_per_launch = n * (12.0 * n_prop + 4.0 * n_carry) / n_prop
     achieved = bytes_per_launch / avg_prop_s
     # whole-tick algorithmic traffic incl. resampling (read x', l; write x)
     bytes_total = bytes_prop + 12.0 * n * rcount
@@ -367,8 +378,9 @@ def main():
                 "propagate": ms[0] / max(cnt[0], 1),
                 "finalize": ms[1] / max(cnt[1], 1),
                 "resample": ms[2] / max(cnt[2], 1),
-                "resample_on_resample_ticks": float(tm[rs_mask, 2].mean()) if rs_mask.any() else None,
-                "resample_on_other_ticks": float(tm[~rs_mask, 2].mean()) if (~rs_mask).any() else None,
+                "resample_on_resample_ticks": nanmean_or_none(tm[rs_mask, 2]),
+                "resample_on_other_ticks": nanmean_or_none(tm[~rs_mask, 2]),
+                "timed_ticks": int(cnt[0]),
             },
         },
         "roofline": {

@@ -13,6 +13,7 @@
 //                              write of the offspring into the other buffer
 // The reference issues 12-16 launches and two host syncs per tick
 // (gpu_bpf/gpu_bpf_ptx_full.cu:591-745).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "bpf_common.hpp"
@@ -52,16 +53,16 @@ __global__ __launch_bounds__(kBlock) void k_init(InitArgs a) {
 // of the lane adds 2 pcg_g), uniform, Acklam ICDF.  Central draws are
 // evaluated in place; tail uniforms (4.85%) go through an LDS queue that the
 // whole block evaluates densely, so no wave runs the long tail branch for a
-// single lane.  Item (v, c) of lane tid is particle base + (v*kBlock+tid)*VEC+c.
+// single lane.  Item (v, c) of lane tid is particle base + (v*BLOCK+tid)*VEC+c.
 // Contains two block barriers; every thread of the block must call it.
 // ---------------------------------------------------------------------------
 
-template <int K>
+template <int BLOCK, int K>
 __device__ inline void lane_normals(uint64_t pcg_a, uint64_t pcg_g, long long base, int tid,
                                     float* s_q, int* s_cnt, float (&z)[K]) {
     constexpr int VEC = K < 4 ? K : 4;
     constexpr int NV = K / VEC;
-    constexpr int NW = kBlock / kWave;
+    constexpr int NW = BLOCK / kWave;
     const int lane = tid & (kWave - 1);
     const int wid = tid / kWave;
     bool tail[K];
@@ -69,7 +70,7 @@ __device__ inline void lane_normals(uint64_t pcg_a, uint64_t pcg_g, long long ba
     const uint64_t g2 = pcg_g * 2u;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-        long long i0 = base + (long long)(v * kBlock + tid) * VEC;
+        long long i0 = base + (long long)(v * BLOCK + tid) * VEC;
         uint64_t s = pcg_a + (uint64_t)(2 * i0 + 1) * pcg_g;
 #pragma unroll
         for (int c = 0; c < VEC; ++c) {
@@ -102,7 +103,7 @@ __device__ inline void lane_normals(uint64_t pcg_a, uint64_t pcg_g, long long ba
         pos += tail[k] ? 1 : 0;
     }
     __syncthreads();
-    for (int e = tid; e < total; e += kBlock) s_q[e] = icdf_tail(s_q[e]);
+    for (int e = tid; e < total; e += BLOCK) s_q[e] = icdf_tail(s_q[e]);
     __syncthreads();
     pos = first;
 #pragma unroll
@@ -329,7 +330,7 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
 
     if (a.do_prop) {
         float z[K];
-        lane_normals<K>(a.pcg_a, a.pcg_g, base, tid, s_q, s_cnt, z);
+        lane_normals<kBlock, K>(a.pcg_a, a.pcg_g, base, tid, s_q, s_cnt, z);
         if constexpr (FISH) {
 #pragma unroll
             for (int k = 0; k < K; ++k) {
@@ -889,12 +890,12 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
 // non-decreasing targets.  The CDF is rebuilt bit for bit as in k_resample.
 // ---------------------------------------------------------------------------
 
-template <int K>
-__global__ __launch_bounds__(kBlock) void k_adjoint(AdjArgs a) {
+template <int BLOCK, int K>
+__global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a) {
     constexpr int VEC = K < 4 ? K : 4;
     constexpr int NV = K / VEC;
-    constexpr int TILE = kBlock * K;
-    constexpr int NW = kBlock / kWave;
+    constexpr int TILE = BLOCK * K;
+    constexpr int NW = BLOCK / kWave;
     __shared__ int s_bu[TILE];     // children run end (unwrapped targets) per parent
     __shared__ int s_cnt[NW];
     __shared__ double s_c[TILE + TILE / 16];
@@ -913,11 +914,11 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(AdjArgs a) {
     float x[K], pre[K], z[K];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-        long long off = base + (long long)(v * kBlock + tid) * VEC;
+        long long off = base + (long long)(v * BLOCK + tid) * VEC;
         load_vec<VEC>(a.pre + off, &pre[v * VEC]);
     }
     // draw t of the streams (draw 0, the initial one, at tick 0)
-    lane_normals<K>(a.pcg_a, a.pcg_g, base, tid, s_q, s_cnt, z);
+    lane_normals<BLOCK, K>(a.pcg_a, a.pcg_g, base, tid, s_q, s_cnt, z);
 #pragma unroll
     for (int k = 0; k < K; ++k) x[k] = (a.tick > 0) ? transition(pre[k], a.mu, a.phi, a.sig, z[k]) : pre[k];
     ObsModel obs = a.obs;
@@ -929,7 +930,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(AdjArgs a) {
 #pragma unroll
         for (int c = 0; c < VEC; ++c) {
             const int k = v * VEC + c;
-            const bool valid = base + (long long)(v * kBlock + tid) * VEC + c < n;
+            const bool valid = base + (long long)(v * BLOCK + tid) * VEC + c < n;
             float lk = log_weight(x[k], obs);
             e[k] = (dead || !valid) ? 0.0f : expf_cw(lk - mb);
             float eta2 = obs.ysq * expf_cw(-x[k]);
@@ -939,7 +940,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(AdjArgs a) {
     const double ft = a.ftil[b];
     const float u = *a.u_ptr;
     const int W = *a.w_ptr;
-    const int last = tile_cdf<kBlock, K>(e, b, n, a.n_tiles, ft, a.btil[b], a.btil[b + 1], u, W,
+    const int last = tile_cdf<BLOCK, K>(e, b, n, a.n_tiles, ft, a.btil[b], a.btil[b + 1], u, W,
                                  s_c, s_w, s_rng);
     const int ilo = s_rng[0], ihi = s_rng[1], jlo = s_rng[2], jhi = s_rng[3];
 
@@ -949,7 +950,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(AdjArgs a) {
         for (int v = 0; v < NV; ++v) {
 #pragma unroll
             for (int c = 0; c < VEC; ++c) {
-                const int loc = (v * kBlock + tid) * VEC + c;
+                const int loc = (v * BLOCK + tid) * VEC + c;
                 const double cj = s_c[pad16(loc)];
                 s_bu[loc] = (loc >= last) ? ihi : count_le_fast(cj, u, ilo, ihi, n);
             }
@@ -965,7 +966,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(AdjArgs a) {
 #pragma unroll
         for (int c = 0; c < VEC; ++c) {
             const int k = v * VEC + c;
-            const int loc = (v * kBlock + tid) * VEC + c;
+            const int loc = (v * BLOCK + tid) * VEC + c;
             const bool valid = base + loc < n;
             double xb = (double)e[k] * ft * (double)gp[k];
             if (a.xbar_next && valid) {
@@ -993,7 +994,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(AdjArgs a) {
                 g2 += xb * (double)z[k] * a.dsd_dsig;
             }
         }
-        long long off = base + (long long)(v * kBlock + tid) * VEC;
+        long long off = base + (long long)(v * BLOCK + tid) * VEC;
         store_vec<VEC>(a.xbar_out + off, xb_out);
     }
     g0 = wave_sum(g0);
@@ -1093,16 +1094,21 @@ hipError_t launch_init(const InitArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
-// ev (nullable): four events recorded before propagate, before finalise,
-// before resample and after resample.
+// Launch on st; with events, the start and stop events are tied to the
+// kernel's own dispatch (no marker packets between the kernels).
+template <typename Kern, typename Arg>
+static void launch_timed(Kern k, dim3 grid, dim3 block, hipStream_t st, const hipEvent_t* ev,
+                         const Arg& a) {
+    if (ev) hipExtLaunchKernelGGL(k, grid, block, 0, st, ev[0], ev[1], 0, a);
+    else hipLaunchKernelGGL(k, grid, block, 0, st, a);
+}
+
+// ev (nullable): start/stop event pairs of propagate, finalise, resample.
 template <int K, bool FISH>
 static hipError_t launch_tick_k(const PropArgs& p, const FinArgs& f, const ResArgs& r,
                                 hipStream_t st, hipEvent_t* ev) {
-    if (ev) (void)hipEventRecord(ev[0], st);
-    hipLaunchKernelGGL((k_propagate<K, FISH>), dim3(p.part.n_tiles), dim3(kBlock), 0, st, p);
-    if (ev) (void)hipEventRecord(ev[1], st);
-    hipLaunchKernelGGL(k_finalize, dim3(f.part.n_groups), dim3(kFinBlock), 0, st, f);
-    if (ev) (void)hipEventRecord(ev[2], st);
+    launch_timed(k_propagate<K, FISH>, dim3(p.part.n_tiles), dim3(kBlock), st, ev, p);
+    launch_timed(k_finalize, dim3(f.part.n_groups), dim3(kFinBlock), st, ev ? ev + 2 : nullptr, f);
     // resample runs the same tile with 512 lanes (TILE >= 512): twice the
     // waves per CU and half the per-lane chain of the 256-lane layout
     constexpr int TILE = kBlock * K;
@@ -1121,8 +1127,7 @@ static hipError_t launch_tick_k(const PropArgs& p, const FinArgs& f, const ResAr
     // claims of a block overlap its neighbours' tiles; many more blocks make
     // the counter the bottleneck)
     const int grid = r.part.n_tiles < 3 * resident ? r.part.n_tiles : 3 * resident;
-    hipLaunchKernelGGL((k_resample<RB, TILE / RB, FISH>), dim3(grid), dim3(RB), 0, st, r);
-    if (ev) (void)hipEventRecord(ev[3], st);
+    launch_timed(k_resample<RB, TILE / RB, FISH>, dim3(grid), dim3(RB), st, ev ? ev + 4 : nullptr, r);
     return hipGetLastError();
 }
 
@@ -1142,9 +1147,13 @@ hipError_t launch_tick(int K, const PropArgs& p, const FinArgs& f, const ResArgs
     }
 }
 
+// The adjoint runs each tile with the resample pass's lane layout, so that
+// its in-tile FP64 CDF (tile_cdf) has the same association, bit for bit.
 template <int K>
 static hipError_t launch_adj_k(const AdjArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_adjoint<K>, dim3(a.n_tiles), dim3(kBlock), 0, st, a);
+    constexpr int TILE = kBlock * K;
+    constexpr int RB = TILE >= 512 ? 512 : TILE;
+    hipLaunchKernelGGL((k_adjoint<RB, TILE / RB>), dim3(a.n_tiles), dim3(RB), 0, st, a);
     return hipGetLastError();
 }
 

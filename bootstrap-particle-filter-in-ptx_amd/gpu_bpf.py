@@ -254,14 +254,15 @@ class GpuBpf:
         self._check(self.L.gpu_bpf_score_backward(self.h, 1 if x0_reparam else 0, _ptr(g)))
         return g
 
-    def enable_kernel_timing(self, on=True):
-        self._check(self.L.gpu_bpf_enable_kernel_timing(self.h, 1 if on else 0))
+    def enable_kernel_timing(self, on=True, stride=1):
+        """Event timing of the three kernels of every stride-th tick."""
+        self._check(self.L.gpu_bpf_enable_kernel_timing(self.h, int(stride) if on else 0))
 
     def kernel_tick_times(self, n_ticks):
         """[n, 3] per-tick milliseconds (propagate, finalise, resample)."""
         ms = np.zeros((n_ticks, 3), np.float32)
         got = self._check(self.L.gpu_bpf_kernel_tick_times(self.h, _ptr(ms), n_ticks))
-        return ms[n_ticks - got:] if got < n_ticks else ms
+        return ms[:got]
 
     def kernel_times(self, reset=True):
         """(ms[3], launches[3]) for propagate, finalise, resample."""

@@ -182,15 +182,16 @@ int gpu_bpf_score_backward(GpuBpfState* state, int x0_reparam, double* score);
  * returns it (x0_reparam is not used).  No counterpart in the reference. */
 int gpu_bpf_set_score_estimator(GpuBpfState* state, int estimator);
 
-/* Per-kernel timing with HIP events recorded on the state's own stream
- * around each of the tick's three kernels (propagate, finalise, resample).
- * Enable before enqueuing ticks; gpu_bpf_kernel_times synchronises, then
- * returns the summed milliseconds and launch counts accumulated since the
- * last reset (arrays of 3: propagate, finalise, resample). */
+/* Per-kernel timing with HIP start/stop events tied to the dispatches of
+ * the tick's three kernels (propagate, finalise, resample) on the state's
+ * own stream, for every enable-th tick (1 = every tick, 0 = off).  Enable before enqueuing ticks;
+ * gpu_bpf_kernel_times synchronises, then returns the summed milliseconds
+ * and launch counts of the timed ticks since the last reset (arrays of 3:
+ * propagate, finalise, resample). */
 int gpu_bpf_enable_kernel_timing(GpuBpfState* state, int enable);
 int gpu_bpf_kernel_times(GpuBpfState* state, double* ms, int* launches, int reset);
-/* The per-tick milliseconds (3 per tick) of the last n_ticks timed ticks;
- * returns how many ticks were copied. */
+/* The per-tick milliseconds (3 per tick, NaN where not timed) of the last
+ * n_ticks timed ticks; returns how many ticks were copied. */
 int gpu_bpf_kernel_tick_times(GpuBpfState* state, float* ms, int n_ticks);
 
 /* Verification aid (no reference counterpart): evaluates the device's
