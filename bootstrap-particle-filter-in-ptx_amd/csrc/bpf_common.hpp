@@ -165,7 +165,6 @@ __device__ inline void finalize_tick(Ctl& c, int n_particles, float threshold, f
         c.carry = 0;
         c.off = 0.0;
         c.resample_count += 1;
-        c.tile_next = 0;
     } else {
         c.carry = 1;
         c.off += Md;
@@ -183,7 +182,21 @@ __device__ inline void finalize_tick(Ctl& c, int n_particles, float threshold, f
 // Systematic targets t_i = u + i/N (wrapped once at 1 from i = W on).
 // ---------------------------------------------------------------------------
 
-__device__ inline int count_le(double v, float u, int lo, int hi, int n, bool wrapped) {
+// The target counts are taken by a few threads per tile; kept out of line so
+// that their loops do not raise the register allocation of the kernels that
+// call them.
+__device__ __noinline__ int count_le(double v, float u, int lo, int hi, int n, bool wrapped) {
+    if (lo >= hi) return lo;
+    // the ends first: all or none of the range (the usual answer for the
+    // wrapped targets, which all lie below u)
+    {
+        float t = target_unwrapped(u, hi - 1, n);
+        if (wrapped) t -= 1.0f;
+        if ((double)t <= v) return hi;
+        t = target_unwrapped(u, lo, n);
+        if (wrapped) t -= 1.0f;
+        if ((double)t > v) return lo;
+    }
     while (lo < hi) {
         int mid = (int)(((unsigned)lo + (unsigned)hi) >> 1);
         float t = target_unwrapped(u, mid, n);
@@ -197,7 +210,7 @@ __device__ inline int count_le(double v, float u, int lo, int hi, int n, bool wr
 // exact correction with the float target rule; falls back to the binary
 // search if the estimate is off by more than a few (it is off by at most
 // about N * 2^-24 positions, so this does not happen below ~2^28 particles).
-__device__ inline int count_le_fast(double v, float u, int lo, int hi, int n) {
+__device__ __noinline__ int count_le_fast(double v, float u, int lo, int hi, int n) {
     if (lo >= hi) return lo;
     double est = floor((v - (double)u) * (double)n) + 1.0;
     est = fmin(fmax(est, (double)lo), (double)hi);

@@ -540,6 +540,7 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
     }
     __syncthreads();
     if (!s_flag) return;
+    if (tid < kClaimRanges) a.tile_ctr[tid * kClaimStride] = 0;
 
     // FP64 exclusive prefix of the group weights, normalised by S, clamped to
     // [0, 1] and made non-decreasing by a prefix maximum (the block scan's
@@ -589,10 +590,11 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
 // (gpu_bpf/gpu_bpf.cu:297-310, clamp to N-1 as ptx:816-830) on this CDF.
 // ---------------------------------------------------------------------------
 
+// next_out (nullable): thread 0 stores `next` there before the final barrier.
 template <int BLOCK, int K>
 __device__ inline int tile_cdf(const float (&e)[K], int b, int n, int n_tiles, double ft,
                                double Bb, double Bb1, float u, int W, double* s_c,
-                               double* s_w, int* s_rng) {
+                               double* s_w, int* s_rng, int* next_out = nullptr, int next = 0) {
     constexpr int VEC = K < 4 ? K : 4;
     constexpr int NV = K / VEC;
     constexpr int TILE = BLOCK * K;
@@ -630,25 +632,26 @@ __device__ inline int tile_cdf(const float (&e)[K], int b, int n, int n_tiles, d
 
     const long long lastll = (b == n_tiles - 1) ? ((long long)n - 1 - base) : (long long)(TILE - 1);
     const int last = (int)lastll;
-    double cv[K];
+    // c_j = min(Bb + ft L_j, Bb1) is non-decreasing within the lane; the
+    // lane's last value first (the same sequential sum, recomputed below
+    // rather than held in registers)
     double L = excl;
 #pragma unroll
-    for (int q = 0; q < K; ++q) {
-        L += (double)ev[q];
-        cv[q] = fmin(fma(ft, L, Bb), Bb1);
-    }
+    for (int q = 0; q < K; ++q) L += (double)ev[q];
     // make the in-tile CDF non-decreasing across lanes (prefix max)
-    double pm = wave_incl_max(cv[K - 1], lane);
+    double pm = wave_incl_max(fmin(fma(ft, L, Bb), Bb1), lane);
     double prev = __shfl_up(pm, 1, kWave);
     prev = (lane == 0) ? Bb : prev;
     __syncthreads();   // s_w reads and s_e (aliasing s_c) reads are done
     if (lane == kWave - 1) s_w[wid] = pm;
     __syncthreads();
     for (int w = 0; w < wid; ++w) prev = fmax(prev, s_w[w]);
+    L = excl;
 #pragma unroll
     for (int q = 0; q < K; ++q) {
         const int j = tid * K + q;
-        double c = fmax(cv[q], prev);
+        L += (double)ev[q];
+        double c = fmax(fmin(fma(ft, L, Bb), Bb1), prev);
         c = (j == last) ? Bb1 : c;
         c = (j > last) ? __builtin_inf() : c;
         s_c[pad16(j)] = c;
@@ -660,6 +663,7 @@ __device__ inline int tile_cdf(const float (&e)[K], int b, int n, int n_tiles, d
         else if (tid == 2) r = (b == 0) ? W : count_le(Bb, u, W, n, n, true);
         else               r = (b == n_tiles - 1) ? n : count_le(Bb1, u, W, n, n, true);
         s_rng[tid] = r;
+        if (tid == 0 && next_out) *next_out = next;
     }
     __syncthreads();
     return last;
@@ -692,12 +696,24 @@ __device__ inline TileScale tile_scale(const Partials& p, const double* gptil, f
     return ts;
 }
 
-// lower_bound over the padded in-tile CDF c[pad16(0..last)].
-// Children runs longer than this are written by the whole block, so that a
-// heavy parent (degenerate weights) cannot hold one lane for the tile; up to
-// kLongRuns of them per tile (further ones are written by their lanes).
-constexpr int kLaneRunCap = 48;
-constexpr int kLongRuns = 64;
+// Next tile for a block that works in range r (tiles [r R, (r+1) R)): from
+// its own range while that lasts, then from the others in turn; n_tiles when
+// all are exhausted.  r and tried (ranges found exhausted) are the block's
+// state (thread 0).
+__device__ inline int claim_tile(int* ctr, int& r, int& tried, int n_tiles) {
+    const int R = (n_tiles + kClaimRanges - 1) / kClaimRanges;
+    while (tried < kClaimRanges) {
+        const int lo = r * R;
+        const int len = min(R, n_tiles - lo);
+        if (len > 0) {
+            const int t = atomicAdd(&ctr[r * kClaimStride], 1);
+            if (t < len) return lo + t;
+        }
+        r = (r + 1) % kClaimRanges;
+        ++tried;
+    }
+    return n_tiles;
+}
 
 // FISH: the Fisher-identity vectors are gathered with x (staged in LDS).
 template <int BLOCK, int K, bool FISH>
@@ -713,15 +729,10 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
     __shared__ double s_c[TP];
     __shared__ double s_w[NW];
     __shared__ int s_rng[4];
-    __shared__ int s_edge[NW];
-    __shared__ int s_nq;
-    __shared__ int s_q[kLongRuns];     // lanes whose children run is long
-    __shared__ int s_qa[kLongRuns];
-    __shared__ int s_qb[kLongRuns];
     __shared__ int s_tile;
     __shared__ float s_al[FISH ? 3 : 1][FISH ? TP : 1];
 
-    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const int tid = threadIdx.x;
     const int n = a.n;
     const float* __restrict__ xs = ctl->src ? a.x1 : a.x0;
     float* __restrict__ xd = ctl->dst ? a.x1 : a.x0;
@@ -744,27 +755,33 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
 
     // A grid of resident blocks takes tiles from a work counter (reset by
     // the finalise pass): one launch wave, so a tick without resampling costs
-    // one short wave of early exits, and heavy tiles (long offspring runs)
-    // do not pile up on a few blocks.  Each block claims its next tile while
-    // it loads the current one and publishes the claim before it stores
-    // anything (waiting on the claim then waits on no store); every block
-    // leaves once the counter has passed the last tile.
-    int nb = 0;
-    if (tid == 0) s_tile = atomicAdd(&ctl->tile_next, 1);
+    // one short wave of early exits, and heavy tiles do not pile up on a few
+    // blocks.  Each block claims its next tile while it works on the current
+    // one, publishes the claim with the tile's CDF (before any store, so
+    // waiting on the claim waits on no store) and loads the next tile's
+    // states and log-weights while it writes the current tile's offspring;
+    // every block leaves once the counter has passed the last tile.
+    // (block-uniform values read from LDS go through readfirstlane, so that
+    // tile bases are scalar and per-lane addresses 32-bit offsets)
+    int cr = blockIdx.x % kClaimRanges, ctried = 0;   // blocks alternate between XCDs
+    if (tid == 0) s_tile = claim_tile(a.tile_ctr, cr, ctried, a.part.n_tiles);
     __syncthreads();
-    for (;;) {
-        const int b = s_tile;
-        if (b >= a.part.n_tiles) break;
-        if (tid == 0) nb = atomicAdd(&ctl->tile_next, 1);
-        const long long base = (long long)b * TILE;
-
-        float xv[K], lv[K];
+    int b = __builtin_amdgcn_readfirstlane(s_tile);
+    float xv[K], lv[K];
+    if (b < a.part.n_tiles) {
+        const float* xb = xs + (long long)b * TILE;
+        const float* lb = a.lw + (long long)b * TILE;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const int loc0 = (v * BLOCK + tid) * VEC;
-            load_vec<VEC>(xs + base + loc0, &xv[v * VEC]);
-            if (a.lw) load_vec<VEC>(a.lw + base + loc0, &lv[v * VEC]);
+            load_vec<VEC>(xb + loc0, &xv[v * VEC]);
+            if (a.lw) load_vec<VEC>(lb + loc0, &lv[v * VEC]);
         }
+    }
+    while (b < a.part.n_tiles) {
+        int nb = 0;
+        if (tid == 0) nb = claim_tile(a.tile_ctr, cr, ctried, a.part.n_tiles);
+        const long long base = (long long)b * TILE;
         if constexpr (FISH) {
 #pragma unroll
             for (int q = 0; q < 3; ++q)
@@ -786,8 +803,13 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
             a.b_log[b] = ts.b0;
             if (b == a.part.n_tiles - 1) a.b_log[b + 1] = ts.b1;
         }
-        if (tid == 0) s_nq = 0;
 
+        // log-weights: stored (ESS mode) or recomputed from x (a block-uniform
+        // branch, so only one of the two is evaluated)
+        if (!a.lw) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) lv[k] = log_weight(xv[k], obs);
+        }
         float e[K];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -795,72 +817,67 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
             for (int c = 0; c < VEC; ++c) {
                 const int k = v * VEC + c;
                 const int loc = (v * BLOCK + tid) * VEC + c;
-                float lk = a.lw ? lv[k] : log_weight(xv[k], obs);
                 bool valid = base + loc < n;
-                e[k] = (dead || !valid) ? 0.0f : expf_cw(lk - mb);
+                e[k] = (dead || !valid) ? 0.0f : expf_cw(lv[k] - mb);
                 s_x[pad16(loc)] = xv[k];
             }
         }
         const int last = tile_cdf<BLOCK, K>(e, b, n, a.part.n_tiles, ts.ft, ts.b0, ts.b1, u, W, s_c, s_w,
-                                            s_rng);
-        const int ilo = s_rng[0], ihi = s_rng[1], jlo = s_rng[2], jhi = s_rng[3];
-        if (tid == 0) s_tile = nb;     // every thread has read b (barriers in tile_cdf)
-
-        // Lane tid owns parents j in [tid*K, tid*K+K): their CDF values and states.
-        double cl[K];
-        float xl[K];
+                                            s_rng, &s_tile, nb);
+        const int ilo = __builtin_amdgcn_readfirstlane(s_rng[0]);
+        const int ihi = __builtin_amdgcn_readfirstlane(s_rng[1]);
+        const int jlo = __builtin_amdgcn_readfirstlane(s_rng[2]);
+        const int jhi = __builtin_amdgcn_readfirstlane(s_rng[3]);
+        const int bn = __builtin_amdgcn_readfirstlane(s_tile);
+        if (bn < a.part.n_tiles) {     // the next tile's loads overlap this tile's stores
+            const float* xb = xs + (long long)bn * TILE;
+            const float* lb = a.lw + (long long)bn * TILE;
 #pragma unroll
-        for (int q = 0; q < K; ++q) {
-            cl[q] = s_c[pad16(tid * K + q)];
-            xl[q] = s_x[pad16(tid * K + q)];
-        }
-        // End of the lane's children run among the unwrapped targets: the first i
-        // whose target exceeds its last CDF value; the start is the previous
-        // lane's end (ilo for lane 0).
-        const int lastj = tid * K + K - 1;
-        int ib = (lastj >= last) ? ihi : count_le_fast(cl[K - 1], u, ilo, ihi, n);
-        int ia = __shfl_up(ib, 1, kWave);
-        if (lane == kWave - 1) s_edge[wid] = ib;
-        __syncthreads();
-        ia = (lane == 0) ? ((wid == 0) ? ilo : s_edge[wid - 1]) : ia;
-
-        int slot = kLongRuns;
-        if (ib - ia > kLaneRunCap) slot = atomicAdd(&s_nq, 1);
-        if (slot < kLongRuns) {
-            s_q[slot] = tid;
-            s_qa[slot] = ia;
-            s_qb[slot] = ib;
-        } else {
-            for (int i = ia; i < ib; ++i) {
-                const double t = (double)target_unwrapped(u, i, n);
-                // first owned parent with c >= t: the count of smaller values
-                int q = 0;
-#pragma unroll
-                for (int r = 0; r < K - 1; ++r) q += (cl[r] < t) ? 1 : 0;
-                float v = xl[0];
-#pragma unroll
-                for (int r = 1; r < K; ++r) v = (q == r) ? xl[r] : v;
-                xd[i] = v;
-                if constexpr (FISH) {
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) ald[c][i] = s_al[c][pad16(tid * K + q)];
-                }
+            for (int v = 0; v < NV; ++v) {
+                const int loc0 = (v * BLOCK + tid) * VEC;
+                load_vec<VEC>(xb + loc0, &xv[v * VEC]);
+                if (a.lw) load_vec<VEC>(lb + loc0, &lv[v * VEC]);
             }
         }
-        __syncthreads();
-        // long runs: the whole block writes them, coalesced
-        const int nq = min(s_nq, kLongRuns);
-        for (int k = 0; k < nq; ++k) {
-            const int owner = s_q[k], qa = s_qa[k], qb = s_qb[k];
-            for (int i = qa + tid; i < qb; i += BLOCK) {
-                const double t = (double)target_unwrapped(u, i, n);
-                int q = 0;
+
+        // Offspring by merge path.  The owned unwrapped targets t_ilo..t_ihi-1
+        // and the in-tile CDF c_0..c_last are both non-decreasing, and output
+        // i goes to the first parent j with c_j >= t_i (lower_bound; every
+        // owned target is <= c_last).  Merging the two sequences -- emit output
+        // a when t_a <= c_j, else pass parent j -- takes n_out + P steps; lane
+        // tid takes the tid-th slice of D steps and finds where its slice
+        // starts by a binary search along the merge path, so every lane does
+        // the same work whatever the weight degeneracy.
+        const int n_out = ihi - ilo;
+        if (n_out > 0) {
+            const int P = last + 1;
+            const int Lm = n_out + P;
+            const int D = (Lm + BLOCK - 1) / BLOCK;
+            const int d0 = min(tid * D, Lm), d1 = min(d0 + D, Lm);
+            // outputs among the first d0 merge steps: output m comes before
+            // parent d0 - m - 1 is passed iff t_m <= c_(d0-m-1)
+            int lo = max(0, d0 - P), hi = min(d0, n_out);
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                const double tm = (double)target_unwrapped(u, ilo + mid, n);
+                if (tm <= s_c[pad16(d0 - mid - 1)]) lo = mid + 1; else hi = mid;
+            }
+            int oa = lo, j = d0 - lo;
+            double cj = (j < P) ? s_c[pad16(j)] : __builtin_inf();
+            double tv = (oa < n_out) ? (double)target_unwrapped(u, ilo + oa, n) : 0.0;
+            for (int d = d0; d < d1; ++d) {
+                if (oa < n_out && tv <= cj) {
+                    const int jp = pad16(j);
+                    xd[ilo + oa] = s_x[jp];
+                    if constexpr (FISH) {
 #pragma unroll
-                for (int r = 0; r < K - 1; ++r) q += (s_c[pad16(owner * K + r)] < t) ? 1 : 0;
-                xd[i] = s_x[pad16(owner * K + q)];
-                if constexpr (FISH) {
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) ald[c][i] = s_al[c][pad16(owner * K + q)];
+                        for (int c = 0; c < 3; ++c) ald[c][ilo + oa] = s_al[c][jp];
+                    }
+                    ++oa;
+                    tv = (oa < n_out) ? (double)target_unwrapped(u, ilo + oa, n) : 0.0;
+                } else {
+                    ++j;
+                    cj = (j < P) ? s_c[pad16(j)] : __builtin_inf();
                 }
             }
         }
@@ -875,6 +892,7 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
             }
         }
         __syncthreads();          // the next tile reuses the LDS arrays
+        b = bn;
     }
 }
 
@@ -1123,10 +1141,9 @@ static hipError_t launch_tick_k(const PropArgs& p, const FinArgs& f, const ResAr
         if (e != hipSuccess) return e;
         resident = per_cu > 0 && cus > 0 ? per_cu * cus : 1024;
     }
-    // three blocks per resident slot: measured best at 51.2M particles (the
-    // claims of a block overlap its neighbours' tiles; many more blocks make
-    // the counter the bottleneck)
-    const int grid = r.part.n_tiles < 3 * resident ? r.part.n_tiles : 3 * resident;
+    // two blocks per resident slot (one to three measure within 2 % of each
+    // other at 51.2M particles; a tick without resampling costs the same)
+    const int grid = r.part.n_tiles < 2 * resident ? r.part.n_tiles : 2 * resident;
     launch_timed(k_resample<RB, TILE / RB, FISH>, dim3(grid), dim3(RB), st, ev ? ev + 4 : nullptr, r);
     return hipGetLastError();
 }

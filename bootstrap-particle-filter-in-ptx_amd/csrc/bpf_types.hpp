@@ -15,6 +15,12 @@ constexpr int kGroup = 256;
 // Largest group count the finalise kernel's last block holds (256 threads x
 // 4 groups): 2^30 particles at 16 per lane.
 constexpr int kMaxGroups = 1024;
+// The resample pass hands out tiles from kClaimRanges contiguous ranges, each
+// with its own work counter (counters kClaimStride ints apart, in separate
+// cache lines): atomics on one address serialise, so one counter for all
+// tiles would bound the pass.
+constexpr int kClaimRanges = 8;
+constexpr int kClaimStride = 64;
 
 // Device-resident control block.  Written only by the finalise kernel's last
 // block each tick and read by the next kernels on the same stream, so
@@ -28,7 +34,6 @@ struct Ctl {
     int wrap_start;      // first i whose unwrapped target u + i/N is >= 1
     int resample_count;
     int tick;            // ticks finalised so far
-    int tile_next;       // resample work counter: next tile to hand out
     float m_prev;        // max of the log-weights carried into the next tick
     float u;             // resampling offset u / N of the latest resample
     float M;             // global max log-weight of the latest tick
@@ -95,6 +100,7 @@ struct FinArgs {
     float threshold;      // ESS/N threshold; <= 0 resamples every tick
     float* u_log;         // optional: resampling offset of this tick
     int* w_log;           // optional: wrap start of this tick
+    int* tile_ctr;        // resample work counters, cleared on resampling ticks
 };
 
 struct ResArgs {
@@ -103,7 +109,8 @@ struct ResArgs {
     const float* lw;      // stored log-weights, or null: recompute from x
     Partials part;
     const double* gptil;
-    Ctl* ctl;             // tile_next is the kernel's work counter
+    Ctl* ctl;
+    int* tile_ctr;        // work counters of the tile ranges (kClaimRanges, kClaimStride apart)
     const float* y_ptr;
     float y;
     ObsModel obs;
