@@ -168,10 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_init(BatchInitArgs a) {
 template <bool STUDENT>
 __device__ inline float dlw_dx(float x, const ObsModel& o) {
     const float eta2 = o.ysq * expf_cw(-x);
-    if (STUDENT) {
-        const float r = eta2 / o.nu;
-        return o.nph * (r / (1.0f + r)) - 0.5f;
-    }
+    if (STUDENT) return o.nph * eta2 / (o.nu + eta2) - 0.5f;   // as the reverse sweep
     return __builtin_fmaf(0.5f, eta2, -0.5f);
 }
 
@@ -293,6 +290,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
         double acc[NS];
 #pragma unroll
         for (int k = 0; k < NS; ++k) acc[k] = 0.0;
+        double ls[3] = {0.0, 0.0, 0.0};    // lane sums of the score terms
         if (mt != NEG_INF) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -312,8 +310,12 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
                 if (TAN) {
                     const double eg = (j < n) ? (FISHER ? de : de * (double)dlw_dx<STUDENT>(x[q], obs)) : 0.0;
 #pragma unroll
-                    for (int k = 0; k < 3; ++k) acc[4 + k] += wave_sum(eg * (double)d[k][q]);
+                    for (int k = 0; k < 3; ++k) ls[k] = fma(eg, (double)d[k][q], ls[k]);
                 }
+            }
+            if (TAN) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) acc[4 + k] = wave_sum(ls[k]);
             }
         } else {
             // the single filter's zero sums, added over its four waves
