@@ -666,7 +666,7 @@ static hipError_t launch_batch_tick_b(const BatchArgs& a, int n_blocks, bool stu
     return hipGetLastError();
 }
 
-// block: 64, 128, 256 or 512 threads per inner filter
+// block: 64, 128, 256, 512 or 1024 threads per inner filter
 hipError_t launch_batch_tick(const BatchArgs& a, int n_blocks, int block, hipStream_t st) {
     const bool student = a.obs.nu > 0.0f;
     switch (block) {
@@ -674,6 +674,7 @@ hipError_t launch_batch_tick(const BatchArgs& a, int n_blocks, int block, hipStr
         case 128: return launch_batch_tick_b<128>(a, n_blocks, student, st);
         case 256: return launch_batch_tick_b<256>(a, n_blocks, student, st);
         case 512: return launch_batch_tick_b<512>(a, n_blocks, student, st);
+        case 1024: return launch_batch_tick_b<1024>(a, n_blocks, student, st);
         default: return hipErrorInvalidValue;
     }
 }

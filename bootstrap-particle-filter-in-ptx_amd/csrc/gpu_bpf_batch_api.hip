@@ -26,7 +26,7 @@ static_assert(GPU_BPF_BATCH_MAX_N == kBatchMaxN, "header and kernels disagree on
 namespace {
 
 constexpr int kGraphTicks = 32;        // launches per captured graph
-constexpr int kBlocks[4] = {64, 128, 256, 512};
+constexpr int kBlocks[5] = {64, 128, 256, 512, 1024};
 
 double c_obs_student(double nu) {
     return lgamma((nu + 1.0) / 2.0) - lgamma(nu / 2.0) - 0.5 * log(nu * 3.14159265358979323846);
@@ -283,8 +283,9 @@ int gpu_bpf_batch_set_block(GpuBpfBatch* b, int threads) {
         set_err("gpu_bpf_batch_set_block: null batch");
         return -1;
     }
-    if (threads != 0 && threads != 64 && threads != 128 && threads != 256 && threads != 512) {
-        set_err("gpu_bpf_batch_set_block: %d is not 0, 64, 128, 256 or 512", threads);
+    if (threads != 0 && threads != 64 && threads != 128 && threads != 256 && threads != 512 &&
+        threads != 1024) {
+        set_err("gpu_bpf_batch_set_block: %d is not 0, 64, 128, 256, 512 or 1024", threads);
         return -1;
     }
     b->block = threads;

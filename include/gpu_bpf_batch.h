@@ -32,7 +32,7 @@ GpuBpfBatch* gpu_bpf_batch_create(int n_filters, const int* n_particles, const f
                                   float nu_obs, float ess_threshold);
 void gpu_bpf_batch_destroy(GpuBpfBatch* batch);
 
-/* Threads per inner filter (64, 128, 256 or 512); 0 = chosen by the
+/* Threads per inner filter (64, 128, 256, 512 or 1024); 0 = chosen by the
  * autotuner at the first run.  Returns 0 or -1. */
 int gpu_bpf_batch_set_block(GpuBpfBatch* batch, int threads);
 int gpu_bpf_batch_get_block(GpuBpfBatch* batch);

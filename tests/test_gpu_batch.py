@@ -79,7 +79,7 @@ def test_batch_equals_looped_plain_filters(ydata, thr):
     b.close()
 
 
-@pytest.mark.parametrize("block", [64, 128, 512])
+@pytest.mark.parametrize("block", [64, 128, 512, 1024])
 def test_block_sizes_give_identical_results(ydata, block):
     rng = np.random.default_rng(5)
     n, mu, phi, sig, seeds = filters(9, rng)
@@ -131,7 +131,7 @@ def test_autotune_picks_a_block(ydata):
     say("autotune")
     b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds)
     blk = b.autotune(ydata, 40)
-    assert blk in (64, 128, 256, 512) and b.block == blk
+    assert blk in (64, 128, 256, 512, 1024) and b.block == blk
     b.run(ydata[:10], stats=False)
     assert b.timestep() == 10
     b.close()
