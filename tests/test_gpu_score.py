@@ -139,14 +139,15 @@ def test_autograd_function_matches_torch_restatement(orc, ydata):
     np.testing.assert_allclose(th.grad.numpy(), tg, rtol=1e-4, atol=1e-4 * np.max(np.abs(tg)))
 
 
-def fisher_score(n, y, nu=0.0, theta=(MU, RHO, SIG), seed=42):
+def fisher_score(n, y, nu=0.0, theta=(MU, RHO, SIG), seed=42, items=None, want_x=False):
     mu, rho, sig = theta
-    f = gpu_bpf.GpuBpf(n, rho, sig, mu, nu_obs=nu, seed=seed)
+    f = gpu_bpf.GpuBpf(n, rho, sig, mu, nu_obs=nu, seed=seed, items_per_thread=items)
     f.set_score_estimator("fisher")
     ll = f.score_forward(y)
     g = f.score_backward()
+    x = f.particles() if want_x else None
     f.close()
-    return ll, g
+    return (ll, g, x) if want_x else (ll, g)
 
 
 @pytest.mark.parametrize("n,nu", [(3000, 0.0), (16384, 0.0), (5000, 5.0)])
@@ -167,15 +168,17 @@ def test_fisher_single_filter_equals_batch(ydata, n, nu):
     np.testing.assert_allclose(g, gb, rtol=1e-9, atol=1e-12 * np.max(np.abs(gb)))
 
 
-def test_fisher_single_filter_matches_exact_score(orc, ydata):
+@pytest.mark.parametrize("items", [None, 16])
+def test_fisher_single_filter_matches_exact_score(orc, ydata, items):
     """Fisher-identity estimates of four 2^20-particle filters (independent
     seeds) against the exact score of the SV log-likelihood (central
-    differences of the point-mass filter)."""
+    differences of the point-mass filter), with the default tiles (512
+    particles) and with 4096-particle tiles."""
     y = ydata[:60]
     th = (MU, RHO, SIG)
     exact = score_reference.exact_score(orc, y, th)
-    say("fisher single vs exact", exact.tolist())
-    gs = np.array([fisher_score(1 << 20, y, seed=s)[1] for s in (1, 2, 3, 4)])
+    say("fisher single vs exact", items, exact.tolist())
+    gs = np.array([fisher_score(1 << 20, y, seed=s, items=items)[1] for s in (1, 2, 3, 4)])
     mean, se = gs.mean(0), gs.std(0, ddof=1) / 2.0
     say("fisher single", mean.tolist(), se.tolist())
     assert np.all(np.abs(mean - exact) < 4 * se + 0.02 * np.abs(exact)), (mean, se, exact)
@@ -188,3 +191,26 @@ def test_fisher_estimator_argument_checked():
         f.set_score_estimator("bogus")
     assert f.L.gpu_bpf_set_score_estimator(f.h, 7) == -1
     f.close()
+
+
+@pytest.mark.parametrize("items", [1, 2, 4, 8, 16])
+def test_fisher_every_tile_width(ydata, items):
+    """The Fisher-identity kernels of every tile width (up to 4096 particles
+    per tile, 16 per lane): the particles are the plain filter's bit for bit,
+    and at tick 0 (initial vectors, no gather yet) the score is the same as
+    with 256-particle tiles up to FP32 weight rounding.  Later ticks are not
+    compared across widths: the tile maxima differ, so the FP32 weights, the
+    CDF and hence some ancestors differ."""
+    y = ydata[:60]
+    n = 70_001
+    say("fisher tile width", items)
+    ll, g, x = fisher_score(n, y, items=items, want_x=True)
+    f = gpu_bpf.GpuBpf(n, RHO, SIG, MU, seed=42, ess_threshold=0.0, items_per_thread=items)
+    _, st = f.run(y)
+    xp = f.particles()
+    f.close()
+    np.testing.assert_array_equal(x, xp)
+    assert ll == st["ll_cum"][-1]
+    _, g0 = fisher_score(n, y[:1], items=items)
+    _, g1 = fisher_score(n, y[:1], items=1)
+    np.testing.assert_allclose(g0, g1, rtol=1e-6, atol=1e-9 * np.max(np.abs(g1)))
