@@ -764,7 +764,7 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
     // (block-uniform values read from LDS go through readfirstlane, so that
     // tile bases are scalar and per-lane addresses 32-bit offsets)
     int cr = blockIdx.x % kClaimRanges, ctried = 0;   // blocks alternate between XCDs
-    if (tid == 0) s_tile = claim_tile(a.tile_ctr, cr, ctried, a.part.n_tiles);
+    if (tid == 0) s_tile = a.one_tile ? (int)blockIdx.x : claim_tile(a.tile_ctr, cr, ctried, a.part.n_tiles);
     __syncthreads();
     int b = __builtin_amdgcn_readfirstlane(s_tile);
     float xv[K], lv[K];
@@ -780,7 +780,7 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
     }
     while (b < a.part.n_tiles) {
         int nb = 0;
-        if (tid == 0) nb = claim_tile(a.tile_ctr, cr, ctried, a.part.n_tiles);
+        if (tid == 0) nb = a.one_tile ? a.part.n_tiles : claim_tile(a.tile_ctr, cr, ctried, a.part.n_tiles);
         const long long base = (long long)b * TILE;
         if constexpr (FISH) {
 #pragma unroll
@@ -1121,16 +1121,23 @@ static void launch_timed(Kern k, dim3 grid, dim3 block, hipStream_t st, const hi
     else hipLaunchKernelGGL(k, grid, block, 0, st, a);
 }
 
+// Threads of a resample (and adjoint) block for a tile, measured on MI355X
+// (always-resampling ticks): 512 lanes for 4096-particle tiles (half the
+// per-lane chain of 256 lanes), 256 for 2048, 128 for 512-1024 (small blocks,
+// more tiles in flight per CU), and one lane per particle for 256-particle
+// tiles, which keeps their CDF arithmetic that of the batched filters.
+constexpr int resample_block(int tile) {
+    return tile >= 4096 ? 512 : (tile == 2048 ? 256 : (tile >= 512 ? 128 : tile));
+}
+
 // ev (nullable): start/stop event pairs of propagate, finalise, resample.
 template <int K, bool FISH>
 static hipError_t launch_tick_k(const PropArgs& p, const FinArgs& f, const ResArgs& r,
                                 hipStream_t st, hipEvent_t* ev) {
     launch_timed(k_propagate<K, FISH>, dim3(p.part.n_tiles), dim3(kBlock), st, ev, p);
     launch_timed(k_finalize, dim3(f.part.n_groups), dim3(kFinBlock), st, ev ? ev + 2 : nullptr, f);
-    // resample runs the same tile with 512 lanes (TILE >= 512): twice the
-    // waves per CU and half the per-lane chain of the 256-lane layout
     constexpr int TILE = kBlock * K;
-    constexpr int RB = TILE >= 512 ? 512 : TILE;
+    constexpr int RB = resample_block(TILE);
     static int resident = 0;     // resample blocks the device holds at once
     if (resident == 0) {
         int per_cu = 0, cus = 0, dev = 0;
@@ -1141,10 +1148,14 @@ static hipError_t launch_tick_k(const PropArgs& p, const FinArgs& f, const ResAr
         if (e != hipSuccess) return e;
         resident = per_cu > 0 && cus > 0 ? per_cu * cus : 1024;
     }
-    // two blocks per resident slot (one to three measure within 2 % of each
-    // other at 51.2M particles; a tick without resampling costs the same)
-    const int grid = r.part.n_tiles < 2 * resident ? r.part.n_tiles : 2 * resident;
-    launch_timed(k_resample<RB, TILE / RB, FISH>, dim3(grid), dim3(RB), st, ev ? ev + 4 : nullptr, r);
+    // Up to four blocks per resident slot: one block per tile, no work
+    // counters.  Beyond: a persistent grid of two blocks per resident slot
+    // (one to three measure within 2 % of each other at 51.2M particles; a
+    // tick without resampling costs the same).
+    ResArgs ra = r;
+    ra.one_tile = r.part.n_tiles <= 4 * resident;
+    const int grid = ra.one_tile ? r.part.n_tiles : 2 * resident;
+    launch_timed(k_resample<RB, TILE / RB, FISH>, dim3(grid), dim3(RB), st, ev ? ev + 4 : nullptr, ra);
     return hipGetLastError();
 }
 
@@ -1169,7 +1180,7 @@ hipError_t launch_tick(int K, const PropArgs& p, const FinArgs& f, const ResArgs
 template <int K>
 static hipError_t launch_adj_k(const AdjArgs& a, hipStream_t st) {
     constexpr int TILE = kBlock * K;
-    constexpr int RB = TILE >= 512 ? 512 : TILE;
+    constexpr int RB = resample_block(TILE);
     hipLaunchKernelGGL((k_adjoint<RB, TILE / RB>), dim3(a.n_tiles), dim3(RB), 0, st, a);
     return hipGetLastError();
 }

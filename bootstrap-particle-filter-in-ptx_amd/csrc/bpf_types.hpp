@@ -111,6 +111,7 @@ struct ResArgs {
     const double* gptil;
     Ctl* ctl;
     int* tile_ctr;        // work counters of the tile ranges (kClaimRanges, kClaimStride apart)
+    int one_tile;         // 1: block b does tile b only (grid = n_tiles), no counters
     const float* y_ptr;
     float y;
     ObsModel obs;
