@@ -215,6 +215,16 @@ __device__ inline float log_weight_t(float x, const ObsModel& o) {
     return __builtin_fmaf(-0.5f, x, lw);
 }
 
+// d lw / dx of the observation log-density at the propagated state, in the
+// reference's form (ptx_assembly/bpf_kernels_full.ptx:1172-1189): with
+// eta^2 = y^2 e^{-x}, Gaussian -1/2 + eta^2/2, Student-t
+// -1/2 + (nu+1)/2 * eta^2 / (nu + eta^2) (correctly rounded division).
+__device__ inline float learn_dlw(float x, const ObsModel& o) {
+    const float eta2 = o.ysq * expf_cw(-x);
+    if (o.nu > 0.0f) return __builtin_fmaf(o.nph, eta2 / (o.nu + eta2), -0.5f);
+    return __builtin_fmaf(eta2, 0.5f, -0.5f);
+}
+
 __device__ inline float log_weight(float x, const ObsModel& o) {
     return o.nu > 0.0f ? log_weight_t(x, o) : log_weight_gauss(x, o.ysq);
 }

@@ -258,12 +258,46 @@ __device__ inline void lane_fisher_sums(const float (&l)[K], const float (&al)[3
     }
 }
 
+// FP64 sums of e * v_q, v = (g, g^2, g d, g^2 d^2), of the lane's particles
+// for online learning: g = d lw/dx at the propagated state, d = x_prev - mu
+// (ptx_assembly/bpf_kernels_full.ptx:1191-1208 with w = e; padding
+// contributes zero).
+template <int K>
+__device__ inline void lane_learn_sums(const float (&x)[K], const float (&l)[K], const float (&dp)[K],
+                                       const ObsModel& obs, float mb, int tid, int rem,
+                                       double (&L)[4]) {
+    constexpr int VEC = K < 4 ? K : 4;
+    constexpr int NV = K / VEC;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) {
+            const int k = v * VEC + c;
+            const int li = (v * kBlock + tid) * VEC + c;
+            if (li < rem) {
+                const double de = (double)expf_cw_nonpos(l[k] - mb);
+                const double g = (double)learn_dlw(x[k], obs);
+                const double d = (double)dp[k];
+                const double t0 = de * g;
+                const double t1 = t0 * g;
+                L[0] += t0;
+                L[1] += t1;
+                L[2] += t0 * d;
+                L[3] += (t1 * d) * d;
+            }
+        }
+    }
+}
+
 // FISH: also carry the Fisher-identity vector alpha = grad log p(x_0:t) of
 // each particle through the transition (alpha += grad log N(x'; mu + phi
 // (x - mu), sigma^2), with z = (x' - mu - phi (x - mu)) / sigma the drawn
 // normal) and write the per-tile sums of e * alpha.
-template <int K, bool FISH>
+// LEARN: also write the per-tile sums of online learning (lane_learn_sums);
+// d = x_prev - mu is 0 on tick 0, which propagates nothing.
+template <int K, bool FISH, bool LEARN>
 __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
+    static_assert(!(FISH && LEARN), "one extra statistic per launch");
     constexpr int VEC = K < 4 ? K : 4;
     constexpr int NV = K / VEC;
     constexpr int TILE = kBlock * K;
@@ -272,7 +306,7 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
     __shared__ float s_q[TILE];
     __shared__ int s_cnt[NW];
     __shared__ float s_m[NW];
-    __shared__ double s_d[FISH ? 7 : 4][NW];
+    __shared__ double s_d[FISH ? 7 : (LEARN ? 8 : 4)][NW];
 
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
@@ -326,6 +360,12 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
                 long long off = base + (long long)(v * kBlock + tid) * VEC;
                 load_vec<VEC>(a.alpha[cur][q] + off, &al[q][v * VEC]);
             }
+    }
+
+    float dp[LEARN ? K : 1];
+    if constexpr (LEARN) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) dp[k] = a.do_prop ? x[k] - a.mu : 0.0f;
     }
 
     if (a.do_prop) {
@@ -407,6 +447,15 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
             if (lane == 0) s_d[4 + q][wid] = F[q];
         }
     }
+    if constexpr (LEARN) {
+        double L[4] = {0.0, 0.0, 0.0, 0.0};
+        if (mb != -__builtin_inff()) lane_learn_sums<K>(x, l, dp, obs, mb, tid, rem, L);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            L[q] = wave_sum(L[q]);
+            if (lane == 0) s_d[4 + q][wid] = L[q];
+        }
+    }
     __syncthreads();
     if (tid == 0) {
         double r[4];
@@ -429,6 +478,15 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
 #pragma unroll
                 for (int w = 1; w < NW; ++w) acc += s_d[4 + q][w];
                 a.fsum[(size_t)q * a.part.n_tiles + b] = acc;
+            }
+        }
+        if constexpr (LEARN) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                double acc = s_d[4 + q][0];
+#pragma unroll
+                for (int w = 1; w < NW; ++w) acc += s_d[4 + q][w];
+                a.lsum[(size_t)q * a.part.n_tiles + b] = acc;
             }
         }
     }
@@ -1074,8 +1132,52 @@ __global__ __launch_bounds__(kBlock) void k_fisher_score(FisherArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Online learning: acc[q] += sum_i w_i v_qi of the latest tick, the tile sums
+// of e * v_q scaled by tile b's normalised weight scale tf_b exp(gm_g - M) / S
+// (the reference's kernel-14 atomics into d_scalars[5..8],
+// ptx_assembly/bpf_kernels_full.ptx:1373-1377).  One block, fixed order.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_learn_accum(LearnArgs a) {
+    constexpr int NW = kBlock / kWave;
+    constexpr float NEG_INF = -__builtin_inff();
+    __shared__ double s_r[4][NW];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const float M = a.ctl->M;
+    const double S = a.ctl->S;
+    const int G = a.part.n_tiles;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = tid; b < G; b += kBlock) {
+        const float gm = a.part.gm[b / kGroup];
+        const double fg = (gm == NEG_INF) ? 0.0 : (double)expf_cw(gm - M);
+        const double f = a.part.tf[b] * fg;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[c] = fma(f, a.lsum[(size_t)c * G + b], acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        acc[c] = wave_sum(acc[c]);
+        if (lane == 0) s_r[c][wid] = acc[c];
+    }
+    __syncthreads();
+    if (tid < 4) {
+        double r = s_r[tid][0];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) r += s_r[tid][w];
+        // a tick without finite positive weight adds nothing (its statistics
+        // are NaN and it does not resample)
+        if (S > 0.0 && isfinite(S)) a.acc[tid] += r / S;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
+
+hipError_t launch_learn_accum(const LearnArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_learn_accum, dim3(1), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
 
 hipError_t launch_fisher_score(const FisherArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_fisher_score, dim3(1), dim3(kBlock), 0, st, a);
@@ -1131,10 +1233,10 @@ constexpr int resample_block(int tile) {
 }
 
 // ev (nullable): start/stop event pairs of propagate, finalise, resample.
-template <int K, bool FISH>
+template <int K, bool FISH, bool LEARN>
 static hipError_t launch_tick_k(const PropArgs& p, const FinArgs& f, const ResArgs& r,
                                 hipStream_t st, hipEvent_t* ev) {
-    launch_timed(k_propagate<K, FISH>, dim3(p.part.n_tiles), dim3(kBlock), st, ev, p);
+    launch_timed(k_propagate<K, FISH, LEARN>, dim3(p.part.n_tiles), dim3(kBlock), st, ev, p);
     launch_timed(k_finalize, dim3(f.part.n_groups), dim3(kFinBlock), st, ev ? ev + 2 : nullptr, f);
     constexpr int TILE = kBlock * K;
     constexpr int RB = resample_block(TILE);
@@ -1159,18 +1261,31 @@ static hipError_t launch_tick_k(const PropArgs& p, const FinArgs& f, const ResAr
     return hipGetLastError();
 }
 
+template <int K>
+static hipError_t launch_tick_v(int var, const PropArgs& p, const FinArgs& f, const ResArgs& r,
+                                hipStream_t st, hipEvent_t* ev) {
+    switch (var) {
+        case 0: return launch_tick_k<K, false, false>(p, f, r, st, ev);
+        case 1: return launch_tick_k<K, true, false>(p, f, r, st, ev);
+        default: return launch_tick_k<K, false, true>(p, f, r, st, ev);
+    }
+}
+
 // The Fisher-identity variants are launched when PropArgs.alpha is set (the
-// resample pass must then carry the same buffers).
+// resample pass must then carry the same buffers), the learning variants
+// when PropArgs.lsum is set; not both.
 hipError_t launch_tick(int K, const PropArgs& p, const FinArgs& f, const ResArgs& r,
                        hipStream_t st, hipEvent_t* ev) {
     const bool fish = p.alpha[0][0] != nullptr;
-    if (fish != (r.alpha[0][0] != nullptr)) return hipErrorInvalidValue;
+    const bool learn = p.lsum != nullptr;
+    if (fish != (r.alpha[0][0] != nullptr) || (fish && learn)) return hipErrorInvalidValue;
+    const int var = fish ? 1 : (learn ? 2 : 0);
     switch (K) {
-        case 1: return fish ? launch_tick_k<1, true>(p, f, r, st, ev) : launch_tick_k<1, false>(p, f, r, st, ev);
-        case 2: return fish ? launch_tick_k<2, true>(p, f, r, st, ev) : launch_tick_k<2, false>(p, f, r, st, ev);
-        case 4: return fish ? launch_tick_k<4, true>(p, f, r, st, ev) : launch_tick_k<4, false>(p, f, r, st, ev);
-        case 8: return fish ? launch_tick_k<8, true>(p, f, r, st, ev) : launch_tick_k<8, false>(p, f, r, st, ev);
-        case 16: return fish ? launch_tick_k<16, true>(p, f, r, st, ev) : launch_tick_k<16, false>(p, f, r, st, ev);
+        case 1: return launch_tick_v<1>(var, p, f, r, st, ev);
+        case 2: return launch_tick_v<2>(var, p, f, r, st, ev);
+        case 4: return launch_tick_v<4>(var, p, f, r, st, ev);
+        case 8: return launch_tick_v<8>(var, p, f, r, st, ev);
+        case 16: return launch_tick_v<16>(var, p, f, r, st, ev);
         default: return hipErrorInvalidValue;
     }
 }

@@ -88,6 +88,11 @@ struct PropArgs {
     // (component c at fsum + c * n_tiles)
     float* alpha[2][3];
     double* fsum;
+    // online mu/rho learning (null: off): per tile FP64 sums of e * v_q with
+    // the reference's kernel-14 terms v = (g, g^2, g d, g^2 d^2), g = d lw/dx
+    // of the propagated particle and d = x_prev - mu (component q at
+    // lsum + q * n_tiles)
+    double* lsum;
 };
 
 struct FinArgs {
@@ -151,6 +156,15 @@ struct InitArgs {
     int n, n_pad;
     float* alpha[3];      // optional grad log N(x_0; mu, sd^2), zero on padding
     float dsd_dphi, dsd_dsig;
+};
+
+// Online learning: adds the latest tick's normalised sums sum_i w_i v_qi
+// (tile sums of e * v_q scaled as the Fisher score's) to acc[0..3].
+struct LearnArgs {
+    Partials part;
+    const double* lsum;
+    const Ctl* ctl;
+    double* acc;
 };
 
 // Fisher-identity score of the latest tick: sum_i w_i alpha_i from the tile

@@ -78,6 +78,7 @@ def load(path=LIB_PATH):
         "gpu_bpf_get_rho": (F, [P]),
         "gpu_bpf_set_mu": (None, [P, F]),
         "gpu_bpf_set_rho": (None, [P, F]),
+        "gpu_bpf_get_learn_sums": (I, [P, P, P]),
         "gpu_bpf_get_sigma_z": (F, [P]),
         "gpu_bpf_set_sigma_z": (None, [P, F]),
         "gpu_bpf_enable_mu_learning": (None, [P, I, I, F, F, F]),
@@ -181,6 +182,22 @@ class GpuBpf:
 
     def get_rho(self):
         return self.L.gpu_bpf_get_rho(self.h)
+
+    def enable_mu_learning(self, mode=1, K=50, c=0.1, t0=10.0, gamma=0.667):
+        self.L.gpu_bpf_enable_mu_learning(self.h, int(mode), int(K), float(c), float(t0), float(gamma))
+
+    def disable_mu_learning(self):
+        self.L.gpu_bpf_disable_mu_learning(self.h)
+
+    def enable_rho_learning(self, enable=True):
+        self.L.gpu_bpf_enable_rho_learning(self.h, int(enable))
+
+    def learn_sums(self):
+        """(sums[4], ticks): the learner's accumulators since its last update."""
+        out = np.zeros(4, np.float64)
+        n = ctypes.c_int(0)
+        self._check(self.L.gpu_bpf_get_learn_sums(self.h, _ptr(out), ctypes.byref(n)))
+        return out, n.value
 
     # build-owned
     def run(self, y, stats=True):

@@ -88,16 +88,28 @@ void gpu_bpf_set_ess_threshold(GpuBpfState* state, float threshold);
 /* gpu_bpf_full.cuh:285 / gpu_bpf_ptx_full.cu:862. */
 int gpu_bpf_get_resample_count(GpuBpfState* state);
 
-/* gpu_bpf_full.cuh:207-230 / gpu_bpf_ptx_full.cu:806-831.  set_rho clamps to
- * [0.001, 0.999] as the reference does.  Changes apply from the next tick. */
+/* gpu_bpf_full.cuh:207-230 / gpu_bpf_ptx_full.cu:802-831.  set_rho clamps to
+ * [0.001, 0.999] as the reference does.  Changes apply from the next tick.
+ * With online learning on, set_mu / set_rho also restart the learner's step
+ * counter and accumulators, and get_mu / get_rho return the learned values. */
 float gpu_bpf_get_mu(GpuBpfState* state);
 float gpu_bpf_get_rho(GpuBpfState* state);
 void  gpu_bpf_set_mu(GpuBpfState* state, float mu);
 void  gpu_bpf_set_rho(GpuBpfState* state, float rho);
 
-/* gpu_bpf_full.cuh:197-204 (online mu/rho learning).  Not provided by this
- * library: enable_* record an error and leave learning off; disable is a
- * no-op.  Kept so that callers link. */
+/* gpu_bpf_full.cuh:176-204 / gpu_bpf_ptx_full.cu:469-537,777-800 (online
+ * mu/rho learning by natural gradient + Robbins-Monro).  Each tick adds the
+ * weighted one-step terms of the reference's kernel 14
+ * (ptx_assembly/bpf_kernels_full.ptx:1116-1384) to four FP64 accumulators:
+ * sum w g, sum w g^2, sum w g d, sum w g^2 d^2 with g = d log p(y_t | x_t)/dx_t
+ * and d = x_{t-1} - mu (0 on tick 0, where the reference reads an unset
+ * buffer).  They are computed inside the propagate pass (no extra particle
+ * traffic).  Every K ticks the tick synchronises once and applies the
+ * reference's update to mu (and rho when enabled); the new values apply from
+ * the next tick.  mode 0 is off and any other value is on: the reference's
+ * code applies the same natural-gradient step for modes 1 and 2.  K <= 0
+ * updates every tick.  gpu_bpf_score_forward refuses to run while learning
+ * is on. */
 void gpu_bpf_enable_mu_learning(GpuBpfState* state, int mode, int K,
                                 float c, float t0, float gamma);
 void gpu_bpf_disable_mu_learning(GpuBpfState* state);
@@ -118,6 +130,12 @@ void gpu_bpf_set_adaptive_bands(int n_particles,
 
 /* Thread-local text of the last error ("" if none). */
 const char* gpu_bpf_last_error(void);
+
+/* The learner's four accumulated sums since its last update (sum over ticks
+ * of sum_i w_i g_i, sum w g^2, sum w g d, sum w g^2 d^2; the reference's
+ * d_scalars[5..8], gpu_bpf_full.cuh:88-89) and the ticks they cover.
+ * Synchronises the filter's stream.  Returns 0, or -1 with gpu_bpf_last_error(). */
+int gpu_bpf_get_learn_sums(GpuBpfState* state, double* sums, int* n_ticks);
 
 /* Replaces reading GpuBpfState.sigma_z (gpu_bpf_full.cuh:107). */
 float gpu_bpf_get_sigma_z(GpuBpfState* state);

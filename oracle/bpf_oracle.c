@@ -412,11 +412,75 @@ static float tiled_weights(const float* l, int n, int tile, float* tmax, double*
     return gm;
 }
 
+/* d lw / dx of the observation density at the propagated state, in the form
+ * of ptx_assembly/bpf_kernels_full.ptx:1172-1189 (eta^2 = y^2 e^{-x}):
+ * Gaussian -1/2 + eta^2/2, Student-t -1/2 + (nu+1)/2 eta^2 / (nu + eta^2). */
+static inline float learn_dlw(float x, float ysq, float nu_obs, float nph) {
+    float eta2 = ysq * orc_expf(-x);
+    if (nu_obs > 0.0f) return fmaf(nph, eta2 / (nu_obs + eta2), -0.5f);
+    return fmaf(eta2, g_fault == 6 ? 1.0f : 0.5f, -0.5f);
+}
+
+/* gpu_bpf/gpu_bpf_ptx_full.cu:485-537 */
+void orc_learn_update(const OrcLearn* lp, const double* sums, int k, int* rm_step,
+                      float* mu, float* rho) {
+    float sm[4];
+    for (int q = 0; q < 4; q++) sm[q] = (float)sums[q];
+    (*rm_step)++;
+    float eta = lp->c / powf((float)(*rm_step) + lp->t0, lp->gamma);
+    float K_f = (float)k;
+    float K_inv = 1.0f / K_f;
+    {
+        float mean_g = sm[0] * K_inv;
+        float mean_f = sm[1] * K_inv;
+        float dir = mean_g / fmaxf(mean_f, 1e-8f);
+        if (lp->grad_clip > 0.0f && fabsf(dir) > lp->grad_clip) dir *= lp->grad_clip / fabsf(dir);
+        float snr = fabsf(mean_g) / fmaxf(sqrtf(mean_f / K_f), 1e-10f);
+        float gate = fminf(snr / 2.0f, 1.0f);
+        float alpha = *mu * (1.0f - *rho);
+        alpha += eta * gate * dir;
+        float mu_new = alpha / fmaxf(1.0f - *rho, 1e-6f);
+        *mu = fmaxf(fminf(mu_new, 2.0f), -5.0f);
+    }
+    if (lp->learn_rho) {
+        float mean_g = sm[2] * K_inv;
+        float mean_f = sm[3] * K_inv;
+        float dir = mean_g / fmaxf(mean_f, 1e-8f);
+        if (lp->grad_clip > 0.0f && fabsf(dir) > lp->grad_clip) dir *= lp->grad_clip / fabsf(dir);
+        float snr = fabsf(mean_g) / fmaxf(sqrtf(mean_f / K_f), 1e-10f);
+        float gate = fminf(snr / 2.0f, 1.0f);
+        float rho_new = *rho + eta * gate * dir;
+        *rho = fmaxf(fminf(rho_new, 0.999f), 0.001f);
+    }
+}
+
+static int bpf_run_impl(const OrcParams* p, int mode, const float* y, const float* mu_t,
+                        int n_ticks, OrcTick* out, float* x_out, double* x64_out,
+                        int* anc_out, int n_threads, const OrcLearn* lp, double* lsum_out,
+                        float* par_out);
+
 int orc_bpf_run(const OrcParams* p, int mode, const float* y, const float* mu_t,
                 int n_ticks, OrcTick* out, float* x_out, double* x64_out,
                 int* anc_out, int n_threads) {
+    return bpf_run_impl(p, mode, y, mu_t, n_ticks, out, x_out, x64_out, anc_out, n_threads,
+                        NULL, NULL, NULL);
+}
+
+int orc_bpf_run_learn(const OrcParams* p, const OrcLearn* lp, const float* y, int n_ticks,
+                      OrcTick* out, double* lsum_out, float* par_out, float* x_out,
+                      int n_threads) {
+    if (!lp) return -1;
+    return bpf_run_impl(p, ORC_MODE_F32, y, NULL, n_ticks, out, x_out, NULL, NULL, n_threads,
+                        lp, lsum_out, par_out);
+}
+
+static int bpf_run_impl(const OrcParams* p, int mode, const float* y, const float* mu_t,
+                        int n_ticks, OrcTick* out, float* x_out, double* x64_out,
+                        int* anc_out, int n_threads, const OrcLearn* lp, double* lsum_out,
+                        float* par_out) {
     const int n = p->n_particles;
     if (n <= 0 || n_ticks < 0 || p->tile <= 0) return -1;
+    if (lp && (mode != ORC_MODE_F32 || mu_t)) return -1;
     const int tile = p->tile;
     const int n_tiles = (n + tile - 1) / tile;
 #ifdef _OPENMP
@@ -440,13 +504,19 @@ int orc_bpf_run(const OrcParams* p, int mode, const float* y, const float* mu_t,
     double* xdt = (double*)malloc(sizeof(double) * (size_t)n);
     float* tmax = (float*)malloc(sizeof(float) * (size_t)n_tiles);
     double* tfd = (double*)malloc(sizeof(double) * (size_t)n_tiles);
-    if (!x || !l || !xd || !ld || !w || !wd || !cdf || !anc || !xt || !xdt || !tmax || !tfd) {
+    float* xp = lp ? (float*)malloc(sizeof(float) * (size_t)n) : NULL;
+    if (!x || !l || !xd || !ld || !w || !wd || !cdf || !anc || !xt || !xdt || !tmax || !tfd ||
+        (lp && !xp)) {
         free(x); free(l); free(xd); free(ld); free(w); free(wd); free(cdf); free(anc);
-        free(xt); free(xdt); free(tmax); free(tfd);
+        free(xt); free(xdt); free(tmax); free(tfd); free(xp);
         return -1;
     }
 
-    const float rho = p->rho, sig = p->sigma_z;
+    float rho = p->rho;
+    const float sig = p->sigma_z;
+    float mu_cur = p->mu;              /* moved by the learner */
+    double lacc[4] = {0.0, 0.0, 0.0, 0.0};
+    int grad_count = 0, rm_step = 0;
     const float nu_obs = p->nu_obs;
     const double c_obs_d = nu_obs > 0.0f ? c_obs_f64((double)nu_obs) : 0.0;
     const float c_obs = (float)c_obs_d;
@@ -473,7 +543,7 @@ int orc_bpf_run(const OrcParams* p, int mode, const float* y, const float* mu_t,
     const double ll_floor = (double)f32_bits(0x0D6BF94Eu);
 
     for (int t = 0; t < n_ticks; t++) {
-        const float mu = mu_t ? mu_t[t] : p->mu;
+        const float mu = mu_t ? mu_t[t] : mu_cur;
         const float yt = y[t];
         const float ysq = yt * yt;
         const PcgJump jt = pcg_jump(p->seed, (uint32_t)t);
@@ -483,6 +553,7 @@ int orc_bpf_run(const OrcParams* p, int mode, const float* y, const float* mu_t,
 #pragma omp parallel for num_threads(n_threads) schedule(static)
             for (int i = 0; i < n; i++) {
                 float xi = x[i];
+                if (xp) xp[i] = xi;
                 if (t > 0) {
                     float z = normal_j(jt, (uint32_t)i);
                     float d = xi - mu;
@@ -573,6 +644,37 @@ int orc_bpf_run(const OrcParams* p, int mode, const float* y, const float* mu_t,
         r.resampled = resample;
         if (out) out[t] = r;
 
+        if (lp) {
+            /* the learner's one-step sums of this tick on the FP32 particles
+             * (w = the tiled weights, normalised by their sum) */
+            double L[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int i = 0; i < n; i++) {
+                const double g = (double)learn_dlw(x[i], ysq, nu_obs, nph);
+                const float dpf = (t > 0) ? ((g_fault == 7 ? x[i] : xp[i]) - mu) : 0.0f;
+                const double d = (double)dpf;
+                const double t0 = w[i] * g;
+                const double t1 = t0 * g;
+                L[0] += t0;
+                L[1] += t1;
+                L[2] += t0 * d;
+                L[3] += (t1 * d) * d;
+            }
+            for (int q = 0; q < 4; q++) {
+                const double v = (isfinite(S) && S > 0.0) ? L[q] / S : 0.0;
+                if (lsum_out) lsum_out[4 * (size_t)t + q] = v;
+                lacc[q] += v;
+            }
+            if (++grad_count >= lp->update_K) {
+                orc_learn_update(lp, lacc, grad_count, &rm_step, &mu_cur, &rho);
+                lacc[0] = lacc[1] = lacc[2] = lacc[3] = 0.0;
+                grad_count = 0;
+            }
+            if (par_out) {
+                par_out[2 * (size_t)t] = mu_cur;
+                par_out[2 * (size_t)t + 1] = rho;
+            }
+        }
+
         if (resample) {
             uint32_t hr = orc_pcg_host_next(&hrng);
             float u = ((float)(hr >> 9) * (1.0f / 8388608.0f)) / (float)n;
@@ -614,7 +716,7 @@ int orc_bpf_run(const OrcParams* p, int mode, const float* y, const float* mu_t,
     if (x_out) memcpy(x_out, x, sizeof(float) * (size_t)n);
     if (x64_out) memcpy(x64_out, xd, sizeof(double) * (size_t)n);
     free(x); free(l); free(xd); free(ld); free(w); free(wd); free(cdf); free(anc);
-    free(xt); free(xdt); free(tmax); free(tfd);
+    free(xt); free(xdt); free(tmax); free(tfd); free(xp);
     return count;
 }
 

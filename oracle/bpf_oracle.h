@@ -105,6 +105,32 @@ int orc_bpf_run(const OrcParams* p, int mode, const float* y, const float* mu_t,
                 int n_ticks, OrcTick* out, float* x_out, double* x64_out,
                 int* anc_out, int n_threads);
 
+/* Online mu/rho learner of gpu_bpf/gpu_bpf_ptx_full.cu:469-537,777-800
+ * (natural gradient + Robbins-Monro on the one-step terms of the reference's
+ * kernel 14, ptx_assembly/bpf_kernels_full.ptx:1116-1384). */
+typedef struct {
+    int   update_K;      /* ticks between updates (<= 0: every tick) */
+    int   learn_rho;     /* also update rho */
+    float c, t0, gamma;  /* eta_k = c / (k + t0)^gamma */
+    float grad_clip;     /* the reference sets 5 */
+} OrcLearn;
+
+/* One learner update in the reference's FP32 arithmetic: sums = the four
+ * accumulated sums (FP64, rounded to FP32 first), k = ticks accumulated,
+ * *rm_step the step counter (incremented); updates *mu and, if learn_rho,
+ * *rho. */
+void orc_learn_update(const OrcLearn* lp, const double* sums, int k, int* rm_step,
+                      float* mu, float* rho);
+
+/* orc_bpf_run (FP32 restatement only, mode 0) with the learner on from tick
+ * 0.  lsum_out (nullable, 4*T doubles) receives each tick's normalised sums
+ * sum_i w_i (g, g^2, g d, g^2 d^2)_i with g = d lw/dx of the propagated
+ * particle and d = x_prev - mu (0 on tick 0); par_out (nullable, 2*T floats)
+ * the (mu, rho) in force after each tick. */
+int orc_bpf_run_learn(const OrcParams* p, const OrcLearn* lp, const float* y, int n_ticks,
+                      OrcTick* out, double* lsum_out, float* par_out, float* x_out,
+                      int n_threads);
+
 /* Systematic resampling given a normalised FP64 CDF: lower_bound of the float
  * targets u + i/n (wrapped at 1) clamped to n-1. */
 void orc_resample_ancestors(const double* cdf, int n, float u, int* anc);
@@ -122,7 +148,9 @@ int orc_bpf_score(const OrcParams* p, const float* y, int n_ticks, int x0_repara
 
 /* Plant a mistake in the oracle (tests only): 0 none, 1 drop the -x/2 term
  * of the log-weight, 2 negate phi in the transition, 3 upper_bound instead
- * of lower_bound in resampling, 4 drop the weight carry.  Process-global. */
+ * of lower_bound in resampling, 4 drop the weight carry, 6 drop the factor
+ * 1/2 of eta^2 in the learner's d lw/dx, 7 take d = x' - mu (the propagated
+ * state) instead of x_prev - mu in the learner.  Process-global. */
 void orc_set_fault(int code);
 
 /* Exact log p(y_{1:T}) of the Gaussian SV model by a point-mass filter. */

@@ -31,6 +31,17 @@ class OrcParams(ctypes.Structure):
     ]
 
 
+class OrcLearn(ctypes.Structure):
+    _fields_ = [
+        ("update_K", ctypes.c_int),
+        ("learn_rho", ctypes.c_int),
+        ("c", ctypes.c_float),
+        ("t0", ctypes.c_float),
+        ("gamma", ctypes.c_float),
+        ("grad_clip", ctypes.c_float),
+    ]
+
+
 TICK_DTYPE = np.dtype(
     [
         ("mean", "f8"),
@@ -96,6 +107,12 @@ def lib():
         L.orc_dgp_regimes.argtypes = [i32, f64, f64, P, f64, u64, P, P]
         L.orc_bpf_run.restype = i32
         L.orc_bpf_run.argtypes = [ctypes.POINTER(OrcParams), i32, P, P, i32, P, P, P, P, i32]
+        L.orc_bpf_run_learn.restype = i32
+        L.orc_bpf_run_learn.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcLearn), P, i32,
+                                        P, P, P, P, i32]
+        L.orc_learn_update.restype = None
+        L.orc_learn_update.argtypes = [ctypes.POINTER(OrcLearn), P, i32, ctypes.POINTER(i32),
+                                       ctypes.POINTER(f32), ctypes.POINTER(f32)]
         L.orc_resample_ancestors.restype = None
         L.orc_resample_ancestors.argtypes = [P, i32, f32, P]
         L.orc_bpf_score.restype = i32
@@ -199,6 +216,39 @@ def run_filter(y, n_particles, rho=0.97, sigma_z=0.15, mu=-1.0, nu_obs=0.0,
     if cnt < 0:
         raise ValueError("oracle rejected the arguments")
     return out, cnt, (x if x is not None else x64), anc
+
+
+def run_filter_learn(y, n_particles, rho=0.97, sigma_z=0.15, mu=-1.0, nu_obs=0.0,
+                     ess_threshold=0.0, seed=42, tile=256, update_K=50, learn_rho=False,
+                     c=0.1, t0=10.0, gamma=0.667, grad_clip=5.0, want_x=False, threads=1):
+    """FP32 filter with the online mu/rho learner on from tick 0.
+    Returns (ticks, per-tick learner sums [T, 4], (mu, rho) after each tick [T, 2], x)."""
+    y = np.ascontiguousarray(y, np.float32)
+    T = len(y)
+    p = OrcParams(n_particles, rho, sigma_z, mu, nu_obs, ess_threshold, seed, tile)
+    lp = OrcLearn(int(update_K), 1 if learn_rho else 0, c, t0, gamma, grad_clip)
+    out = np.zeros(T, TICK_DTYPE)
+    ls = np.zeros((T, 4), np.float64)
+    par = np.zeros((T, 2), np.float32)
+    x = np.empty(n_particles, np.float32) if want_x else None
+    cnt = lib().orc_bpf_run_learn(ctypes.byref(p), ctypes.byref(lp), _ptr(y), T, _ptr(out),
+                                  _ptr(ls), _ptr(par), _ptr(x), int(threads))
+    if cnt < 0:
+        raise ValueError("oracle rejected the arguments")
+    return out, ls, par, x
+
+
+def learn_update(sums, k, rm_step, mu, rho, learn_rho=True, c=0.1, t0=10.0, gamma=0.667,
+                 grad_clip=5.0):
+    """One learner update; returns (rm_step, mu, rho) after it."""
+    lp = OrcLearn(1, 1 if learn_rho else 0, c, t0, gamma, grad_clip)
+    sm = np.ascontiguousarray(sums, np.float64)
+    st = ctypes.c_int(int(rm_step))
+    m = ctypes.c_float(mu)
+    r = ctypes.c_float(rho)
+    lib().orc_learn_update(ctypes.byref(lp), _ptr(sm), int(k), ctypes.byref(st),
+                           ctypes.byref(m), ctypes.byref(r))
+    return st.value, m.value, r.value
 
 
 def score(y, n_particles, rho=0.97, sigma_z=0.15, mu=-1.0, nu_obs=0.0, seed=42, tile=256,
