@@ -202,26 +202,33 @@ struct ObsModel {
     float nph;      // (nu + 1) / 2
 };
 
-__device__ inline float log_weight_gauss(float x, float ysq) {
-    float eta2 = ysq * expf_cw(-x);
+// Log-weights given eta^2 = y^2 e^{-x}.
+__device__ inline float log_weight_gauss_eta(float x, float eta2) {
     float lw = __builtin_fmaf(-0.5f, eta2, bits_f32(0xBF6B3F8Eu));
     return __builtin_fmaf(-0.5f, x, lw);
 }
 
-__device__ inline float log_weight_t(float x, const ObsModel& o) {
-    float eta2 = o.ysq * expf_cw(-x);
+__device__ inline float log_weight_t_eta(float x, float eta2, const ObsModel& o) {
     float ta = eta2 / o.nu + 1.0f;
     float lw = __builtin_fmaf(-o.nph, logf_cw(ta), o.c_obs);
     return __builtin_fmaf(-0.5f, x, lw);
+}
+
+__device__ inline float log_weight_gauss(float x, float ysq) {
+    return log_weight_gauss_eta(x, ysq * expf_cw(-x));
+}
+
+__device__ inline float log_weight_t(float x, const ObsModel& o) {
+    return log_weight_t_eta(x, o.ysq * expf_cw(-x), o);
 }
 
 // d lw / dx of the observation log-density at the propagated state, in the
 // reference's form (ptx_assembly/bpf_kernels_full.ptx:1172-1189): with
 // eta^2 = y^2 e^{-x}, Gaussian -1/2 + eta^2/2, Student-t
 // -1/2 + (nu+1)/2 * eta^2 / (nu + eta^2) (correctly rounded division).
-__device__ inline float learn_dlw(float x, const ObsModel& o) {
-    const float eta2 = o.ysq * expf_cw(-x);
-    if (o.nu > 0.0f) return __builtin_fmaf(o.nph, eta2 / (o.nu + eta2), -0.5f);
+template <bool STUDENT>
+__device__ inline float learn_dlw_eta(float eta2, const ObsModel& o) {
+    if (STUDENT) return __builtin_fmaf(o.nph, eta2 / (o.nu + eta2), -0.5f);
     return __builtin_fmaf(eta2, 0.5f, -0.5f);
 }
 

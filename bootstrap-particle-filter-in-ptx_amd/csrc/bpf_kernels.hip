@@ -122,8 +122,9 @@ __device__ inline void lane_normals(uint64_t pcg_a, uint64_t pcg_g, long long ba
 // the group's sums.
 // ---------------------------------------------------------------------------
 
-__device__ inline void group_fold(const Partials& p, int g, int tid, float* s_f, double* s_a,
-                                  double (*s_d)[kGroup / kWave]) {
+// Returns the calling thread's tile scale f (0 past the last tile).
+__device__ inline double group_fold(const Partials& p, int g, int tid, float* s_f, double* s_a,
+                                    double (*s_d)[kGroup / kWave]) {
     constexpr int NW = kGroup / kWave;
     const int lane = tid & (kWave - 1), wid = tid / kWave;
     const int t = g * kGroup + tid;
@@ -176,6 +177,7 @@ __device__ inline void group_fold(const Partials& p, int g, int tid, float* s_f,
         p.gsxx[g] = r[2];
         p.gsq[g] = r[3];
     }
+    return f;
 }
 
 // ---------------------------------------------------------------------------
@@ -258,14 +260,40 @@ __device__ inline void lane_fisher_sums(const float (&l)[K], const float (&al)[3
     }
 }
 
-// FP64 sums of e * v_q, v = (g, g^2, g d, g^2 d^2), of the lane's particles
-// for online learning: g = d lw/dx at the propagated state, d = x_prev - mu
-// (ptx_assembly/bpf_kernels_full.ptx:1191-1208 with w = e; padding
-// contributes zero).
-template <int K>
-__device__ inline void lane_learn_sums(const float (&x)[K], const float (&l)[K], const float (&dp)[K],
-                                       const ObsModel& obs, float mb, int tid, int rem,
-                                       double (&L)[4]) {
+// Learning variant of lane_log_weights: also returns g = d lw/dx at each
+// particle from the log-weight's own eta^2 (learn_dlw_eta; the log-weight
+// bits are those of lane_log_weights).
+template <int K, bool STUDENT, bool MASK>
+__device__ inline float lane_log_weights_grad(const float (&x)[K], float (&l)[K], float (&g)[K],
+                                              const ObsModel& obs, float moff, int tid, int rem) {
+    constexpr int VEC = K < 4 ? K : 4;
+    constexpr int NV = K / VEC;
+    float mloc = -__builtin_inff();
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) {
+            const int k = v * VEC + c;
+            const int li = (v * kBlock + tid) * VEC + c;
+            const float eta2 = obs.ysq * expf_cw(-x[k]);
+            const float lw = STUDENT ? log_weight_t_eta(x[k], eta2, obs) : log_weight_gauss_eta(x[k], eta2);
+            g[k] = learn_dlw_eta<STUDENT>(eta2, obs);
+            const float lk = lw + (l[k] - moff);
+            l[k] = (!MASK || li < rem) ? lk : -__builtin_inff();
+            mloc = fmaxf(mloc, l[k]);
+        }
+    }
+    return mloc;
+}
+
+// lane_weight_sums plus the FP64 learning sums e * v_q, v = (g, g^2, g d,
+// g^2 d^2) with d = x_prev - mu (ptx_assembly/bpf_kernels_full.ptx:1191-1208
+// with w = e); padding contributes zero.
+template <int K, bool MASK>
+__device__ inline void lane_weight_learn_sums(const float (&x)[K], const float (&l)[K],
+                                              const float (&g)[K], const float (&dp)[K], float mb,
+                                              int tid, int rem, double& S, double& SX, double& SXX,
+                                              double& Q, double (&L)[4]) {
     constexpr int VEC = K < 4 ? K : 4;
     constexpr int NV = K / VEC;
 #pragma unroll
@@ -274,17 +302,22 @@ __device__ inline void lane_learn_sums(const float (&x)[K], const float (&l)[K],
         for (int c = 0; c < VEC; ++c) {
             const int k = v * VEC + c;
             const int li = (v * kBlock + tid) * VEC + c;
-            if (li < rem) {
-                const double de = (double)expf_cw_nonpos(l[k] - mb);
-                const double g = (double)learn_dlw(x[k], obs);
-                const double d = (double)dp[k];
-                const double t0 = de * g;
-                const double t1 = t0 * g;
-                L[0] += t0;
-                L[1] += t1;
-                L[2] += t0 * d;
-                L[3] += (t1 * d) * d;
-            }
+            const bool live = !MASK || li < rem;
+            const double de = live ? (double)expf_cw_nonpos(l[k] - mb) : 0.0;
+            const double dx = live ? (double)x[k] : 0.0;
+            const double t = de * dx;
+            S += de;
+            SX += t;
+            SXX = fma(t, dx, SXX);
+            Q = fma(de, de, Q);
+            const double gg = live ? (double)g[k] : 0.0;
+            const double d = live ? (double)dp[k] : 0.0;
+            const double t0 = de * gg;
+            const double t1 = t0 * gg;
+            L[0] += t0;
+            L[1] += t1;
+            L[2] += t0 * d;
+            L[3] += (t1 * d) * d;
         }
     }
 }
@@ -293,7 +326,7 @@ __device__ inline void lane_learn_sums(const float (&x)[K], const float (&l)[K],
 // each particle through the transition (alpha += grad log N(x'; mu + phi
 // (x - mu), sigma^2), with z = (x' - mu - phi (x - mu)) / sigma the drawn
 // normal) and write the per-tile sums of e * alpha.
-// LEARN: also write the per-tile sums of online learning (lane_learn_sums);
+// LEARN: also write the per-tile sums of online learning (lane_weight_learn_sums);
 // d = x_prev - mu is 0 on tick 0, which propagates nothing.
 template <int K, bool FISH, bool LEARN>
 __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
@@ -302,7 +335,10 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
     constexpr int NV = K / VEC;
     constexpr int TILE = kBlock * K;
     constexpr int NW = kBlock / kWave;
-    constexpr bool NT = false;     // non-temporal streams measured slower on gfx950 here
+    // x and l stream through with non-temporal loads and stores: measured on
+    // MI355X at 51.2M particles, 149 us per launch against 156 us with plain
+    // ones (non-temporal stores alone: 156 us, loads alone: 152 us)
+    constexpr bool NTL = true, NTS = true;
     __shared__ float s_q[TILE];
     __shared__ int s_cnt[NW];
     __shared__ float s_m[NW];
@@ -325,14 +361,14 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         long long off = base + (long long)(v * kBlock + tid) * VEC;
-        if constexpr (NT) load_vec_nt<VEC>(xb + off, &x[v * VEC]);
+        if constexpr (NTL) load_vec_nt<VEC>(xb + off, &x[v * VEC]);
         else load_vec<VEC>(xb + off, &x[v * VEC]);
     }
     if (carry) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             long long off = base + (long long)(v * kBlock + tid) * VEC;
-            if constexpr (NT) load_vec_nt<VEC>(a.lw + off, &l[v * VEC]);
+            if constexpr (NTL) load_vec_nt<VEC>(a.lw + off, &l[v * VEC]);
             else load_vec<VEC>(a.lw + off, &l[v * VEC]);
         }
     }
@@ -392,7 +428,7 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             long long off = base + (long long)(v * kBlock + tid) * VEC;
-            if constexpr (NT) store_vec_nt<VEC>(xb + off, &x[v * VEC]);
+            if constexpr (NTS) store_vec_nt<VEC>(xb + off, &x[v * VEC]);
             else store_vec<VEC>(xb + off, &x[v * VEC]);
         }
     }
@@ -404,14 +440,20 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
     const int rem = (a.n - base >= TILE) ? TILE : (int)(a.n - base);   // live particles
     const bool full = rem == TILE;
     float mloc_l;
-    if (obs.nu > 0.0f) mloc_l = lane_log_weights<K, true, true>(x, l, obs, moff, tid, rem);
-    else if (full) mloc_l = lane_log_weights<K, false, false>(x, l, obs, moff, tid, rem);
-    else mloc_l = lane_log_weights<K, false, true>(x, l, obs, moff, tid, rem);
+    float g[LEARN ? K : 1];
+    if constexpr (LEARN) {
+        if (obs.nu > 0.0f) mloc_l = lane_log_weights_grad<K, true, true>(x, l, g, obs, moff, tid, rem);
+        else mloc_l = lane_log_weights_grad<K, false, true>(x, l, g, obs, moff, tid, rem);
+    } else {
+        if (obs.nu > 0.0f) mloc_l = lane_log_weights<K, true, true>(x, l, obs, moff, tid, rem);
+        else if (full) mloc_l = lane_log_weights<K, false, false>(x, l, obs, moff, tid, rem);
+        else mloc_l = lane_log_weights<K, false, true>(x, l, obs, moff, tid, rem);
+    }
     if (a.store_l) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             long long off = base + (long long)(v * kBlock + tid) * VEC;
-            if constexpr (NT) store_vec_nt<VEC>(a.lw + off, &l[v * VEC]);
+            if constexpr (NTS) store_vec_nt<VEC>(a.lw + off, &l[v * VEC]);
             else store_vec<VEC>(a.lw + off, &l[v * VEC]);
         }
     }
@@ -424,9 +466,14 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
     for (int w = 1; w < NW; ++w) mb = fmaxf(mb, s_m[w]);
 
     double S = 0.0, SX = 0.0, SXX = 0.0, Q = 0.0;
+    double L[LEARN ? 4 : 1] = {};
     if (mb != -__builtin_inff()) {      // a tile without live weight keeps zero sums
-        if (full) lane_weight_sums<K, false>(x, l, mb, tid, rem, S, SX, SXX, Q);
-        else lane_weight_sums<K, true>(x, l, mb, tid, rem, S, SX, SXX, Q);
+        if constexpr (LEARN) {
+            lane_weight_learn_sums<K, true>(x, l, g, dp, mb, tid, rem, S, SX, SXX, Q, L);
+        } else {
+            if (full) lane_weight_sums<K, false>(x, l, mb, tid, rem, S, SX, SXX, Q);
+            else lane_weight_sums<K, true>(x, l, mb, tid, rem, S, SX, SXX, Q);
+        }
     }
     S = wave_sum(S);
     SX = wave_sum(SX);
@@ -448,8 +495,6 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
         }
     }
     if constexpr (LEARN) {
-        double L[4] = {0.0, 0.0, 0.0, 0.0};
-        if (mb != -__builtin_inff()) lane_learn_sums<K>(x, l, dp, obs, mb, tid, rem, L);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             L[q] = wave_sum(L[q]);
@@ -509,12 +554,32 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
     __shared__ int s_flag;
     __shared__ double s_S;
     __shared__ int s_last;
+    __shared__ double s_l[4][NW];
 
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const int Gg = a.part.n_groups;
     constexpr float NEG_INF = -__builtin_inff();
 
-    group_fold(a.part, blockIdx.x, tid, s_f, s_a, s_d);
+    const double ft = group_fold(a.part, blockIdx.x, tid, s_f, s_a, s_d);
+    if (a.lsum) {
+        // the group's online-learning sums: tile sums scaled by the tile scales
+        const int t = blockIdx.x * kGroup + tid;
+        const int G = a.part.n_tiles;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double v = (t < G) ? ft * a.lsum[(size_t)q * G + t] : 0.0;
+            v = wave_sum(v);
+            if (lane == 0) s_l[q][wid] = v;
+        }
+        __syncthreads();
+        if (tid < 4) {
+            double acc = s_l[tid][0];
+#pragma unroll
+            for (int w = 1; w < NW; ++w) acc += s_l[tid][w];
+            a.glsum[(size_t)tid * Gg + blockIdx.x] = acc;
+        }
+        __syncthreads();
+    }
     if (tid == 0) {
         __threadfence();
         const int old = atomicAdd(a.part.gcnt, 1);
@@ -575,8 +640,45 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
         s_d[2][wid] = SXX;
         s_d[3][wid] = Q;
     }
+    if (a.lsum) {
+        // online learning: sum over groups of f_g * (group sums), as the weights
+        double Lq[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < kFinPer; ++j) {
+            const int g = g0 + j;
+            if (j < C && g < Gg) {
+                const double f = (gm[j] == NEG_INF) ? 0.0 : (double)expf_cw_nonpos(gm[j] - M);
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    Lq[q] = fma(f, __hip_atomic_load(&a.glsum[(size_t)q * Gg + g], __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT), Lq[q]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            Lq[q] = wave_sum(Lq[q]);
+            if (lane == 0) s_l[q][wid] = Lq[q];
+        }
+    }
     __syncthreads();
 
+    if (tid == 0 && a.lsum) {
+        // the reference's kernel-14 accumulators d_scalars[5..8]
+        // (ptx_assembly/bpf_kernels_full.ptx:1373-1377), normalised by the
+        // tick's weight sum; a tick without finite positive weight adds nothing
+        double St = s_d[0][0];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) St += s_d[0][w];
+        if (St > 0.0 && isfinite(St)) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                double r = s_l[q][0];
+#pragma unroll
+                for (int w = 1; w < NW; ++w) r += s_l[q][w];
+                a.lacc[q] += r / St;
+            }
+        }
+    }
     if (tid == 0) {
         double t4[4];
 #pragma unroll
@@ -1132,52 +1234,9 @@ __global__ __launch_bounds__(kBlock) void k_fisher_score(FisherArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Online learning: acc[q] += sum_i w_i v_qi of the latest tick, the tile sums
-// of e * v_q scaled by tile b's normalised weight scale tf_b exp(gm_g - M) / S
-// (the reference's kernel-14 atomics into d_scalars[5..8],
-// ptx_assembly/bpf_kernels_full.ptx:1373-1377).  One block, fixed order.
-// ---------------------------------------------------------------------------
-
-__global__ __launch_bounds__(kBlock) void k_learn_accum(LearnArgs a) {
-    constexpr int NW = kBlock / kWave;
-    constexpr float NEG_INF = -__builtin_inff();
-    __shared__ double s_r[4][NW];
-    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
-    const float M = a.ctl->M;
-    const double S = a.ctl->S;
-    const int G = a.part.n_tiles;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int b = tid; b < G; b += kBlock) {
-        const float gm = a.part.gm[b / kGroup];
-        const double fg = (gm == NEG_INF) ? 0.0 : (double)expf_cw(gm - M);
-        const double f = a.part.tf[b] * fg;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) acc[c] = fma(f, a.lsum[(size_t)c * G + b], acc[c]);
-    }
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        acc[c] = wave_sum(acc[c]);
-        if (lane == 0) s_r[c][wid] = acc[c];
-    }
-    __syncthreads();
-    if (tid < 4) {
-        double r = s_r[tid][0];
-#pragma unroll
-        for (int w = 1; w < NW; ++w) r += s_r[tid][w];
-        // a tick without finite positive weight adds nothing (its statistics
-        // are NaN and it does not resample)
-        if (S > 0.0 && isfinite(S)) a.acc[tid] += r / S;
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 
-hipError_t launch_learn_accum(const LearnArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_learn_accum, dim3(1), dim3(kBlock), 0, st, a);
-    return hipGetLastError();
-}
 
 hipError_t launch_fisher_score(const FisherArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_fisher_score, dim3(1), dim3(kBlock), 0, st, a);

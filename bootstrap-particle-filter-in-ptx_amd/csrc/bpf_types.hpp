@@ -106,6 +106,12 @@ struct FinArgs {
     float* u_log;         // optional: resampling offset of this tick
     int* w_log;           // optional: wrap start of this tick
     int* tile_ctr;        // resample work counters, cleared on resampling ticks
+    // online learning (null: off): the propagate pass's tile sums (4 x
+    // n_tiles), per-group folds of them (4 x n_groups), and the accumulators
+    // to which the tick's normalised sums are added (4)
+    const double* lsum;
+    double* glsum;
+    double* lacc;
 };
 
 struct ResArgs {
@@ -156,15 +162,6 @@ struct InitArgs {
     int n, n_pad;
     float* alpha[3];      // optional grad log N(x_0; mu, sd^2), zero on padding
     float dsd_dphi, dsd_dsig;
-};
-
-// Online learning: adds the latest tick's normalised sums sum_i w_i v_qi
-// (tile sums of e * v_q scaled as the Fisher score's) to acc[0..3].
-struct LearnArgs {
-    Partials part;
-    const double* lsum;
-    const Ctl* ctl;
-    double* acc;
 };
 
 // Fisher-identity score of the latest tick: sum_i w_i alpha_i from the tile
