@@ -423,8 +423,28 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
                     store_vec<VEC>(a.alpha[cur][q] + off, &al[q][v * VEC]);
                 }
         }
+        // sigma_z * k_band: one scale for the tile unless a band boundary
+        // falls inside it (default: one band, k = 1)
+        float sig_t = a.sig * a.bands.k[0];
+        bool split = false;
+        if (a.bands.n > 1) {
+            const long long last = base + TILE - 1;
+            sig_t = a.sig * band_k(a.bands, base);
+            split = band_of(a.bands, base) != band_of(a.bands, last);
+        }
+        if (split) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) x[k] = transition(x[k], a.mu, a.phi, a.sig, z[k]);
+            for (int v = 0; v < NV; ++v)
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) {
+                    const int k = v * VEC + c;
+                    const long long i = base + (long long)(v * kBlock + tid) * VEC + c;
+                    x[k] = transition(x[k], a.mu, a.phi, a.sig * band_k(a.bands, i), z[k]);
+                }
+        } else {
+#pragma unroll
+            for (int k = 0; k < K; ++k) x[k] = transition(x[k], a.mu, a.phi, sig_t, z[k]);
+        }
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             long long off = base + (long long)(v * kBlock + tid) * VEC;

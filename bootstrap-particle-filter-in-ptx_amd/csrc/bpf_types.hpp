@@ -66,6 +66,25 @@ struct Partials {
     int n_groups;
 };
 
+// sigma_z scale bands of the reference's MixBandConfig (gpu_bpf_full.cuh:
+// 51-55): particle i < bnd[0] scales sigma_z by k[0], i < bnd[1] by k[1],
+// i < bnd[2] by k[2], else by k[3]; with n <= 1 every particle takes k[0]
+// (ptx_assembly/bpf_kernels_full.ptx:297-324).
+struct Bands {
+    int n;
+    int bnd[3];
+    float k[4];
+};
+
+__device__ __host__ inline int band_of(const Bands& b, long long i) {
+    return i < b.bnd[0] ? 0 : (i < b.bnd[1] ? 1 : (i < b.bnd[2] ? 2 : 3));
+}
+
+// k of particle i's band (selects, no indexed access)
+__device__ __host__ inline float band_k(const Bands& b, long long i) {
+    return i < b.bnd[0] ? b.k[0] : (i < b.bnd[1] ? b.k[1] : (i < b.bnd[2] ? b.k[2] : b.k[3]));
+}
+
 struct PropArgs {
     float* x0;
     float* x1;
@@ -80,6 +99,7 @@ struct PropArgs {
     ObsModel obs;
     int n;
     int do_prop;          // 0 on tick 0 (ptx_assembly/bpf_kernels_full.ptx:294)
+    Bands bands;          // sigma_z scales of the tick's transition
     int store_l;          // keep log-weights for a possible carry
     float* x_save;        // optional copy of the pre-propagation states
     // Fisher-identity score (null: off): per particle a 3-vector

@@ -85,6 +85,8 @@ def load(path=LIB_PATH):
         "gpu_bpf_disable_mu_learning": (None, [P]),
         "gpu_bpf_enable_rho_learning": (None, [P, I]),
         "gpu_bpf_set_bands": (None, [I, I, P, P]),
+        "gpu_bpf_set_adaptive_bands": (None, [I, P, P, I, P, P, I, P, P, I, F, F]),
+        "gpu_bpf_get_band_regime": (I, [P]),
         "gpu_bpf_last_error": (ctypes.c_char_p, []),
         "gpu_bpf_run": (I, [P, P, I, P, P]),
         "gpu_bpf_run_device": (I, [P, P, I]),
@@ -192,6 +194,9 @@ class GpuBpf:
     def enable_rho_learning(self, enable=True):
         self.L.gpu_bpf_enable_rho_learning(self.h, int(enable))
 
+    def band_regime(self):
+        return self._check(self.L.gpu_bpf_get_band_regime(self.h))
+
     def learn_sums(self):
         """(sums[4], ticks): the learner's accumulators since its last update."""
         out = np.zeros(4, np.float64)
@@ -287,6 +292,30 @@ class GpuBpf:
         cnt = np.zeros(3, np.int32)
         self._check(self.L.gpu_bpf_kernel_times(self.h, _ptr(ms), _ptr(cnt), 1 if reset else 0))
         return ms, cnt
+
+
+def set_bands(n_particles, fracs, scales):
+    """gpu_bpf_set_bands: process-wide sigma_z scale bands (adopted at each filter's next tick)."""
+    f = np.ascontiguousarray(fracs, np.float32)
+    sc = np.ascontiguousarray(scales, np.float32)
+    load().gpu_bpf_set_bands(int(n_particles), len(sc), _ptr(f), _ptr(sc))
+
+
+def set_adaptive_bands(n_particles, calm, alert, panic, thresh_alert=2.0, thresh_panic=4.0):
+    """gpu_bpf_set_adaptive_bands; calm / alert / panic are (fracs, scales) pairs."""
+    arrs = []
+    args = [int(n_particles)]
+    for fr, sc in (calm, alert, panic):
+        f = np.ascontiguousarray(fr, np.float32)
+        c = np.ascontiguousarray(sc, np.float32)
+        arrs += [f, c]
+        args += [_ptr(f), _ptr(c), len(c)]
+    load().gpu_bpf_set_adaptive_bands(*args, float(thresh_alert), float(thresh_panic))
+
+
+def reset_bands(n_particles=1):
+    """One band of scale 1 (the default)."""
+    set_bands(n_particles, [1.0], [1.0])
 
 
 def run_rmse(returns, true_h, n_particles, rho, sigma_z, mu, nu_state=0.0, nu_obs=0.0, seed=42):

@@ -115,9 +115,18 @@ void gpu_bpf_enable_mu_learning(GpuBpfState* state, int mode, int K,
 void gpu_bpf_disable_mu_learning(GpuBpfState* state);
 void gpu_bpf_enable_rho_learning(GpuBpfState* state, int enable);
 
-/* gpu_bpf_full.cuh:245-272 (sigma_z scale bands).  Only the trivial single
- * band (n_bands <= 1, scale 1) is provided; anything else records an error
- * and is ignored.  Kept so that callers link. */
+/* gpu_bpf_full.cuh:232-272 / gpu_bpf_ptx_full.cu:257-334 (sigma_z scale
+ * bands).  Particles [0, b0) scale sigma_z by scales[0], [b0, b1) by
+ * scales[1], ... with b the rounded cumulative fractions of n_particles
+ * (the last band takes the rest); 1 to 4 bands.  As in the reference the
+ * setters take no filter: the configuration is process-wide and every filter
+ * adopts it at its next tick.  The adaptive variant holds a calm / alert /
+ * panic table switched by the surprise |y_t| e^{-h_t/2} of the previous
+ * gpu_bpf_step (EMA 0.3, thresholds as given); the regime machine runs per
+ * filter (the reference keeps one for the process).  gpu_bpf_run steps tick
+ * by tick while adaptive bands are set; gpu_bpf_step_async and
+ * gpu_bpf_run_device leave the surprise unchanged, as the reference's
+ * step_async does.  Bad arguments record an error and change nothing. */
 void gpu_bpf_set_bands(int n_particles, int n_bands, const float* fracs,
                        const float* scales);
 void gpu_bpf_set_adaptive_bands(int n_particles,
@@ -130,6 +139,10 @@ void gpu_bpf_set_adaptive_bands(int n_particles,
 
 /* Thread-local text of the last error ("" if none). */
 const char* gpu_bpf_last_error(void);
+
+/* The adaptive bands' current regime of this filter: 0 calm, 1 alert,
+ * 2 panic (0 without adaptive bands), or -1 with gpu_bpf_last_error(). */
+int gpu_bpf_get_band_regime(GpuBpfState* state);
 
 /* The learner's four accumulated sums since its last update (sum over ticks
  * of sum_i w_i g_i, sum w g^2, sum w g d, sum w g^2 d^2; the reference's

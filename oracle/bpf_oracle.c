@@ -454,16 +454,55 @@ void orc_learn_update(const OrcLearn* lp, const double* sums, int k, int* rm_ste
     }
 }
 
+/* MixBandConfig and its builder, gpu_bpf/gpu_bpf_ptx_full.cu:257-275 */
+typedef struct { int n; int bnd[4]; float k[4]; } BandCfg;
+
+static BandCfg band_build(int n_particles, int n_bands, const float* fracs, const float* scales) {
+    BandCfg c;
+    memset(&c, 0, sizeof c);
+    c.n = n_bands;
+    int cum = 0;
+    for (int b = 0; b < n_bands; b++) {
+        int count = (b < n_bands - 1) ? (int)(fracs[b] * n_particles + 0.5f) : (n_particles - cum);
+        cum += count;
+        c.bnd[b] = cum;
+        c.k[b] = scales[b];
+    }
+    for (int b = n_bands; b < 4; b++) {
+        c.bnd[b] = cum;
+        c.k[b] = 1.0f;
+    }
+    return c;
+}
+
+/* ptx_assembly/bpf_kernels_full.ptx:297-324: one band uses k[0] for all */
+static inline float band_scale(const BandCfg* c, int i) {
+    if (c->n <= 1) return c->k[0];
+    if (i < c->bnd[0]) return c->k[0];
+    if (i < c->bnd[1]) return c->k[1];
+    if (i < c->bnd[2]) return c->k[2];
+    return c->k[3];
+}
+
 static int bpf_run_impl(const OrcParams* p, int mode, const float* y, const float* mu_t,
                         int n_ticks, OrcTick* out, float* x_out, double* x64_out,
                         int* anc_out, int n_threads, const OrcLearn* lp, double* lsum_out,
-                        float* par_out);
+                        float* par_out, const OrcBands* bd, int* regime_out);
 
 int orc_bpf_run(const OrcParams* p, int mode, const float* y, const float* mu_t,
                 int n_ticks, OrcTick* out, float* x_out, double* x64_out,
                 int* anc_out, int n_threads) {
     return bpf_run_impl(p, mode, y, mu_t, n_ticks, out, x_out, x64_out, anc_out, n_threads,
-                        NULL, NULL, NULL);
+                        NULL, NULL, NULL, NULL, NULL);
+}
+
+int orc_bpf_run_bands(const OrcParams* p, const OrcBands* bd, const float* y, int n_ticks,
+                      OrcTick* out, float* x_out, int* regime_out, int n_threads) {
+    if (!bd) return -1;
+    for (int c = 0; c < 3; c++)
+        if (bd->n_bands[c] < 1 || bd->n_bands[c] > 4) return -1;
+    return bpf_run_impl(p, ORC_MODE_F32, y, NULL, n_ticks, out, x_out, NULL, NULL, n_threads,
+                        NULL, NULL, NULL, bd, regime_out);
 }
 
 int orc_bpf_run_learn(const OrcParams* p, const OrcLearn* lp, const float* y, int n_ticks,
@@ -471,16 +510,24 @@ int orc_bpf_run_learn(const OrcParams* p, const OrcLearn* lp, const float* y, in
                       int n_threads) {
     if (!lp) return -1;
     return bpf_run_impl(p, ORC_MODE_F32, y, NULL, n_ticks, out, x_out, NULL, NULL, n_threads,
-                        lp, lsum_out, par_out);
+                        lp, lsum_out, par_out, NULL, NULL);
 }
 
 static int bpf_run_impl(const OrcParams* p, int mode, const float* y, const float* mu_t,
                         int n_ticks, OrcTick* out, float* x_out, double* x64_out,
                         int* anc_out, int n_threads, const OrcLearn* lp, double* lsum_out,
-                        float* par_out) {
+                        float* par_out, const OrcBands* bd, int* regime_out) {
     const int n = p->n_particles;
     if (n <= 0 || n_ticks < 0 || p->tile <= 0) return -1;
     if (lp && (mode != ORC_MODE_F32 || mu_t)) return -1;
+    BandCfg bcfg[3];
+    int regime = 0;
+    float surprise = 0.0f, sur_ema = 0.0f;
+    if (bd) {
+        if (mode != ORC_MODE_F32) return -1;
+        for (int c = 0; c < 3; c++)
+            bcfg[c] = band_build(bd->n_particles, bd->n_bands[c], bd->fracs[c], bd->scales[c]);
+    }
     const int tile = p->tile;
     const int n_tiles = (n + tile - 1) / tile;
 #ifdef _OPENMP
@@ -547,6 +594,30 @@ static int bpf_run_impl(const OrcParams* p, int mode, const float* y, const floa
         const float yt = y[t];
         const float ysq = yt * yt;
         const PcgJump jt = pcg_jump(p->seed, (uint32_t)t);
+        /* the tick's band configuration; the regime switches on the previous
+         * tick's surprise before propagating (gpu_bpf_ptx_full.cu:312-334,585-588) */
+        const BandCfg* act = NULL;
+        if (bd) {
+            if (bd->adaptive) {
+                if (t > 0) {
+                    sur_ema = 0.3f * surprise + (1.0f - 0.3f) * sur_ema;
+                    if (surprise < bd->thr_alert) {
+                        regime = 0;
+                        sur_ema = surprise;
+                    } else if (sur_ema >= bd->thr_panic) {
+                        regime = 2;
+                    } else if (sur_ema >= bd->thr_alert) {
+                        regime = 1;
+                    } else {
+                        regime = 0;
+                    }
+                }
+                act = &bcfg[regime];
+            } else {
+                act = &bcfg[0];
+            }
+            if (regime_out) regime_out[t] = bd->adaptive ? regime : 0;
+        }
 
         double M = 0.0;
         if (do32) {
@@ -559,7 +630,7 @@ static int bpf_run_impl(const OrcParams* p, int mode, const float* y, const floa
                     float d = xi - mu;
                     d = (g_fault == 2 ? -rho : rho) * d;
                     xi = mu + d;
-                    xi = fmaf(sig, z, xi);
+                    xi = fmaf(act ? sig * band_scale(act, i) : sig, z, xi);
                     x[i] = xi;
                 }
                 float lw = lw_f32(xi, ysq, nu_obs, c_obs, nph);
@@ -643,6 +714,8 @@ static int bpf_run_impl(const OrcParams* p, int mode, const float* y, const floa
                        (p->ess_threshold <= 0.0f || (ess_dec / (double)n) < (double)p->ess_threshold);
         r.resampled = resample;
         if (out) out[t] = r;
+        /* surprise of this tick for the next (gpu_bpf_ptx_full.cu:767-769) */
+        surprise = fabsf(yt) * expf(-r.h_mean * 0.5f);
 
         if (lp) {
             /* the learner's one-step sums of this tick on the FP32 particles

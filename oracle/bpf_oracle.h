@@ -131,6 +131,24 @@ int orc_bpf_run_learn(const OrcParams* p, const OrcLearn* lp, const float* y, in
                       OrcTick* out, double* lsum_out, float* par_out, float* x_out,
                       int n_threads);
 
+/* sigma_z scale bands (gpu_bpf/gpu_bpf_ptx_full.cu:257-334,
+ * ptx_assembly/bpf_kernels_full.ptx:297-324).  Config 0 is the static one;
+ * with adaptive set, configs 0/1/2 are calm/alert/panic, switched before each
+ * propagating tick by the previous tick's surprise |y| e^{-h_mean/2}. */
+typedef struct {
+    int   adaptive;
+    int   n_particles;   /* the count the fractions refer to */
+    int   n_bands[3];
+    float fracs[3][4];
+    float scales[3][4];
+    float thr_alert, thr_panic;
+} OrcBands;
+
+/* orc_bpf_run (FP32 restatement, mode 0) with bands; regime_out (nullable,
+ * T ints) receives the regime each tick propagated with. */
+int orc_bpf_run_bands(const OrcParams* p, const OrcBands* bd, const float* y, int n_ticks,
+                      OrcTick* out, float* x_out, int* regime_out, int n_threads);
+
 /* Systematic resampling given a normalised FP64 CDF: lower_bound of the float
  * targets u + i/n (wrapped at 1) clamped to n-1. */
 void orc_resample_ancestors(const double* cdf, int n, float u, int* anc);

@@ -42,6 +42,18 @@ class OrcLearn(ctypes.Structure):
     ]
 
 
+class OrcBands(ctypes.Structure):
+    _fields_ = [
+        ("adaptive", ctypes.c_int),
+        ("n_particles", ctypes.c_int),
+        ("n_bands", ctypes.c_int * 3),
+        ("fracs", (ctypes.c_float * 4) * 3),
+        ("scales", (ctypes.c_float * 4) * 3),
+        ("thr_alert", ctypes.c_float),
+        ("thr_panic", ctypes.c_float),
+    ]
+
+
 TICK_DTYPE = np.dtype(
     [
         ("mean", "f8"),
@@ -110,6 +122,9 @@ def lib():
         L.orc_bpf_run_learn.restype = i32
         L.orc_bpf_run_learn.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcLearn), P, i32,
                                         P, P, P, P, i32]
+        L.orc_bpf_run_bands.restype = i32
+        L.orc_bpf_run_bands.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcBands), P, i32,
+                                        P, P, P, i32]
         L.orc_learn_update.restype = None
         L.orc_learn_update.argtypes = [ctypes.POINTER(OrcLearn), P, i32, ctypes.POINTER(i32),
                                        ctypes.POINTER(f32), ctypes.POINTER(f32)]
@@ -236,6 +251,37 @@ def run_filter_learn(y, n_particles, rho=0.97, sigma_z=0.15, mu=-1.0, nu_obs=0.0
     if cnt < 0:
         raise ValueError("oracle rejected the arguments")
     return out, ls, par, x
+
+
+def run_filter_bands(y, n_particles, configs, adaptive=False, band_n=None, rho=0.97, sigma_z=0.15,
+                     mu=-1.0, nu_obs=0.0, ess_threshold=0.0, seed=42, tile=256, thr_alert=2.0,
+                     thr_panic=4.0, want_x=False, threads=1):
+    """FP32 filter with sigma_z bands.  configs: one (fracs, scales) pair (static) or
+    three (calm, alert, panic; adaptive).  band_n: the particle count the
+    fractions refer to (default n_particles).  Returns (ticks, regimes[T], x)."""
+    y = np.ascontiguousarray(y, np.float32)
+    T = len(y)
+    p = OrcParams(n_particles, rho, sigma_z, mu, nu_obs, ess_threshold, seed, tile)
+    b = OrcBands()
+    b.adaptive = 1 if adaptive else 0
+    b.n_particles = int(n_particles if band_n is None else band_n)
+    cfgs = list(configs) if adaptive else [configs] * 3
+    for c, (fr, sc) in enumerate(cfgs):
+        b.n_bands[c] = len(sc)
+        for q, v in enumerate(fr):
+            b.fracs[c][q] = v
+        for q, v in enumerate(sc):
+            b.scales[c][q] = v
+    b.thr_alert = thr_alert
+    b.thr_panic = thr_panic
+    out = np.zeros(T, TICK_DTYPE)
+    reg = np.zeros(T, np.int32)
+    x = np.empty(n_particles, np.float32) if want_x else None
+    cnt = lib().orc_bpf_run_bands(ctypes.byref(p), ctypes.byref(b), _ptr(y), T, _ptr(out), _ptr(x),
+                                  _ptr(reg), int(threads))
+    if cnt < 0:
+        raise ValueError("oracle rejected the arguments")
+    return out, reg, x
 
 
 def learn_update(sums, k, rm_step, mu, rho, learn_rho=True, c=0.1, t0=10.0, gamma=0.667,
