@@ -328,9 +328,12 @@ __device__ inline void lane_weight_learn_sums(const float (&x)[K], const float (
 // normal) and write the per-tile sums of e * alpha.
 // LEARN: also write the per-tile sums of online learning (lane_weight_learn_sums);
 // d = x_prev - mu is 0 on tick 0, which propagates nothing.
-template <int K, bool FISH, bool LEARN>
+// TSTATE: Student-t state noise, eps = z / sqrt(chi2 / nu) with chi2 from
+// nu_int further normals per particle (ptx_assembly/bpf_kernels_full.ptx:
+// 393-406, chi2 as gpu_bpf/gpu_bpf_ptx_full.cu:50-62).
+template <int K, bool FISH, bool LEARN, bool TSTATE>
 __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
-    static_assert(!(FISH && LEARN), "one extra statistic per launch");
+    static_assert(!(FISH && LEARN) && !(FISH && TSTATE), "one extra statistic per launch");
     constexpr int VEC = K < 4 ? K : 4;
     constexpr int NV = K / VEC;
     constexpr int TILE = kBlock * K;
@@ -421,6 +424,23 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
                 for (int v = 0; v < NV; ++v) {
                     long long off = base + (long long)(v * kBlock + tid) * VEC;
                     store_vec<VEC>(a.alpha[cur][q] + off, &al[q][v * VEC]);
+                }
+        }
+        if constexpr (TSTATE) {
+            // chi2 = sum of nu_int squared normals, in order (FP32)
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) {
+                    const int k = v * VEC + c;
+                    const long long i = base + (long long)(v * kBlock + tid) * VEC + c;
+                    float chi2 = 0.0f;
+                    for (int q = 0; q < a.nu_int; ++q) {
+                        const uint64_t j = (uint64_t)i * (uint64_t)a.nu_int + (uint64_t)q;
+                        const float g = icdf(pcg_uniform(pcg_output(a.chi_a + (2u * j + 1u) * a.chi_g)));
+                        chi2 = __builtin_fmaf(g, g, chi2);
+                    }
+                    z[k] = z[k] * (1.0f / __builtin_sqrtf(chi2 / a.nu_state));
                 }
         }
         // sigma_z * k_band: one scale for the tile unless a band boundary
@@ -1254,8 +1274,59 @@ __global__ __launch_bounds__(kBlock) void k_fisher_score(FisherArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Silverman jitter after a resample (gpu_bpf/gpu_bpf_ptx_full.cu:708-739,
+// ptx_assembly/bpf_kernels_full.ptx:852-1113): the spread of the resampled
+// particles about the tick's mean (FP64 per-block sums, added on the host in
+// block order), then x += bw * z with z from one more draw of the particle's
+// stream, clamped to [mu - 8, mu + 8].
+// ---------------------------------------------------------------------------
+constexpr int kJitterGrid = 1024;
+
+__global__ __launch_bounds__(kBlock) void k_jitter_var(JitterArgs a) {
+    constexpr int NW = kBlock / kWave;
+    __shared__ double s_r[NW];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    double acc = 0.0;
+    for (long long i = (long long)blockIdx.x * kBlock + tid; i < a.n; i += (long long)gridDim.x * kBlock) {
+        const double d = (double)(a.x[i] - a.mean);
+        acc = fma(d, d, acc);
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) s_r[wid] = acc;
+    __syncthreads();
+    if (tid == 0) {
+        double r = s_r[0];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) r += s_r[w];
+        a.part[blockIdx.x] = r;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_jitter(JitterArgs a) {
+    const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.n) return;
+    const uint64_t st = a.pcg_a + (uint64_t)(2 * i + 1) * a.pcg_g;
+    const float z = icdf(pcg_uniform(pcg_output(st)));
+    const float h = __builtin_fmaf(a.bw, z, a.x[i]);
+    a.x[i] = fminf(fmaxf(h, a.lo), a.hi);
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
+
+hipError_t launch_jitter_var(const JitterArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_jitter_var, dim3(kJitterGrid), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+int jitter_var_blocks() { return kJitterGrid; }
+
+hipError_t launch_jitter(const JitterArgs& a, hipStream_t st) {
+    const int grid = (int)(((long long)a.n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_jitter, dim3(grid), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
 
 
 hipError_t launch_fisher_score(const FisherArgs& a, hipStream_t st) {
@@ -1312,10 +1383,10 @@ constexpr int resample_block(int tile) {
 }
 
 // ev (nullable): start/stop event pairs of propagate, finalise, resample.
-template <int K, bool FISH, bool LEARN>
+template <int K, bool FISH, bool LEARN, bool TSTATE>
 static hipError_t launch_tick_k(const PropArgs& p, const FinArgs& f, const ResArgs& r,
                                 hipStream_t st, hipEvent_t* ev) {
-    launch_timed(k_propagate<K, FISH, LEARN>, dim3(p.part.n_tiles), dim3(kBlock), st, ev, p);
+    launch_timed(k_propagate<K, FISH, LEARN, TSTATE>, dim3(p.part.n_tiles), dim3(kBlock), st, ev, p);
     launch_timed(k_finalize, dim3(f.part.n_groups), dim3(kFinBlock), st, ev ? ev + 2 : nullptr, f);
     constexpr int TILE = kBlock * K;
     constexpr int RB = resample_block(TILE);
@@ -1344,21 +1415,25 @@ template <int K>
 static hipError_t launch_tick_v(int var, const PropArgs& p, const FinArgs& f, const ResArgs& r,
                                 hipStream_t st, hipEvent_t* ev) {
     switch (var) {
-        case 0: return launch_tick_k<K, false, false>(p, f, r, st, ev);
-        case 1: return launch_tick_k<K, true, false>(p, f, r, st, ev);
-        default: return launch_tick_k<K, false, true>(p, f, r, st, ev);
+        case 0: return launch_tick_k<K, false, false, false>(p, f, r, st, ev);
+        case 1: return launch_tick_k<K, true, false, false>(p, f, r, st, ev);
+        case 2: return launch_tick_k<K, false, true, false>(p, f, r, st, ev);
+        case 3: return launch_tick_k<K, false, false, true>(p, f, r, st, ev);
+        default: return launch_tick_k<K, false, true, true>(p, f, r, st, ev);
     }
 }
 
 // The Fisher-identity variants are launched when PropArgs.alpha is set (the
 // resample pass must then carry the same buffers), the learning variants
-// when PropArgs.lsum is set; not both.
+// when PropArgs.lsum is set, the Student-t state-noise variants when
+// PropArgs.nu_int > 0; Fisher with neither of the others.
 hipError_t launch_tick(int K, const PropArgs& p, const FinArgs& f, const ResArgs& r,
                        hipStream_t st, hipEvent_t* ev) {
     const bool fish = p.alpha[0][0] != nullptr;
     const bool learn = p.lsum != nullptr;
-    if (fish != (r.alpha[0][0] != nullptr) || (fish && learn)) return hipErrorInvalidValue;
-    const int var = fish ? 1 : (learn ? 2 : 0);
+    const bool tstate = p.nu_int > 0;
+    if (fish != (r.alpha[0][0] != nullptr) || (fish && (learn || tstate))) return hipErrorInvalidValue;
+    const int var = fish ? 1 : (tstate ? (learn ? 4 : 3) : (learn ? 2 : 0));
     switch (K) {
         case 1: return launch_tick_v<1>(var, p, f, r, st, ev);
         case 2: return launch_tick_v<2>(var, p, f, r, st, ev);

@@ -100,6 +100,13 @@ struct PropArgs {
     int n;
     int do_prop;          // 0 on tick 0 (ptx_assembly/bpf_kernels_full.ptx:294)
     Bands bands;          // sigma_z scales of the tick's transition
+    // Student-t state noise (k_propagate's TSTATE variant): eps = z / sqrt(chi2
+    // / nu_state), chi2 the sum of nu_int squared normals of particle i, from
+    // PCG32 streams j = i * nu_int + k of seed * 31337 (state = chi_a + inc_j *
+    // chi_g, inc_j = 2j + 1) in place of the reference's cuRAND normals
+    float nu_state;
+    int nu_int;
+    uint64_t chi_a, chi_g;
     int store_l;          // keep log-weights for a possible carry
     float* x_save;        // optional copy of the pre-propagation states
     // Fisher-identity score (null: off): per particle a 3-vector
@@ -113,6 +120,20 @@ struct PropArgs {
     // of the propagated particle and d = x_prev - mu (component q at
     // lsum + q * n_tiles)
     double* lsum;
+};
+
+// Silverman jitter after a resample (gpu_bpf/gpu_bpf_ptx_full.cu:708-739):
+// k_jitter_var writes per-block FP64 sums of (x - mean)^2; k_jitter adds
+// bw * z (z from one more draw of each particle's stream) and clamps to
+// [mu - 8, mu + 8].
+struct JitterArgs {
+    float* x;
+    double* part;         // k_jitter_var: one sum per block
+    float mean;
+    float bw;
+    float lo, hi;
+    uint64_t pcg_a, pcg_g;
+    int n;
 };
 
 struct FinArgs {

@@ -87,6 +87,7 @@ def load(path=LIB_PATH):
         "gpu_bpf_set_bands": (None, [I, I, P, P]),
         "gpu_bpf_set_adaptive_bands": (None, [I, P, P, I, P, P, I, P, P, I, F, F]),
         "gpu_bpf_get_band_regime": (I, [P]),
+        "gpu_bpf_set_silverman_shrink": (None, [P, F]),
         "gpu_bpf_last_error": (ctypes.c_char_p, []),
         "gpu_bpf_run": (I, [P, P, I, P, P]),
         "gpu_bpf_run_device": (I, [P, P, I]),
@@ -193,6 +194,9 @@ class GpuBpf:
 
     def enable_rho_learning(self, enable=True):
         self.L.gpu_bpf_enable_rho_learning(self.h, int(enable))
+
+    def set_silverman_shrink(self, shrink):
+        self.L.gpu_bpf_set_silverman_shrink(self.h, float(shrink))
 
     def band_regime(self):
         return self._check(self.L.gpu_bpf_get_band_regime(self.h))

@@ -56,8 +56,14 @@ typedef struct GpuBpfState GpuBpfState;
 
 /* gpu_bpf_full.cuh:144 / gpu_bpf_ptx_full.cu:340.  Seeds per-particle PCG32
  * streams and draws x_0 from N(mu, sigma_z^2 / max(1 - rho^2, 1e-6)).
- * nu_obs > 0 selects Student-t observation noise.  nu_state > 0 (Student-t
- * state noise) is not supported: returns NULL.  Default ESS threshold 0.5. */
+ * nu_obs > 0 selects Student-t observation noise.  nu_state > 0 selects
+ * Student-t state noise eps = z / sqrt(chi2 / nu_state), chi2 the sum of
+ * round(nu_state) squared normals per particle and tick
+ * (gpu_bpf_ptx_full.cu:50-62,392-404; ptx:393-406).  The reference draws
+ * those normals with cuRAND; here they come from PCG32 streams seeded with
+ * seed * 31337 (stream i * round(nu_state) + k for normal k of particle i,
+ * draw t - 1 at tick t), so runs repeat exactly.  round(nu_state) must be
+ * 1..64, else NULL.  Default ESS threshold 0.5. */
 GpuBpfState* gpu_bpf_create(int n_particles, float rho, float sigma_z, float mu,
                             float nu_state, float nu_obs, int seed);
 
@@ -139,6 +145,13 @@ void gpu_bpf_set_adaptive_bands(int n_particles,
 
 /* Thread-local text of the last error ("" if none). */
 const char* gpu_bpf_last_error(void);
+
+/* Replaces writing GpuBpfState.silverman_shrink (gpu_bpf_full.cuh:109,
+ * tests/test_cond_resample.cu:77).  > 0 turns on the Silverman jitter after
+ * each resample (gpu_bpf_ptx_full.cu:708-739): bandwidth shrink * sigma_hat *
+ * 1.05922 * N^-0.2, one more draw of every particle's stream, clamp to
+ * [mu - 8, mu + 8]; like the reference, a jittered tick waits on the host. */
+void gpu_bpf_set_silverman_shrink(GpuBpfState* state, float shrink);
 
 /* The adaptive bands' current regime of this filter: 0 calm, 1 alert,
  * 2 panic (0 without adaptive bands), or -1 with gpu_bpf_last_error(). */

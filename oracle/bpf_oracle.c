@@ -487,13 +487,20 @@ static inline float band_scale(const BandCfg* c, int i) {
 static int bpf_run_impl(const OrcParams* p, int mode, const float* y, const float* mu_t,
                         int n_ticks, OrcTick* out, float* x_out, double* x64_out,
                         int* anc_out, int n_threads, const OrcLearn* lp, double* lsum_out,
-                        float* par_out, const OrcBands* bd, int* regime_out);
+                        float* par_out, const OrcBands* bd, int* regime_out, float nu_state,
+                        float shrink);
 
 int orc_bpf_run(const OrcParams* p, int mode, const float* y, const float* mu_t,
                 int n_ticks, OrcTick* out, float* x_out, double* x64_out,
                 int* anc_out, int n_threads) {
     return bpf_run_impl(p, mode, y, mu_t, n_ticks, out, x_out, x64_out, anc_out, n_threads,
-                        NULL, NULL, NULL, NULL, NULL);
+                        NULL, NULL, NULL, NULL, NULL, 0.0f, 0.0f);
+}
+
+int orc_bpf_run_ext(const OrcParams* p, float nu_state, float silverman_shrink, const float* y,
+                    int n_ticks, OrcTick* out, float* x_out, int n_threads) {
+    return bpf_run_impl(p, ORC_MODE_F32, y, NULL, n_ticks, out, x_out, NULL, NULL, n_threads,
+                        NULL, NULL, NULL, NULL, NULL, nu_state, silverman_shrink);
 }
 
 int orc_bpf_run_bands(const OrcParams* p, const OrcBands* bd, const float* y, int n_ticks,
@@ -502,7 +509,7 @@ int orc_bpf_run_bands(const OrcParams* p, const OrcBands* bd, const float* y, in
     for (int c = 0; c < 3; c++)
         if (bd->n_bands[c] < 1 || bd->n_bands[c] > 4) return -1;
     return bpf_run_impl(p, ORC_MODE_F32, y, NULL, n_ticks, out, x_out, NULL, NULL, n_threads,
-                        NULL, NULL, NULL, bd, regime_out);
+                        NULL, NULL, NULL, bd, regime_out, 0.0f, 0.0f);
 }
 
 int orc_bpf_run_learn(const OrcParams* p, const OrcLearn* lp, const float* y, int n_ticks,
@@ -510,15 +517,24 @@ int orc_bpf_run_learn(const OrcParams* p, const OrcLearn* lp, const float* y, in
                       int n_threads) {
     if (!lp) return -1;
     return bpf_run_impl(p, ORC_MODE_F32, y, NULL, n_ticks, out, x_out, NULL, NULL, n_threads,
-                        lp, lsum_out, par_out, NULL, NULL);
+                        lp, lsum_out, par_out, NULL, NULL, 0.0f, 0.0f);
 }
 
 static int bpf_run_impl(const OrcParams* p, int mode, const float* y, const float* mu_t,
                         int n_ticks, OrcTick* out, float* x_out, double* x64_out,
                         int* anc_out, int n_threads, const OrcLearn* lp, double* lsum_out,
-                        float* par_out, const OrcBands* bd, int* regime_out) {
+                        float* par_out, const OrcBands* bd, int* regime_out, float nu_state,
+                        float shrink) {
     const int n = p->n_particles;
     if (n <= 0 || n_ticks < 0 || p->tile <= 0) return -1;
+    /* Student-t state noise: round(nu_state) normals per particle and tick
+     * from PCG32 streams of seed * 31337 (the reference's cuRAND generator
+     * seed, gpu_bpf_ptx_full.cu:399-404), stream i * nu_int + k, draw t - 1 */
+    const int nu_int = nu_state > 0.0f ? (int)(nu_state + 0.5f) : 0;
+    if (nu_state > 0.0f && (nu_int < 1 || nu_int > 64)) return -1;
+    if ((nu_int > 0 || shrink > 0.0f) && mode != ORC_MODE_F32) return -1;
+    const uint64_t chi_seed = (uint64_t)(int64_t)p->seed * 31337ULL;
+    uint32_t dnext = 1;     /* next unused draw of the particle streams (draw 0: x_0) */
     if (lp && (mode != ORC_MODE_F32 || mu_t)) return -1;
     BandCfg bcfg[3];
     int regime = 0;
@@ -593,7 +609,14 @@ static int bpf_run_impl(const OrcParams* p, int mode, const float* y, const floa
         const float mu = mu_t ? mu_t[t] : mu_cur;
         const float yt = y[t];
         const float ysq = yt * yt;
-        const PcgJump jt = pcg_jump(p->seed, (uint32_t)t);
+        /* tick t >= 1 takes the next draw; a Silverman jitter takes one more */
+        const PcgJump jt = pcg_jump(p->seed, t > 0 ? dnext : 0u);
+        if (t > 0) dnext++;
+        PcgJump jc = {0, 0};
+        if (nu_int > 0 && t > 0) {
+            jc = pcg_jump(1, (uint32_t)(t - 1));
+            jc.a *= chi_seed;
+        }
         /* the tick's band configuration; the regime switches on the previous
          * tick's surprise before propagating (gpu_bpf_ptx_full.cu:312-334,585-588) */
         const BandCfg* act = NULL;
@@ -627,6 +650,16 @@ static int bpf_run_impl(const OrcParams* p, int mode, const float* y, const floa
                 if (xp) xp[i] = xi;
                 if (t > 0) {
                     float z = normal_j(jt, (uint32_t)i);
+                    if (nu_int > 0) {
+                        /* ptx_assembly/bpf_kernels_full.ptx:393-406 with IEEE ops */
+                        float chi2 = 0.0f;
+                        for (int q = 0; q < nu_int; q++) {
+                            const uint64_t j = (uint64_t)i * (uint64_t)nu_int + (uint64_t)q;
+                            const float g = orc_icdf_f32(orc_uniform(orc_pcg_output(jc.a + (2u * j + 1u) * jc.g)));
+                            chi2 = fmaf(g, g, chi2);
+                        }
+                        z = z * (1.0f / sqrtf(chi2 / nu_state));
+                    }
                     float d = xi - mu;
                     d = (g_fault == 2 ? -rho : rho) * d;
                     xi = mu + d;
@@ -774,6 +807,24 @@ static int bpf_run_impl(const OrcParams* p, int mode, const float* y, const floa
 #pragma omp parallel for num_threads(n_threads) schedule(static)
                 for (int i = 0; i < n; i++) xt[i] = x[anc[i]];
                 float* sw = x; x = xt; xt = sw;
+            }
+            if (shrink > 0.0f) {
+                /* Silverman jitter, gpu_bpf_ptx_full.cu:708-739 and
+                 * ptx_assembly/bpf_kernels_full.ptx:852-1113 */
+                double v = 0.0;
+                for (int i = 0; i < n; i++) {
+                    const double d = (double)(x[i] - r.h_mean);
+                    v = fma(d, d, v);
+                }
+                const float var_sum = (float)v;
+                const float sigma_hat = sqrtf(var_sum / (float)n);
+                const float bw = shrink * sigma_hat * 1.05922f * powf((float)n, -0.2f);
+                const PcgJump jj = pcg_jump(p->seed, dnext);
+                dnext++;
+                const float lo = mu - 8.0f, hi = mu + 8.0f;
+#pragma omp parallel for num_threads(n_threads) schedule(static)
+                for (int i = 0; i < n; i++)
+                    x[i] = fminf(fmaxf(fmaf(bw, normal_j(jj, (uint32_t)i), x[i]), lo), hi);
             }
             if (anc_out) memcpy(anc_out, anc, sizeof(int) * (size_t)n);
             count++;

@@ -149,6 +149,14 @@ typedef struct {
 int orc_bpf_run_bands(const OrcParams* p, const OrcBands* bd, const float* y, int n_ticks,
                       OrcTick* out, float* x_out, int* regime_out, int n_threads);
 
+/* orc_bpf_run (FP32 restatement, mode 0) with Student-t state noise
+ * (nu_state > 0; chi2 normals from PCG32 streams of seed * 31337, stream
+ * i * round(nu_state) + k, draw t - 1 at tick t) and / or the Silverman
+ * jitter after every resample (silverman_shrink > 0; one more draw of every
+ * particle stream per jitter). */
+int orc_bpf_run_ext(const OrcParams* p, float nu_state, float silverman_shrink, const float* y,
+                    int n_ticks, OrcTick* out, float* x_out, int n_threads);
+
 /* Systematic resampling given a normalised FP64 CDF: lower_bound of the float
  * targets u + i/n (wrapped at 1) clamped to n-1. */
 void orc_resample_ancestors(const double* cdf, int n, float u, int* anc);

@@ -125,6 +125,8 @@ def lib():
         L.orc_bpf_run_bands.restype = i32
         L.orc_bpf_run_bands.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcBands), P, i32,
                                         P, P, P, i32]
+        L.orc_bpf_run_ext.restype = i32
+        L.orc_bpf_run_ext.argtypes = [ctypes.POINTER(OrcParams), f32, f32, P, i32, P, P, i32]
         L.orc_learn_update.restype = None
         L.orc_learn_update.argtypes = [ctypes.POINTER(OrcLearn), P, i32, ctypes.POINTER(i32),
                                        ctypes.POINTER(f32), ctypes.POINTER(f32)]
@@ -282,6 +284,22 @@ def run_filter_bands(y, n_particles, configs, adaptive=False, band_n=None, rho=0
     if cnt < 0:
         raise ValueError("oracle rejected the arguments")
     return out, reg, x
+
+
+def run_filter_ext(y, n_particles, rho=0.97, sigma_z=0.15, mu=-1.0, nu_obs=0.0, nu_state=0.0,
+                   silverman=0.0, ess_threshold=0.0, seed=42, tile=256, want_x=False, threads=1):
+    """FP32 filter with Student-t state noise and / or Silverman jitter.
+    Returns (ticks, resample count, x)."""
+    y = np.ascontiguousarray(y, np.float32)
+    T = len(y)
+    p = OrcParams(n_particles, rho, sigma_z, mu, nu_obs, ess_threshold, seed, tile)
+    out = np.zeros(T, TICK_DTYPE)
+    x = np.empty(n_particles, np.float32) if want_x else None
+    cnt = lib().orc_bpf_run_ext(ctypes.byref(p), float(nu_state), float(silverman), _ptr(y), T,
+                                _ptr(out), _ptr(x), int(threads))
+    if cnt < 0:
+        raise ValueError("oracle rejected the arguments")
+    return out, cnt, x
 
 
 def learn_update(sums, k, rm_step, mu, rho, learn_rho=True, c=0.1, t0=10.0, gamma=0.667,

@@ -217,7 +217,7 @@ def test_run_rmse(ydata):
 
 def test_errors_are_reported():
     with pytest.raises(gpu_bpf.GpuBpfError, match="nu_state"):
-        gpu_bpf.GpuBpf(1000, RHO, SIG, MU, nu_state=5.0)
+        gpu_bpf.GpuBpf(1000, RHO, SIG, MU, nu_state=0.3)   # rounds to no chi2 normals
     with pytest.raises(gpu_bpf.GpuBpfError, match="positive"):
         gpu_bpf.GpuBpf(0, RHO, SIG, MU)
 
