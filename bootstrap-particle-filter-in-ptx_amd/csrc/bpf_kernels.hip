@@ -331,9 +331,20 @@ __device__ inline void lane_weight_learn_sums(const float (&x)[K], const float (
 // TSTATE: Student-t state noise, eps = z / sqrt(chi2 / nu) with chi2 from
 // nu_int further normals per particle (ptx_assembly/bpf_kernels_full.ptx:
 // 393-406, chi2 as gpu_bpf/gpu_bpf_ptx_full.cu:50-62).
-template <int K, bool FISH, bool LEARN, bool TSTATE>
+// APF1 / APF2: the two stages of an auxiliary-filter tick
+// (gpu_bpf/gpu_bpf_ptx_full.cu:990-1047,1102-1170).  APF1 propagates nothing
+// and weights each particle at its predictive mean mp = mu + phi (x - mu);
+// APF2 propagates and weights by lw(x') - lw(mp), mp from the (resampled)
+// pre-propagation state.
+enum PropMode { kFish = 1, kLearn = 2, kTState = 4, kApf1 = 8, kApf2 = 16 };
+
+template <int K, int MODE>
 __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
+    constexpr bool FISH = MODE & kFish, LEARN = MODE & kLearn, TSTATE = MODE & kTState;
+    constexpr bool APF1 = MODE & kApf1, APF2 = MODE & kApf2;
     static_assert(!(FISH && LEARN) && !(FISH && TSTATE), "one extra statistic per launch");
+    static_assert(!((APF1 || APF2) && (FISH || LEARN)) && !(APF1 && (APF2 || TSTATE)),
+                  "auxiliary-filter stages take no extra statistic");
     constexpr int VEC = K < 4 ? K : 4;
     constexpr int NV = K / VEC;
     constexpr int TILE = kBlock * K;
@@ -406,8 +417,24 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
 #pragma unroll
         for (int k = 0; k < K; ++k) dp[k] = a.do_prop ? x[k] - a.mu : 0.0f;
     }
+    // auxiliary filter: predictive means mp = mu + phi (x - mu), the
+    // transition without its noise term
+    float mp[(APF1 || APF2) ? K : 1];
+    if constexpr (APF1 || APF2) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            float d = x[k] - a.mu;
+            d = a.phi * d;
+            mp[k] = a.mu + d;
+        }
+    }
+    if constexpr (APF2) {
+        // second-stage weight lw(x') - lw(mp): start l at -lw(mp)
+#pragma unroll
+        for (int k = 0; k < K; ++k) l[k] = -log_weight(mp[k], obs);
+    }
 
-    if (a.do_prop) {
+    if (!APF1 && a.do_prop) {
         float z[K];
         lane_normals<kBlock, K>(a.pcg_a, a.pcg_g, base, tid, s_q, s_cnt, z);
         if constexpr (FISH) {
@@ -481,7 +508,11 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
     const bool full = rem == TILE;
     float mloc_l;
     float g[LEARN ? K : 1];
-    if constexpr (LEARN) {
+    if constexpr (APF1) {
+        // first stage: log-weights at the predictive means
+        if (obs.nu > 0.0f) mloc_l = lane_log_weights<K, true, true>(mp, l, obs, moff, tid, rem);
+        else mloc_l = lane_log_weights<K, false, true>(mp, l, obs, moff, tid, rem);
+    } else if constexpr (LEARN) {
         if (obs.nu > 0.0f) mloc_l = lane_log_weights_grad<K, true, true>(x, l, g, obs, moff, tid, rem);
         else mloc_l = lane_log_weights_grad<K, false, true>(x, l, g, obs, moff, tid, rem);
     } else {
@@ -1383,10 +1414,11 @@ constexpr int resample_block(int tile) {
 }
 
 // ev (nullable): start/stop event pairs of propagate, finalise, resample.
-template <int K, bool FISH, bool LEARN, bool TSTATE>
+template <int K, int MODE>
 static hipError_t launch_tick_k(const PropArgs& p, const FinArgs& f, const ResArgs& r,
                                 hipStream_t st, hipEvent_t* ev) {
-    launch_timed(k_propagate<K, FISH, LEARN, TSTATE>, dim3(p.part.n_tiles), dim3(kBlock), st, ev, p);
+    constexpr bool FISH = MODE & kFish;
+    launch_timed(k_propagate<K, MODE>, dim3(p.part.n_tiles), dim3(kBlock), st, ev, p);
     launch_timed(k_finalize, dim3(f.part.n_groups), dim3(kFinBlock), st, ev ? ev + 2 : nullptr, f);
     constexpr int TILE = kBlock * K;
     constexpr int RB = resample_block(TILE);
@@ -1415,11 +1447,15 @@ template <int K>
 static hipError_t launch_tick_v(int var, const PropArgs& p, const FinArgs& f, const ResArgs& r,
                                 hipStream_t st, hipEvent_t* ev) {
     switch (var) {
-        case 0: return launch_tick_k<K, false, false, false>(p, f, r, st, ev);
-        case 1: return launch_tick_k<K, true, false, false>(p, f, r, st, ev);
-        case 2: return launch_tick_k<K, false, true, false>(p, f, r, st, ev);
-        case 3: return launch_tick_k<K, false, false, true>(p, f, r, st, ev);
-        default: return launch_tick_k<K, false, true, true>(p, f, r, st, ev);
+        case 0: return launch_tick_k<K, 0>(p, f, r, st, ev);
+        case kFish: return launch_tick_k<K, kFish>(p, f, r, st, ev);
+        case kLearn: return launch_tick_k<K, kLearn>(p, f, r, st, ev);
+        case kTState: return launch_tick_k<K, kTState>(p, f, r, st, ev);
+        case kLearn | kTState: return launch_tick_k<K, kLearn | kTState>(p, f, r, st, ev);
+        case kApf1: return launch_tick_k<K, kApf1>(p, f, r, st, ev);
+        case kApf2: return launch_tick_k<K, kApf2>(p, f, r, st, ev);
+        case kApf2 | kTState: return launch_tick_k<K, kApf2 | kTState>(p, f, r, st, ev);
+        default: return hipErrorInvalidValue;
     }
 }
 
@@ -1432,8 +1468,9 @@ hipError_t launch_tick(int K, const PropArgs& p, const FinArgs& f, const ResArgs
     const bool fish = p.alpha[0][0] != nullptr;
     const bool learn = p.lsum != nullptr;
     const bool tstate = p.nu_int > 0;
-    if (fish != (r.alpha[0][0] != nullptr) || (fish && (learn || tstate))) return hipErrorInvalidValue;
-    const int var = fish ? 1 : (tstate ? (learn ? 4 : 3) : (learn ? 2 : 0));
+    if (fish != (r.alpha[0][0] != nullptr)) return hipErrorInvalidValue;
+    const int var = (fish ? kFish : 0) | (learn ? kLearn : 0) | (tstate ? kTState : 0) |
+                    (p.apf_stage == 1 ? kApf1 : 0) | (p.apf_stage == 2 ? kApf2 : 0);
     switch (K) {
         case 1: return launch_tick_v<1>(var, p, f, r, st, ev);
         case 2: return launch_tick_v<2>(var, p, f, r, st, ev);

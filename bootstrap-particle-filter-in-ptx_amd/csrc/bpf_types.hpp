@@ -107,6 +107,7 @@ struct PropArgs {
     float nu_state;
     int nu_int;
     uint64_t chi_a, chi_g;
+    int apf_stage;        // 0 bootstrap; 1 / 2: auxiliary-filter stage (k_propagate APF1 / APF2)
     int store_l;          // keep log-weights for a possible carry
     float* x_save;        // optional copy of the pre-propagation states
     // Fisher-identity score (null: off): per particle a 3-vector

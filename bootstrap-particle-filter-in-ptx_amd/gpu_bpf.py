@@ -51,7 +51,7 @@ def header_symbols(path=HEADER_PATH):
     """Function names declared in a boundary header (default include/gpu_bpf.h)."""
     text = open(path).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b((?:gpu_bpf|gpu_smc2)_[a-z_0-9]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b((?:gpu_bpf|gpu_smc2|gpu_apf|gpu_imm)_[a-z_0-9]+)\s*\(", text)))
 
 
 def load(path=LIB_PATH):
@@ -88,6 +88,13 @@ def load(path=LIB_PATH):
         "gpu_bpf_set_adaptive_bands": (None, [I, P, P, I, P, P, I, P, P, I, F, F]),
         "gpu_bpf_get_band_regime": (I, [P]),
         "gpu_bpf_set_silverman_shrink": (None, [P, F]),
+        "gpu_apf_create": (P, [I, F, F, F, F, F, I]),
+        "gpu_apf_destroy": (None, [P]),
+        "gpu_apf_step": (BpfResult, [P, F]),
+        "gpu_apf_run_rmse": (D, [P, P, I, I, F, F, F, F, F, I]),
+        "gpu_apf_run": (I, [P, P, I, P, P]),
+        "gpu_apf_copy_particles": (I, [P, P]),
+        "gpu_apf_get_layout": (I, [P, P, P]),
         "gpu_bpf_last_error": (ctypes.c_char_p, []),
         "gpu_bpf_run": (I, [P, P, I, P, P]),
         "gpu_bpf_run_device": (I, [P, P, I]),
@@ -296,6 +303,58 @@ class GpuBpf:
         cnt = np.zeros(3, np.int32)
         self._check(self.L.gpu_bpf_kernel_times(self.h, _ptr(ms), _ptr(cnt), 1 if reset else 0))
         return ms, cnt
+
+
+class GpuApf:
+    """Auxiliary particle filter (gpu_apf_create ... gpu_apf_destroy)."""
+
+    def __init__(self, n_particles, rho, sigma_z, mu, nu_state=0.0, nu_obs=0.0, seed=42):
+        self.L = load()
+        self.n = int(n_particles)
+        self.h = self.L.gpu_apf_create(self.n, rho, sigma_z, mu, nu_state, nu_obs, int(seed))
+        if not self.h:
+            raise GpuBpfError(last_error())
+
+    def close(self):
+        if self.h:
+            self.L.gpu_apf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc < 0:
+            raise GpuBpfError(last_error())
+        return rc
+
+    def step(self, y):
+        r = self.L.gpu_apf_step(self.h, float(y))
+        return r.h_mean, r.log_lik
+
+    def run(self, y):
+        y = np.ascontiguousarray(y, np.float32)
+        res = np.zeros(len(y), RESULT_DTYPE)
+        st = np.zeros(len(y), STATS_DTYPE)
+        self._check(self.L.gpu_apf_run(self.h, _ptr(y), len(y), _ptr(res), _ptr(st)))
+        return res, st
+
+    def particles(self):
+        x = np.empty(self.n, np.float32)
+        self._check(self.L.gpu_apf_copy_particles(self.h, _ptr(x)))
+        return x
+
+    def layout(self):
+        t, g = ctypes.c_int(), ctypes.c_int()
+        self._check(self.L.gpu_apf_get_layout(self.h, ctypes.byref(t), ctypes.byref(g)))
+        return t.value, g.value
+
+
+def run_rmse_apf(returns, true_h, n_particles, rho, sigma_z, mu, nu_state=0.0, nu_obs=0.0, seed=42):
+    r = np.ascontiguousarray(returns, np.float64)
+    h = np.ascontiguousarray(true_h, np.float64)
+    return load().gpu_apf_run_rmse(_ptr(r), _ptr(h), len(r), int(n_particles), rho, sigma_z, mu,
+                                   nu_state, nu_obs, int(seed))
 
 
 def set_bands(n_particles, fracs, scales):

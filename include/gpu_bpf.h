@@ -245,6 +245,43 @@ int gpu_bpf_kernel_tick_times(GpuBpfState* state, float* ms, int n_ticks);
  * of the weight and tail paths).  Returns 0, or -1 with gpu_bpf_last_error(). */
 int gpu_bpf_eval_device_math(int fn, const float* in, float* out, int n);
 
+/* ---- auxiliary particle filter (gpu_bpf_full.cuh:303-361) ----------------- */
+
+/* gpu_bpf_full.cuh:314-337 (opaque here). */
+typedef struct GpuApfState GpuApfState;
+
+/* gpu_bpf_full.cuh:340 / gpu_bpf_ptx_full.cu:1050.  Pitt & Shephard's
+ * auxiliary filter with the bootstrap filter's streams: x_0 from draw 0,
+ * tick t >= 1 resamples by the first-stage weights g(y_t | mu + phi (x - mu)),
+ * propagates with draw t and weights by g(y_t | x') / g(y_t | mp), then
+ * resamples again; both resampling offsets come from the host stream of
+ * seed * 67890 + 12345 in that order (gpu_bpf_ptx_full.cu:1102-1170).  The
+ * reference draws its noise with cuRAND; here it is PCG32 as in
+ * gpu_bpf_create (Student-t state noise as there).  NULL on bad arguments. */
+GpuApfState* gpu_apf_create(int n_particles, float rho, float sigma_z, float mu,
+                            float nu_state, float nu_obs, int seed);
+/* gpu_bpf_full.cuh:344.  NULL-safe. */
+void gpu_apf_destroy(GpuApfState* state);
+/* gpu_bpf_full.cuh:352.  h_mean = second-stage weighted mean; log_lik =
+ * max + log(sum w / N) of the second-stage weights, as the reference
+ * returns it (gpu_bpf_ptx_full.cu:952-955). */
+BpfResult gpu_apf_step(GpuApfState* state, float y_t);
+/* gpu_bpf_full.cuh:355-358.  RMSE of h_mean against true_h from tick 100. */
+double gpu_apf_run_rmse(const double* returns, const double* true_h, int n_ticks,
+                        int n_particles, float rho, float sigma_z, float mu,
+                        float nu_state, float nu_obs, int seed);
+/* Build-owned: n_ticks ticks without a host wait between them (observations
+ * copied once).  stats (nullable) per tick: the second stage's statistics,
+ * with ll_incr = log mean first-stage weight + log mean second-stage weight
+ * (the auxiliary filter's likelihood increment) and ll_cum its FP64 sum.
+ * Returns 0, or -1 with gpu_bpf_last_error(). */
+int gpu_apf_run(GpuApfState* state, const float* y, int n_ticks, BpfResult* results,
+                BpfTickStats* stats);
+/* Build-owned: particle states after the last tick (n floats), and the tile
+ * layout (as gpu_bpf_copy_particles / gpu_bpf_get_layout). */
+int gpu_apf_copy_particles(GpuApfState* state, float* x);
+int gpu_apf_get_layout(GpuApfState* state, int* tile, int* n_tiles);
+
 #ifdef __cplusplus
 }
 #endif

@@ -845,6 +845,153 @@ static int bpf_run_impl(const OrcParams* p, int mode, const float* y, const floa
 }
 
 /* ========================================================================= */
+/* Auxiliary particle filter: gpu_bpf/gpu_bpf_ptx_full.cu:990-1047 (kernels)  */
+/* and :1102-1170 (tick), restated with this build's streams and arithmetic.  */
+/* ========================================================================= */
+
+/* One resampling of x by the tiled weights w (always): the host stream's
+ * next offset, the normalised 80-bit CDF, lower_bound of u + i/n. */
+static void apf_resample(float* x, float* xt, const double* w, double* cdf, int* anc, int n,
+                         uint64_t* hrng, int n_threads) {
+    uint32_t hr = orc_pcg_host_next(hrng);
+    float u = ((float)(hr >> 9) * (1.0f / 8388608.0f)) / (float)n;
+    build_cdf(w, n, cdf);
+#pragma omp parallel for num_threads(n_threads) schedule(static)
+    for (int i = 0; i < n; i++) {
+        double tt = (double)orc_target(u, i, n);
+        int lo = 0, hi = n - 1;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (cdf[mid] < tt) lo = mid + 1; else hi = mid;
+        }
+        anc[i] = lo;
+    }
+#pragma omp parallel for num_threads(n_threads) schedule(static)
+    for (int i = 0; i < n; i++) xt[i] = x[anc[i]];
+    memcpy(x, xt, sizeof(float) * (size_t)n);
+}
+
+int orc_apf_run(const OrcParams* p, float nu_state, const float* y, int n_ticks, OrcTick* out,
+                float* x_out, int n_threads) {
+    const int n = p->n_particles;
+    if (n <= 0 || n_ticks < 0 || p->tile <= 0) return -1;
+    const int nu_int = nu_state > 0.0f ? (int)(nu_state + 0.5f) : 0;
+    if (nu_state > 0.0f && (nu_int < 1 || nu_int > 64)) return -1;
+#ifdef _OPENMP
+    if (n_threads < 1) n_threads = 1;
+#else
+    (void)n_threads;
+#endif
+    const int tile = p->tile;
+    const int n_tiles = (n + tile - 1) / tile;
+    float* x = (float*)malloc(sizeof(float) * (size_t)n);
+    float* xt = (float*)malloc(sizeof(float) * (size_t)n);
+    float* l = (float*)malloc(sizeof(float) * (size_t)n);
+    double* w = (double*)malloc(sizeof(double) * (size_t)n);
+    double* cdf = (double*)malloc(sizeof(double) * (size_t)n);
+    int* anc = (int*)malloc(sizeof(int) * (size_t)n);
+    float* tmax = (float*)malloc(sizeof(float) * (size_t)n_tiles);
+    double* tfd = (double*)malloc(sizeof(double) * (size_t)n_tiles);
+    if (!x || !xt || !l || !w || !cdf || !anc || !tmax || !tfd) {
+        free(x); free(xt); free(l); free(w); free(cdf); free(anc); free(tmax); free(tfd);
+        return -1;
+    }
+    const float rho = p->rho, sig = p->sigma_z, mu = p->mu, nu_obs = p->nu_obs;
+    const float c_obs = nu_obs > 0.0f ? (float)c_obs_f64((double)nu_obs) : 0.0f;
+    const float nph = (nu_obs + 1.0f) * 0.5f;
+    const uint64_t chi_seed = (uint64_t)(int64_t)p->seed * 31337ULL;
+    {
+        float sd = sqrtf((sig * sig) / fmaxf(1.0f - rho * rho, 1e-6f));
+        PcgJump j0 = pcg_jump(p->seed, 0);
+        for (int i = 0; i < n; i++) x[i] = fmaf(sd, normal_j(j0, (uint32_t)i), mu);
+    }
+    uint64_t hrng = orc_pcg_host_seed(p->seed);
+    const double ll_floor = (double)f32_bits(0x0D6BF94Eu);
+    double ll_cum = 0.0;
+    for (int t = 0; t < n_ticks; t++) {
+        const float ysq = y[t] * y[t];
+        double ll1 = 0.0;
+        if (t > 0) {
+            /* stage 1: weights at the predictive means, then resample */
+#pragma omp parallel for num_threads(n_threads) schedule(static)
+            for (int i = 0; i < n; i++) {
+                float d = x[i] - mu;
+                d = rho * d;
+                l[i] = lw_f32(mu + d, ysq, nu_obs, c_obs, nph);
+            }
+            double M1 = (double)tiled_weights(l, n, tile, tmax, tfd, w, n_threads);
+            long double S1 = 0.0L;
+            for (int i = 0; i < n; i++) S1 += (long double)w[i];
+            ll1 = M1 + log((double)S1 / (double)n);
+            apf_resample(x, xt, w, cdf, anc, n, &hrng, n_threads);
+            /* stage 2: propagate (draw t), weight lw(x') - lw(mp) */
+            const PcgJump jt = pcg_jump(p->seed, (uint32_t)t);
+            PcgJump jc = {0, 0};
+            if (nu_int > 0) {
+                jc = pcg_jump(1, (uint32_t)(t - 1));
+                jc.a *= chi_seed;
+            }
+#pragma omp parallel for num_threads(n_threads) schedule(static)
+            for (int i = 0; i < n; i++) {
+                float d = x[i] - mu;
+                d = rho * d;
+                const float mp = mu + d;
+                float z = normal_j(jt, (uint32_t)i);
+                if (nu_int > 0) {
+                    float chi2 = 0.0f;
+                    for (int q = 0; q < nu_int; q++) {
+                        const uint64_t j = (uint64_t)i * (uint64_t)nu_int + (uint64_t)q;
+                        const float g = orc_icdf_f32(orc_uniform(orc_pcg_output(jc.a + (2u * j + 1u) * jc.g)));
+                        chi2 = fmaf(g, g, chi2);
+                    }
+                    z = z * (1.0f / sqrtf(chi2 / nu_state));
+                }
+                const float xn = fmaf(sig, z, mp);
+                x[i] = xn;
+                l[i] = lw_f32(xn, ysq, nu_obs, c_obs, nph) +
+                       (g_fault == 8 ? 0.0f : -lw_f32(mp, ysq, nu_obs, c_obs, nph));
+            }
+        } else {
+            for (int i = 0; i < n; i++) l[i] = lw_f32(x[i], ysq, nu_obs, c_obs, nph);
+        }
+        const double M = (double)tiled_weights(l, n, tile, tmax, tfd, w, n_threads);
+        /* 80-bit running sums: the sequential order stays far below the
+         * device's hierarchical FP64 error at millions of particles */
+        long double Sl = 0.0L, SXl = 0.0L, SXXl = 0.0L, Ql = 0.0L;
+        for (int i = 0; i < n; i++) {
+            const long double xi = (long double)x[i], wi = (long double)w[i];
+            Sl += wi;
+            SXl += wi * xi;
+            SXXl += wi * xi * xi;
+            Ql += wi * wi;
+        }
+        const double S = (double)Sl, SX = (double)SXl, SXX = (double)SXXl, Q = (double)Ql;
+        OrcTick r;
+        memset(&r, 0, sizeof r);
+        r.tick = t;
+        if (isfinite(S) && S > 0.0 && isfinite(M)) {
+            r.mean = SX / S;
+            r.var = SXX / S - r.mean * r.mean;
+            r.ess = (S * S) / Q;
+            r.ll_ref = M + log(fmax(S / (double)n, ll_floor));
+            r.ll_incr = M + log(S / (double)n) + ll1;
+        } else {
+            r.mean = r.var = r.ess = r.ll_ref = r.ll_incr = NAN;
+        }
+        ll_cum += r.ll_incr;
+        r.ll_cum = ll_cum;
+        r.h_mean = (float)r.mean;
+        r.log_lik = (float)r.ll_ref;
+        r.resampled = 1;
+        if (out) out[t] = r;
+        apf_resample(x, xt, w, cdf, anc, n, &hrng, n_threads);
+    }
+    if (x_out) memcpy(x_out, x, sizeof(float) * (size_t)n);
+    free(x); free(xt); free(l); free(w); free(cdf); free(anc); free(tmax); free(tfd);
+    return n_ticks;
+}
+
+/* ========================================================================= */
 /* Point-mass filter: exact log p(y_{1:T}) of the Gaussian SV model with the  */
 /* reference's tick convention (tick 0 weights the stationary law, ticks >= 1 */
 /* propagate then weight; gpu_bpf/gpu_bpf_ptx_full.cu:591-599).               */

@@ -157,6 +157,13 @@ int orc_bpf_run_bands(const OrcParams* p, const OrcBands* bd, const float* y, in
 int orc_bpf_run_ext(const OrcParams* p, float nu_state, float silverman_shrink, const float* y,
                     int n_ticks, OrcTick* out, float* x_out, int n_threads);
 
+/* Auxiliary particle filter (gpu_bpf/gpu_bpf_ptx_full.cu:990-1170) in FP32
+ * with this build's PCG32 streams: out[t] holds the second stage's statistics,
+ * ll_incr = log mean first-stage weight + log mean second-stage weight, ll_ref
+ * the reference's log_lik (second stage only).  Returns n_ticks. */
+int orc_apf_run(const OrcParams* p, float nu_state, const float* y, int n_ticks, OrcTick* out,
+                float* x_out, int n_threads);
+
 /* Systematic resampling given a normalised FP64 CDF: lower_bound of the float
  * targets u + i/n (wrapped at 1) clamped to n-1. */
 void orc_resample_ancestors(const double* cdf, int n, float u, int* anc);
@@ -176,7 +183,9 @@ int orc_bpf_score(const OrcParams* p, const float* y, int n_ticks, int x0_repara
  * of the log-weight, 2 negate phi in the transition, 3 upper_bound instead
  * of lower_bound in resampling, 4 drop the weight carry, 6 drop the factor
  * 1/2 of eta^2 in the learner's d lw/dx, 7 take d = x' - mu (the propagated
- * state) instead of x_prev - mu in the learner.  Process-global. */
+ * state) instead of x_prev - mu in the learner, 8 drop the first-stage
+ * correction -lw(mp) of the auxiliary filter's second-stage weight.
+ * Process-global. */
 void orc_set_fault(int code);
 
 /* Exact log p(y_{1:T}) of the Gaussian SV model by a point-mass filter. */

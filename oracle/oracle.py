@@ -127,6 +127,8 @@ def lib():
                                         P, P, P, i32]
         L.orc_bpf_run_ext.restype = i32
         L.orc_bpf_run_ext.argtypes = [ctypes.POINTER(OrcParams), f32, f32, P, i32, P, P, i32]
+        L.orc_apf_run.restype = i32
+        L.orc_apf_run.argtypes = [ctypes.POINTER(OrcParams), f32, P, i32, P, P, i32]
         L.orc_learn_update.restype = None
         L.orc_learn_update.argtypes = [ctypes.POINTER(OrcLearn), P, i32, ctypes.POINTER(i32),
                                        ctypes.POINTER(f32), ctypes.POINTER(f32)]
@@ -300,6 +302,20 @@ def run_filter_ext(y, n_particles, rho=0.97, sigma_z=0.15, mu=-1.0, nu_obs=0.0, 
     if cnt < 0:
         raise ValueError("oracle rejected the arguments")
     return out, cnt, x
+
+
+def run_apf(y, n_particles, rho=0.97, sigma_z=0.15, mu=-1.0, nu_obs=0.0, nu_state=0.0, seed=42,
+            tile=256, want_x=False, threads=1):
+    """FP32 auxiliary particle filter; returns (ticks, x)."""
+    y = np.ascontiguousarray(y, np.float32)
+    T = len(y)
+    p = OrcParams(n_particles, rho, sigma_z, mu, nu_obs, 0.0, seed, tile)
+    out = np.zeros(T, TICK_DTYPE)
+    x = np.empty(n_particles, np.float32) if want_x else None
+    if lib().orc_apf_run(ctypes.byref(p), float(nu_state), _ptr(y), T, _ptr(out), _ptr(x),
+                         int(threads)) < 0:
+        raise ValueError("oracle rejected the arguments")
+    return out, x
 
 
 def learn_update(sums, k, rm_step, mu, rho, learn_rho=True, c=0.1, t0=10.0, gamma=0.667,
