@@ -40,6 +40,16 @@ STATS_DTYPE = np.dtype(
 RESULT_DTYPE = np.dtype([("h_mean", "f4"), ("log_lik", "f4")])
 
 
+class ImmModelParams(ctypes.Structure):
+    _fields_ = [("rho", ctypes.c_float), ("sigma_z", ctypes.c_float), ("mu", ctypes.c_float),
+                ("nu_state", ctypes.c_float), ("nu_obs", ctypes.c_float)]
+
+
+class ImmResult(ctypes.Structure):
+    _fields_ = [("h_mean", ctypes.c_float), ("vol", ctypes.c_float), ("log_lik", ctypes.c_float),
+                ("best_model", ctypes.c_int), ("best_prob", ctypes.c_float)]
+
+
 class GpuBpfError(RuntimeError):
     pass
 
@@ -95,6 +105,13 @@ def load(path=LIB_PATH):
         "gpu_apf_run": (I, [P, P, I, P, P]),
         "gpu_apf_copy_particles": (I, [P, P]),
         "gpu_apf_get_layout": (I, [P, P, P]),
+        "gpu_imm_create": (P, [P, I, I, P, I]),
+        "gpu_imm_destroy": (None, [P]),
+        "gpu_imm_step": (ImmResult, [P, F]),
+        "gpu_imm_get_prob": (F, [P, I]),
+        "gpu_imm_get_probs": (None, [P, P]),
+        "gpu_imm_build_grid": (P, [P, I, P, I, P, I, F, F, P]),
+        "gpu_imm_uses_batch": (I, [P]),
         "gpu_bpf_last_error": (ctypes.c_char_p, []),
         "gpu_bpf_run": (I, [P, P, I, P, P]),
         "gpu_bpf_run_device": (I, [P, P, I]),
@@ -348,6 +365,60 @@ class GpuApf:
         t, g = ctypes.c_int(), ctypes.c_int()
         self._check(self.L.gpu_apf_get_layout(self.h, ctypes.byref(t), ctypes.byref(g)))
         return t.value, g.value
+
+
+class GpuImm:
+    """Interacting multiple models (gpu_imm_create ... gpu_imm_destroy).
+    models: sequence of (rho, sigma_z, mu, nu_state, nu_obs)."""
+
+    def __init__(self, models, n_particles, transition=None, seed=42):
+        self.L = load()
+        self.K = len(models)
+        arr = (ImmModelParams * self.K)(*[ImmModelParams(*map(float, m)) for m in models])
+        tm = None if transition is None else np.ascontiguousarray(transition, np.float32)
+        self.h = self.L.gpu_imm_create(arr, self.K, int(n_particles), _ptr(tm), int(seed))
+        if not self.h:
+            raise GpuBpfError(last_error())
+
+    def close(self):
+        if self.h:
+            self.L.gpu_imm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self, y):
+        r = self.L.gpu_imm_step(self.h, float(y))
+        return r.h_mean, r.vol, r.log_lik, r.best_model, r.best_prob
+
+    def probs(self):
+        p = np.zeros(self.K, np.float32)
+        self.L.gpu_imm_get_probs(self.h, _ptr(p))
+        return p
+
+    def prob(self, k):
+        return self.L.gpu_imm_get_prob(self.h, int(k))
+
+    def uses_batch(self):
+        return bool(self.L.gpu_imm_uses_batch(self.h))
+
+
+def imm_build_grid(rhos, sigma_zs, mus, nu_state=0.0, nu_obs=0.0):
+    """gpu_imm_build_grid as a list of (rho, sigma_z, mu, nu_state, nu_obs)."""
+    L = load()
+    r = np.ascontiguousarray(rhos, np.float32)
+    s = np.ascontiguousarray(sigma_zs, np.float32)
+    m = np.ascontiguousarray(mus, np.float32)
+    n = ctypes.c_int(0)
+    ptr = L.gpu_imm_build_grid(_ptr(r), len(r), _ptr(s), len(s), _ptr(m), len(m), float(nu_state),
+                               float(nu_obs), ctypes.byref(n))
+    if not ptr:
+        raise GpuBpfError(last_error())
+    arr = ctypes.cast(ptr, ctypes.POINTER(ImmModelParams * n.value)).contents
+    out = [(a.rho, a.sigma_z, a.mu, a.nu_state, a.nu_obs) for a in arr]
+    ctypes.CDLL(None).free(ctypes.c_void_p(ptr))
+    return out
 
 
 def run_rmse_apf(returns, true_h, n_particles, rho, sigma_z, mu, nu_state=0.0, nu_obs=0.0, seed=42):

@@ -282,6 +282,55 @@ int gpu_apf_run(GpuApfState* state, const float* y, int n_ticks, BpfResult* resu
 int gpu_apf_copy_particles(GpuApfState* state, float* x);
 int gpu_apf_get_layout(GpuApfState* state, int* tile, int* n_tiles);
 
+/* ---- interacting multiple models (gpu_bpf/gpu_bpf.cuh:128-162) ----------- */
+
+#define IMM_MAX_MODELS 64
+
+/* gpu_bpf.cuh:132-136. */
+typedef struct {
+    float rho, sigma_z, mu, nu_state, nu_obs;
+} ImmModelParams;
+
+/* gpu_bpf.cuh:138-142. */
+typedef struct {
+    float h_mean, vol, log_lik;
+    int best_model;
+    float best_prob;
+} ImmResult;
+
+/* gpu_bpf.cuh:144-149 (opaque here). */
+typedef struct GpuImmState GpuImmState;
+
+/* gpu_bpf.cuh:151 / gpu_bpf.cu:961-1005.  K = n_models SV filters of
+ * n_particles_per_model particles, filter k seeded seed + 7919 k, ESS
+ * threshold 0.5; model probabilities start uniform; transition_matrix
+ * (K x K, row j = from model j; NULL: stay 0.95, switch evenly) is taken in
+ * log space floored at 1e-30.  The models step as one batch of filters (one
+ * launch per tick) when they share nu_obs, have no Student-t state noise and
+ * at most GPU_BPF_BATCH_MAX_N particles; else as K filters.  The reference's
+ * filters here are its cuRAND-based CUDA-C ones; these are the PCG32
+ * filters of gpu_bpf_create.  NULL for more than IMM_MAX_MODELS models. */
+GpuImmState* gpu_imm_create(const ImmModelParams* models, int n_models,
+                            int n_particles_per_model, const float* transition_matrix, int seed);
+/* gpu_bpf.cuh:154 / gpu_bpf.cu:1022-1080.  One tick: Markov interaction of
+ * the model probabilities, every model's filter tick, Bayes update with each
+ * filter's BpfResult.log_lik, probability-weighted h_mean (FP64 on the host).
+ * vol = exp(h_mean / 2); log_lik = log sum_k pi_k^- p_k; best model and its
+ * probability. */
+ImmResult gpu_imm_step(GpuImmState* state, float y_t);
+/* gpu_bpf.cuh:155-156 / gpu_bpf.cu:1086-1094. */
+float gpu_imm_get_prob(const GpuImmState* state, int k);
+void gpu_imm_get_probs(const GpuImmState* state, float* probs_out);
+/* gpu_bpf.cuh:157.  NULL-safe. */
+void gpu_imm_destroy(GpuImmState* state);
+/* gpu_bpf.cuh:159-162 / gpu_bpf.cu:1100-1122.  n_rho * n_sigma * n_mu models
+ * (rho outermost); the caller frees the array with free(). */
+ImmModelParams* gpu_imm_build_grid(const float* rhos, int n_rho, const float* sigma_zs, int n_sigma,
+                                   const float* mus, int n_mu, float nu_state, float nu_obs,
+                                   int* out_n_models);
+/* Build-owned: 1 when the models step as one batch of filters. */
+int gpu_imm_uses_batch(const GpuImmState* state);
+
 #ifdef __cplusplus
 }
 #endif
