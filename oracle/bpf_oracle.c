@@ -497,6 +497,16 @@ int orc_bpf_run(const OrcParams* p, int mode, const float* y, const float* mu_t,
                         NULL, NULL, NULL, NULL, NULL, 0.0f, 0.0f);
 }
 
+int orc_bpf_run_all(const OrcParams* p, const OrcLearn* lp, const OrcBands* bd, float nu_state,
+                    float silverman_shrink, const float* y, int n_ticks, OrcTick* out,
+                    float* par_out, int* regime_out, float* x_out, int n_threads) {
+    if (bd)
+        for (int c = 0; c < 3; c++)
+            if (bd->n_bands[c] < 1 || bd->n_bands[c] > 4) return -1;
+    return bpf_run_impl(p, ORC_MODE_F32, y, NULL, n_ticks, out, x_out, NULL, NULL, n_threads,
+                        lp, NULL, par_out, bd, regime_out, nu_state, silverman_shrink);
+}
+
 int orc_bpf_run_ext(const OrcParams* p, float nu_state, float silverman_shrink, const float* y,
                     int n_ticks, OrcTick* out, float* x_out, int n_threads) {
     return bpf_run_impl(p, ORC_MODE_F32, y, NULL, n_ticks, out, x_out, NULL, NULL, n_threads,
@@ -821,7 +831,11 @@ static int bpf_run_impl(const OrcParams* p, int mode, const float* y, const floa
                 const float bw = shrink * sigma_hat * 1.05922f * powf((float)n, -0.2f);
                 const PcgJump jj = pcg_jump(p->seed, dnext);
                 dnext++;
-                const float lo = mu - 8.0f, hi = mu + 8.0f;
+                /* the clamp takes mu after this tick's learner update, which
+                 * the reference applies before resampling (gpu_bpf_ptx_full.cu:
+                 * 660-664 then :708-739) */
+                const float mj = mu_t ? mu : mu_cur;
+                const float lo = mj - 8.0f, hi = mj + 8.0f;
 #pragma omp parallel for num_threads(n_threads) schedule(static)
                 for (int i = 0; i < n; i++)
                     x[i] = fminf(fmaxf(fmaf(bw, normal_j(jj, (uint32_t)i), x[i]), lo), hi);

@@ -164,6 +164,12 @@ int orc_bpf_run_ext(const OrcParams* p, float nu_state, float silverman_shrink, 
 int orc_apf_run(const OrcParams* p, float nu_state, const float* y, int n_ticks, OrcTick* out,
                 float* x_out, int n_threads);
 
+/* Every option at once (each nullable / 0 = off): learner, bands, Student-t
+ * state noise, Silverman jitter. */
+int orc_bpf_run_all(const OrcParams* p, const OrcLearn* lp, const OrcBands* bd, float nu_state,
+                    float silverman_shrink, const float* y, int n_ticks, OrcTick* out,
+                    float* par_out, int* regime_out, float* x_out, int n_threads);
+
 /* Systematic resampling given a normalised FP64 CDF: lower_bound of the float
  * targets u + i/n (wrapped at 1) clamped to n-1. */
 void orc_resample_ancestors(const double* cdf, int n, float u, int* anc);

@@ -129,6 +129,9 @@ def lib():
         L.orc_bpf_run_ext.argtypes = [ctypes.POINTER(OrcParams), f32, f32, P, i32, P, P, i32]
         L.orc_apf_run.restype = i32
         L.orc_apf_run.argtypes = [ctypes.POINTER(OrcParams), f32, P, i32, P, P, i32]
+        L.orc_bpf_run_all.restype = i32
+        L.orc_bpf_run_all.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcLearn),
+                                      ctypes.POINTER(OrcBands), f32, f32, P, i32, P, P, P, P, i32]
         L.orc_learn_update.restype = None
         L.orc_learn_update.argtypes = [ctypes.POINTER(OrcLearn), P, i32, ctypes.POINTER(i32),
                                        ctypes.POINTER(f32), ctypes.POINTER(f32)]
@@ -302,6 +305,49 @@ def run_filter_ext(y, n_particles, rho=0.97, sigma_z=0.15, mu=-1.0, nu_obs=0.0, 
     if cnt < 0:
         raise ValueError("oracle rejected the arguments")
     return out, cnt, x
+
+
+def _bands_struct(configs, adaptive, band_n, thr_alert, thr_panic):
+    b = OrcBands()
+    b.adaptive = 1 if adaptive else 0
+    b.n_particles = int(band_n)
+    cfgs = list(configs) if adaptive else [configs] * 3
+    for c, (fr, sc) in enumerate(cfgs):
+        b.n_bands[c] = len(sc)
+        for q, v in enumerate(fr):
+            b.fracs[c][q] = v
+        for q, v in enumerate(sc):
+            b.scales[c][q] = v
+    b.thr_alert = thr_alert
+    b.thr_panic = thr_panic
+    return b
+
+
+def run_filter_all(y, n_particles, rho=0.97, sigma_z=0.15, mu=-1.0, nu_obs=0.0, ess_threshold=0.5,
+                   seed=42, tile=256, learn=None, bands=None, adaptive=False, thr_alert=2.0,
+                   thr_panic=4.0, nu_state=0.0, silverman=0.0, want_x=False, threads=1):
+    """FP32 filter with any combination of options.  learn: dict(update_K, learn_rho, c, t0,
+    gamma) or None; bands: a (fracs, scales) pair, or three of them with adaptive=True.
+    Returns (ticks, (mu, rho) after each tick [T, 2], regimes [T], x)."""
+    y = np.ascontiguousarray(y, np.float32)
+    T = len(y)
+    p = OrcParams(n_particles, rho, sigma_z, mu, nu_obs, ess_threshold, seed, tile)
+    lp = None
+    if learn is not None:
+        lp = OrcLearn(int(learn.get("update_K", 50)), 1 if learn.get("learn_rho") else 0,
+                      learn.get("c", 0.1), learn.get("t0", 10.0), learn.get("gamma", 0.667), 5.0)
+    bd = None if bands is None else _bands_struct(bands, adaptive, n_particles, thr_alert, thr_panic)
+    out = np.zeros(T, TICK_DTYPE)
+    par = np.zeros((T, 2), np.float32)
+    reg = np.zeros(T, np.int32)
+    x = np.empty(n_particles, np.float32) if want_x else None
+    rc = lib().orc_bpf_run_all(ctypes.byref(p), ctypes.byref(lp) if lp is not None else None,
+                               ctypes.byref(bd) if bd is not None else None, float(nu_state),
+                               float(silverman), _ptr(y), T, _ptr(out), _ptr(par), _ptr(reg), _ptr(x),
+                               int(threads))
+    if rc < 0:
+        raise ValueError("oracle rejected the arguments")
+    return out, par, reg, x
 
 
 def run_apf(y, n_particles, rho=0.97, sigma_z=0.15, mu=-1.0, nu_obs=0.0, nu_state=0.0, seed=42,
