@@ -29,7 +29,8 @@ def _close(a, b, rtol=1e-12):
 
 
 @pytest.mark.parametrize("n, nu_obs, nu_state", [(10000, 0.0, 0.0), (65536, 5.0, 0.0),
-                                                 (2 ** 21 + 77, 0.0, 0.0), (8192, 0.0, 5.0)])
+                                                 (2 ** 21 + 77, 0.0, 0.0), (8192, 0.0, 5.0),
+                                                 (1, 0.0, 0.0), (257, 0.0, 0.0)])
 def test_apf_parity(orc, n, nu_obs, nu_state):
     T = 30
     say("apf", n, nu_obs, nu_state)
@@ -74,3 +75,13 @@ def test_apf_run_rmse(orc):
     e = res["h_mean"][100:].astype(np.float64) - h[100:]
     assert abs(r - np.sqrt(np.mean(e * e))) < 1e-9
     assert 0.05 < r < 0.8
+
+
+def test_apf_errors_and_empty_run():
+    say("apf errors")
+    with pytest.raises(gpu_bpf.GpuBpfError):
+        gpu_bpf.GpuApf(0, 0.97, 0.15, -1.0)
+    a = gpu_bpf.GpuApf(1000, 0.97, 0.15, -1.0)
+    res, st = a.run(np.zeros(0, np.float32))
+    assert len(res) == 0 and len(st) == 0
+    a.close()
