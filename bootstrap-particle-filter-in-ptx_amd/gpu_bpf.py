@@ -61,7 +61,7 @@ def header_symbols(path=HEADER_PATH):
     """Function names declared in a boundary header (default include/gpu_bpf.h)."""
     text = open(path).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b((?:gpu_bpf|gpu_smc2|gpu_apf|gpu_imm)_[a-z_0-9]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b((?:gpu_bpf|gpu_smc2|gpu_apf|gpu_imm|smc2_cuda)_[a-z_0-9]+)\s*\(", text)))
 
 
 def load(path=LIB_PATH):

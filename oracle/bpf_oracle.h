@@ -201,4 +201,29 @@ double orc_grid_loglik(const float* y, int n_ticks, double rho, double sigma_z,
 #ifdef __cplusplus
 }
 #endif
+/* ---- SMC^2 with the RBPF / OCSN inner filter (rbpf_oracle.c) ---------- */
+typedef struct {
+    int n_theta, n_inner, K_rejuv, lag, cap, adaptive;
+    float ess_thr, crho;
+    uint64_t seed;              /* nonzero */
+    float pstd[4];
+    float prior[8];             /* SVPrior field order */
+    float bounds[8];            /* SVBounds field order */
+    float tcurve[3];            /* theta(z): base, scale, rate */
+    float fixed[4];             /* mu_scale, mu_rate, sigma_scale, sigma_rate */
+} OrcRbCfg;
+typedef struct { int n_resamples, n_acc, n_tot, t_cp; } OrcRbCount;
+/* Whole run over T observations (T <= cap).  Outputs (nullable): outer ESS
+ * per tick [T], theta [4][n_theta], log-likelihoods and log-weights
+ * [n_theta], inner ESS [n_theta], inner state [4][n_theta * n_inner],
+ * resample ticks [n_resamples]. */
+int orc_rb_smc2_run(const OrcRbCfg* c, const float* y, int T, float* ess_out, float* theta_out,
+                    double* ll_out, double* lw_out, float* essi_out, float* inner_out,
+                    int* resample_ticks, OrcRbCount* cnt);
+int orc_rb_inner_run(const OrcRbCfg* c, const float* theta, const float* y, int T,
+                     double* ll_incr, double* ess, float* state_out);
+void orc_rb_ocsn(float y, float mp, float vp, float* out3);
+void orc_rb_math(int fn, const float* in, float* out, int n);
+void orc_rb_ocsn_table(float* p, float* m, float* v);
+
 #endif

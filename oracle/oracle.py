@@ -70,6 +70,28 @@ TICK_DTYPE = np.dtype(
 )
 
 
+class OrcRbCfg(ctypes.Structure):
+    _fields_ = [("n_theta", ctypes.c_int), ("n_inner", ctypes.c_int), ("K_rejuv", ctypes.c_int),
+                ("lag", ctypes.c_int), ("cap", ctypes.c_int), ("adaptive", ctypes.c_int),
+                ("ess_thr", ctypes.c_float), ("crho", ctypes.c_float), ("seed", ctypes.c_uint64),
+                ("pstd", ctypes.c_float * 4), ("prior", ctypes.c_float * 8),
+                ("bounds", ctypes.c_float * 8), ("tcurve", ctypes.c_float * 3),
+                ("fixed", ctypes.c_float * 4)]
+
+
+class OrcRbCount(ctypes.Structure):
+    _fields_ = [("n_resamples", ctypes.c_int), ("n_acc", ctypes.c_int), ("n_tot", ctypes.c_int),
+                ("t_cp", ctypes.c_int)]
+
+
+# Defaults of smc2/smc2_rbpf_batch.cu:1080-1110 (prior, bounds, curves, proposal std)
+RB_PRIOR = (0.95, 0.02, 0.1, 0.1, -1.0, 0.5, 0.15, 0.05)
+RB_BOUNDS = (0.8, 0.999, 0.01, 1.0, -10.0, 5.0, 0.01, 1.0)
+RB_TCURVE = (0.02, 0.08, 1.5)
+RB_FIXED = (0.5, 1.0, 0.1, 1.0)
+RB_PSTD = (0.01, 0.02, 0.1, 0.02)
+
+
 def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
@@ -135,6 +157,17 @@ def lib():
         L.orc_learn_update.restype = None
         L.orc_learn_update.argtypes = [ctypes.POINTER(OrcLearn), P, i32, ctypes.POINTER(i32),
                                        ctypes.POINTER(f32), ctypes.POINTER(f32)]
+        L.orc_rb_smc2_run.restype = i32
+        L.orc_rb_smc2_run.argtypes = [ctypes.POINTER(OrcRbCfg), P, i32, P, P, P, P, P, P, P,
+                                      ctypes.POINTER(OrcRbCount)]
+        L.orc_rb_inner_run.restype = i32
+        L.orc_rb_inner_run.argtypes = [ctypes.POINTER(OrcRbCfg), P, P, i32, P, P, P]
+        L.orc_rb_ocsn.restype = None
+        L.orc_rb_ocsn.argtypes = [f32, f32, f32, P]
+        L.orc_rb_math.restype = None
+        L.orc_rb_math.argtypes = [i32, P, P, i32]
+        L.orc_rb_ocsn_table.restype = None
+        L.orc_rb_ocsn_table.argtypes = [P, P, P]
         L.orc_resample_ancestors.restype = None
         L.orc_resample_ancestors.argtypes = [P, i32, f32, P]
         L.orc_bpf_score.restype = i32
@@ -422,3 +455,104 @@ def grid_loglik(y, rho=0.97, sigma_z=0.15, mu=-1.0, n_grid=2048):
     llt = np.empty(len(y), np.float64)
     tot = lib().orc_grid_loglik(_ptr(y), len(y), rho, sigma_z, mu, n_grid, _ptr(llt))
     return tot, llt
+
+
+# ---- SMC^2 with the RBPF / OCSN inner filter (rbpf_oracle.c) -------------
+
+def rb_cfg(n_theta, n_inner, seed=12345, K_rejuv=5, lag=0, cap=None, ess_thr=0.5, crho=0.99,
+           adaptive=True, pstd=RB_PSTD, prior=RB_PRIOR, bounds=RB_BOUNDS, tcurve=RB_TCURVE,
+           fixed=RB_FIXED, T=0):
+    c = OrcRbCfg()
+    c.n_theta, c.n_inner, c.K_rejuv, c.lag = n_theta, n_inner, K_rejuv, lag
+    c.cap = cap if cap is not None else max(2048, T)
+    c.adaptive = 1 if adaptive else 0
+    c.ess_thr, c.crho = ess_thr, crho
+    c.seed = seed if seed else 12345
+    c.pstd[:] = list(pstd)
+    c.prior[:] = list(prior)
+    c.bounds[:] = list(bounds)
+    c.tcurve[:] = list(tcurve)
+    c.fixed[:] = list(fixed)
+    return c
+
+
+def run_rb_smc2(cfg, y):
+    """Whole SMC^2 run; returns a dict of the final state and per-tick ESS."""
+    L = lib()
+    y = np.ascontiguousarray(y, np.float32)
+    T, nt, ni = len(y), cfg.n_theta, cfg.n_inner
+    ess = np.zeros(T, np.float32)
+    th = np.zeros((4, nt), np.float32)
+    ll = np.zeros(nt, np.float64)
+    lw = np.zeros(nt, np.float64)
+    essi = np.zeros(nt, np.float32)
+    inner = np.zeros((4, nt * ni), np.float32)
+    rt = np.zeros(T + 1, np.int32)
+    cnt = OrcRbCount()
+    r = L.orc_rb_smc2_run(ctypes.byref(cfg), _ptr(y), T, _ptr(ess), _ptr(th), _ptr(ll), _ptr(lw),
+                          _ptr(essi), _ptr(inner), _ptr(rt), ctypes.byref(cnt))
+    if r != 0:
+        raise ValueError("orc_rb_smc2_run rejected its arguments")
+    return dict(ess=ess, theta=th, ll=ll, lw=lw, ess_inner=essi, inner=inner,
+                resample_ticks=rt[:cnt.n_resamples].copy(), n_resamples=cnt.n_resamples,
+                n_acc=cnt.n_acc, n_tot=cnt.n_tot, t_cp=cnt.t_cp)
+
+
+def rb_inner_run(cfg, theta, y):
+    """Inner RBPF of one theta-particle (index 0, first epoch), theta fixed."""
+    L = lib()
+    y = np.ascontiguousarray(y, np.float32)
+    T, ni = len(y), cfg.n_inner
+    th = np.ascontiguousarray(theta, np.float32)
+    lli = np.zeros(T, np.float64)
+    ess = np.zeros(T, np.float64)
+    st = np.zeros((4, ni), np.float32)
+    if L.orc_rb_inner_run(ctypes.byref(cfg), _ptr(th), _ptr(y), T, _ptr(lli), _ptr(ess), _ptr(st)) != 0:
+        raise ValueError("orc_rb_inner_run rejected its arguments")
+    return lli, ess, st
+
+
+def rb_ocsn(y, mp, vp):
+    out = np.zeros(3, np.float32)
+    lib().orc_rb_ocsn(float(y), float(mp), float(vp), _ptr(out))
+    return out
+
+
+def rb_math(fn, x):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros_like(x)
+    lib().orc_rb_math(fn, _ptr(x), _ptr(out), len(x))
+    return out
+
+
+def rb_ocsn_table():
+    p, m, v = (np.zeros(10, np.float32) for _ in range(3))
+    lib().orc_rb_ocsn_table(_ptr(p), _ptr(m), _ptr(v))
+    return p, m, v
+
+
+def rb_dgp(T, rho=0.95, sigma_z=0.10, mu_base=-1.0, sigma_base=0.15, fixed=RB_FIXED,
+           tcurve=RB_TCURVE, seed=12345):
+    """Regime-switching SV generator of smc2/test_smc2_rbpf.cu:95-133 (FP64,
+    NumPy normals): returns y = log r^2 (floored at log 1e-30), h and z."""
+    rng = np.random.default_rng(seed)
+    ms, mr, ss, sr = fixed
+    tb, ts, tr = tcurve
+
+    def curve(b, s, r, z):
+        return b + s * (1.0 - np.exp(-r * z))
+
+    zt = sigma_z / np.sqrt(max(1.0 - rho * rho, 1e-6)) * rng.standard_normal()
+    z = 1.5 * (1.0 + np.tanh(zt))
+    th, mu, sh = curve(tb, ts, tr, z), curve(mu_base, ms, mr, z), curve(sigma_base, ss, sr, z)
+    phi = 1.0 - th
+    h = mu + np.sqrt(sh * sh / max(1.0 - phi * phi, 1e-6)) * rng.standard_normal()
+    y, hs, zs = np.zeros(T), np.zeros(T), np.zeros(T)
+    for t in range(T):
+        zt = rho * zt + sigma_z * rng.standard_normal()
+        z = 1.5 * (1.0 + np.tanh(zt))
+        th, mu, sh = curve(tb, ts, tr, z), curve(mu_base, ms, mr, z), curve(sigma_base, ss, sr, z)
+        h = (1.0 - th) * h + th * mu + sh * rng.standard_normal()
+        r2 = max((np.exp(h / 2.0) * rng.standard_normal()) ** 2, 1e-30)
+        y[t], hs[t], zs[t] = np.log(r2), h, z
+    return y.astype(np.float32), hs, zs

@@ -23,7 +23,7 @@ def test_header_declares_reference_surface():
     assert ref <= syms
 
 
-HEADERS = ("gpu_bpf.h", "gpu_bpf_batch.h")
+HEADERS = ("gpu_bpf.h", "gpu_bpf_batch.h", "smc2_rbpf.h")
 
 
 @pytest.mark.parametrize("header", HEADERS)
@@ -76,3 +76,23 @@ def test_python_mirror_fails_loudly_without_library(tmp_path):
     with pytest.raises(mod.GpuBpfError):
         mod.load(str(tmp_path / "missing.so"))
     importlib.reload(gpu_bpf)
+
+
+def test_rbpf_header_is_plain_c_and_mirrored():
+    """include/smc2_rbpf.h compiles as C99 and smc2_rbpf.py binds every
+    entry point it declares (the reference's smc2_rbpf_batch.cuh:458-490 plus
+    the field-access replacements)."""
+    import smc2_rbpf
+    inc = os.path.join(gpu_bpf.REPO_DIR, "include")
+    src = '#include "smc2_rbpf.h"\nint main(void){SMC2StateCUDA* s = 0; SVPrior p; (void)s; (void)p; return 0;}\n'
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", inc, "-x", "c", "-", "-o", "/dev/null"],
+                       input=src, text=True, capture_output=True)
+    assert r.returncode == 0, r.stderr
+    syms = gpu_bpf.header_symbols(os.path.join(inc, "smc2_rbpf.h"))
+    ref = {"smc2_cuda_alloc", "smc2_cuda_free", "smc2_cuda_set_seed", "smc2_cuda_set_noise_capacity",
+           "smc2_cuda_set_fixed_lag", "smc2_cuda_set_cpmmh_rho", "smc2_cuda_set_proposal_std",
+           "smc2_cuda_init_from_prior", "smc2_cuda_update", "smc2_cuda_get_theta_mean",
+           "smc2_cuda_get_theta_std", "smc2_cuda_get_outer_ess"}
+    assert ref <= set(syms)
+    text = open(smc2_rbpf.__file__).read()
+    assert not [s for s in syms if f'"{s}"' not in text]
