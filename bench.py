@@ -247,7 +247,77 @@ def extra_workloads(args, gpu_bpf, oracle):
         "theta_sd": [float(v) for v in tsd],
     }
     log("C4", out["smc2_mala_m1024_t5000_student_t"])
+    out["smc2_rbpf_ntheta1024_t1000"] = rbpf_workload(oracle)
+    log("RBPF", out["smc2_rbpf_ntheta1024_t1000"])
     return out
+
+
+def rbpf_workload(oracle):
+    """SMC^2 with the RBPF / OCSN inner filter, the reference's throughput and
+    latency cases (smc2/test_smc2_rbpf.cu:412-470,600-680): N_theta = 1024,
+    N_inner in {64, 128, 256, 512}, T = 1000 observations of the
+    regime-switching SV model, fixed lag 200, seed 42."""
+    import smc2_rbpf
+    T = 1000
+    y, _, _ = oracle.rb_dgp(T, 0.95, 0.10, -1.0, 0.15, seed=12345)
+    rows = {}
+    for ni in (64, 128, 256, 512):
+        s = smc2_rbpf.Smc2Rbpf(1024, ni, seed=42, fixed_lag=200)
+        s.run(y[:50])
+        s.init_from_prior()
+        lat = np.zeros(T)
+        t0 = time.perf_counter()
+        for t in range(T):
+            a = time.perf_counter()
+            s.update(y[t])
+            lat[t] = time.perf_counter() - a
+        wall = time.perf_counter() - t0
+        c = s.counters()
+        m, sd = s.theta_mean(), s.theta_std()
+        s.close()
+        med = float(np.median(lat))
+        rows[f"n_inner_{ni}"] = {
+            "ms_per_obs": 1e3 * wall / T,
+            "forward_inner_particle_ticks_per_s": 1024 * ni * T / wall,
+            "resamples": c["n_resamples"],
+            "accept_rate": c["n_rejuv_accepts"] / max(c["n_rejuv_total"], 1),
+            "update_us_p50": 1e6 * med,
+            "update_us_p99": 1e6 * float(np.percentile(lat, 99)),
+            "update_us_max": 1e6 * float(lat.max()),
+            "updates_over_3x_median": int((lat > 3 * med).sum()),
+            "theta_mean": [float(v) for v in m],
+            "theta_sd": [float(v) for v in sd],
+        }
+        log("RBPF", ni, rows[f"n_inner_{ni}"])
+    # the reference's learning case (smc2/test_smc2_rbpf.cu:250-335): 1024 x
+    # 512, T = 1200, fixed lag 200; its published time is 300 ms per 1200
+    # ticks on an RTX 5080 (smc2/multi_scale_smc2_architecture.md:34)
+    truth = (0.95, 0.10, -1.0, 0.15)
+    yl, _, _ = oracle.rb_dgp(1200, *truth, seed=12345)
+    s = smc2_rbpf.Smc2Rbpf(1024, 512, seed=12345, noise_capacity=1328, fixed_lag=200)
+    t0 = time.perf_counter()
+    s.run(yl)
+    wall = time.perf_counter() - t0
+    m, sd, c = s.theta_mean(), s.theta_std(), s.counters()
+    s.close()
+    rows["learn_1024x512_t1200"] = {
+        "wall_ms": 1e3 * wall,
+        "reference_wall_ms_rtx5080": 300.0,
+        "speedup_vs_reference": 300.0 / (1e3 * wall),
+        "within_2sd": int(sum(abs(m[q] - truth[q]) <= 2 * sd[q] for q in range(4))),
+        "accept_rate": c["n_rejuv_accepts"] / max(c["n_rejuv_total"], 1),
+        "resamples": c["n_resamples"],
+        "theta_mean": [float(v) for v in m],
+        "theta_sd": [float(v) for v in sd],
+    }
+    log("RBPF learn", rows["learn_1024x512_t1200"])
+    # CPU oracle (OpenMP over theta-particles) on a bounded sample
+    cfg = oracle.rb_cfg(64, 512, seed=42, lag=200, T=T)
+    t0 = time.perf_counter()
+    oracle.run_rb_smc2(cfg, y[:40])
+    rows["cpu_oracle_forward_inner_particle_ticks_per_s"] = 64 * 512 * 40 / (time.perf_counter() - t0)
+    rows["cpu_sample"] = "oracle, N_theta=64, N_inner=512, first 40 observations"
+    return rows
 
 
 def main():
@@ -264,6 +334,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other configurations")
+    ap.add_argument("--rbpf-only", action="store_true",
+                    help="run only the RBPF SMC^2 workload and print its JSON line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 5 device-to-device copies of BYTES (the reference kernel "
                          "for scaling HBM traffic counters) and exit")
@@ -277,6 +349,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 5}))
+        return
+    if args.rbpf_only:
+        import oracle
+        print(json.dumps({"smc2_rbpf_ntheta1024_t1000": rbpf_workload(oracle)}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "

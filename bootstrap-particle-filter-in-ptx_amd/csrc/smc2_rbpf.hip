@@ -278,7 +278,8 @@ struct RbStepArgs {
     Inner in;
     float* zn;
     float* un;
-    double* ll_incr;
+    float* y_hist;            // the observation is stored at index t
+    double* ll_incr;          // mapped pinned host memory
     uint64_t kz, ku;
     long long zrow;
     int ucap1;
@@ -303,6 +304,7 @@ __global__ __launch_bounds__(NI) void k_rb_step(RbStepArgs a) {
         const float n = normal(key(a.ku, (uint64_t)m), (uint32_t)(a.t + 1));
         a.un[(long long)m * a.ucap1 + a.t + 1] = n;
         L.u0 = u0_of(n);
+        if (m == 0) a.y_hist[a.t] = a.y;
     }
     __syncthreads();
     double lli, ess;
@@ -518,13 +520,12 @@ struct SMC2StateCUDA {
     int y_cap = 0;
     int* d_anc = nullptr;
     int* d_acc = nullptr;
-    double* d_lli = nullptr;
+    double* d_lli = nullptr;   // device view of h_lli
     // pinned host staging
     double* h_lli = nullptr;
     int* h_anc = nullptr;
     int* h_acc = nullptr;
     float* h_th = nullptr;     // kParams * nt
-    float* h_y = nullptr;
     // outer state (host, FP64)
     std::vector<double> lw, w;
     int t_cur = -1, t_cp = -1, lag = 0, K = 5;
@@ -753,6 +754,7 @@ int launch_step_t(SMC2StateCUDA* s, float y) {
     a.in = s->in[s->ib];
     a.zn = s->zn[s->nb];
     a.un = s->un[s->nb];
+    a.y_hist = s->d_y;
     a.ll_incr = s->d_lli;
     a.kz = key(base_key(s, kZ), (uint64_t)s->n_res);
     a.ku = key(base_key(s, kU0), (uint64_t)s->n_res);
@@ -942,12 +944,10 @@ void release(SMC2StateCUDA* s) {
     if (s->d_y) (void)hipFree(s->d_y);
     if (s->d_anc) (void)hipFree(s->d_anc);
     if (s->d_acc) (void)hipFree(s->d_acc);
-    if (s->d_lli) (void)hipFree(s->d_lli);
     if (s->h_lli) (void)hipHostFree(s->h_lli);
     if (s->h_anc) (void)hipHostFree(s->h_anc);
     if (s->h_acc) (void)hipHostFree(s->h_acc);
     if (s->h_th) (void)hipHostFree(s->h_th);
-    if (s->h_y) (void)hipHostFree(s->h_y);
     if (s->st) (void)hipStreamDestroy(s->st);
     delete s;
 }
@@ -964,12 +964,13 @@ int alloc_all(SMC2StateCUDA* s) {
         if (dmalloc(&s->cpll[b], n)) return -1;
     }
     if (alloc_noise(s, s->cap, s->zn, s->un)) return -1;
-    if (dmalloc(&s->d_anc, n) || dmalloc(&s->d_acc, n) || dmalloc(&s->d_lli, n)) return -1;
-    HIP_TRY(hipHostMalloc((void**)&s->h_lli, n * 8, hipHostMallocDefault));
+    if (dmalloc(&s->d_anc, n) || dmalloc(&s->d_acc, n)) return -1;
+    // the step kernel writes the per-theta increments straight to host memory
+    HIP_TRY(hipHostMalloc((void**)&s->h_lli, n * 8, hipHostMallocMapped));
+    HIP_TRY(hipHostGetDevicePointer((void**)&s->d_lli, s->h_lli, 0));
     HIP_TRY(hipHostMalloc((void**)&s->h_anc, n * 4, hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void**)&s->h_acc, n * 4, hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void**)&s->h_th, n * 4 * kParams, hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc((void**)&s->h_y, 4, hipHostMallocDefault));
     if (ensure_y(s, 8000)) return -1;
     return 0;
 }
@@ -1072,15 +1073,9 @@ float smc2_cuda_update(SMC2StateCUDA* state, float y_obs) {
     const int t = s->t_cur + 1;
     if (ensure_y(s, t + 1)) return nan;
     if (t >= s->cap && grow_noise(s, 2 * s->cap)) return nan;
-    s->h_y[0] = y_obs;
-    if (hipMemcpyAsync(s->d_y + t, s->h_y, 4, hipMemcpyHostToDevice, s->st) != hipSuccess) {
-        set_err("smc2_cuda_update: history upload failed");
-        return nan;
-    }
     s->t_cur = t;
     if (launch_step(s, y_obs)) return nan;
-    if (hipMemcpyAsync(s->h_lli, s->d_lli, (size_t)s->nt * 8, hipMemcpyDeviceToHost, s->st) != hipSuccess ||
-        hipStreamSynchronize(s->st) != hipSuccess) {
+    if (hipStreamSynchronize(s->st) != hipSuccess) {
         set_err("smc2_cuda_update: tick failed: %s", hipGetErrorString(hipGetLastError()));
         return nan;
     }
