@@ -821,11 +821,17 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
 // (gpu_bpf/gpu_bpf.cu:297-310, clamp to N-1 as ptx:816-830) on this CDF.
 // ---------------------------------------------------------------------------
 
-// next_out (nullable): thread 0 stores `next` there before the final barrier.
-template <int BLOCK, int K>
+// next_out (nullable): thread 0 stores `next` there before the first barrier;
+// pre() runs on every thread right after that barrier (the caller's next
+// tile loads, issued while this tile's CDF is built).
+struct NoPre {
+    __device__ void operator()() const {}
+};
+template <int BLOCK, int K, class Pre = NoPre>
 __device__ inline int tile_cdf(const float (&e)[K], int b, int n, int n_tiles, double ft,
                                double Bb, double Bb1, float u, int W, double* s_c,
-                               double* s_w, int* s_rng, int* next_out = nullptr, int next = 0) {
+                               double* s_w, int* s_rng, int* next_out = nullptr, int next = 0,
+                               Pre pre = Pre()) {
     constexpr int VEC = K < 4 ? K : 4;
     constexpr int NV = K / VEC;
     constexpr int TILE = BLOCK * K;
@@ -843,7 +849,9 @@ __device__ inline int tile_cdf(const float (&e)[K], int b, int n, int n_tiles, d
             s_e[loc + (loc >> 4)] = e[v * VEC + c];
         }
     }
+    if (tid == 0 && next_out) *next_out = next;
     __syncthreads();
+    pre();
 
     // lane-contiguous FP64 scan: lane tid owns tile positions [tid*K, tid*K+K)
     float ev[K];
@@ -894,7 +902,6 @@ __device__ inline int tile_cdf(const float (&e)[K], int b, int n, int n_tiles, d
         else if (tid == 2) r = (b == 0) ? W : count_le(Bb, u, W, n, n, true);
         else               r = (b == n_tiles - 1) ? n : count_le(Bb1, u, W, n, n, true);
         s_rng[tid] = r;
-        if (tid == 0 && next_out) *next_out = next;
     }
     __syncthreads();
     return last;
@@ -960,7 +967,7 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
     __shared__ double s_c[TP];
     __shared__ double s_w[NW];
     __shared__ int s_rng[4];
-    __shared__ int s_tile;
+    __shared__ int s_first, s_tile;      // first tile; next tile (written each tile)
     __shared__ float s_al[FISH ? 3 : 1][FISH ? TP : 1];
 
     const int tid = threadIdx.x;
@@ -995,9 +1002,9 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
     // (block-uniform values read from LDS go through readfirstlane, so that
     // tile bases are scalar and per-lane addresses 32-bit offsets)
     int cr = blockIdx.x % kClaimRanges, ctried = 0;   // blocks alternate between XCDs
-    if (tid == 0) s_tile = a.one_tile ? (int)blockIdx.x : claim_tile(a.tile_ctr, cr, ctried, a.part.n_tiles);
+    if (tid == 0) s_first = a.one_tile ? (int)blockIdx.x : claim_tile(a.tile_ctr, cr, ctried, a.part.n_tiles);
     __syncthreads();
-    int b = __builtin_amdgcn_readfirstlane(s_tile);
+    int b = __builtin_amdgcn_readfirstlane(s_first);
     float xv[K], lv[K];
     if (b < a.part.n_tiles) {
         const float* xb = xs + (long long)b * TILE;
@@ -1053,62 +1060,103 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
                 s_x[pad16(loc)] = xv[k];
             }
         }
+        // the next tile's loads are issued once its index is published (the
+        // first barrier of the CDF build) and overlap the CDF, the range
+        // counts and this tile's stores
+        auto prefetch = [&]() {
+            const int bq = __builtin_amdgcn_readfirstlane(s_tile);
+            if (bq < a.part.n_tiles) {
+                const float* xb = xs + (long long)bq * TILE;
+                const float* lb = a.lw + (long long)bq * TILE;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    const int loc0 = (v * BLOCK + tid) * VEC;
+                    load_vec<VEC>(xb + loc0, &xv[v * VEC]);
+                    if (a.lw) load_vec<VEC>(lb + loc0, &lv[v * VEC]);
+                }
+            }
+        };
         const int last = tile_cdf<BLOCK, K>(e, b, n, a.part.n_tiles, ts.ft, ts.b0, ts.b1, u, W, s_c, s_w,
-                                            s_rng, &s_tile, nb);
+                                            s_rng, &s_tile, nb, prefetch);
         const int ilo = __builtin_amdgcn_readfirstlane(s_rng[0]);
         const int ihi = __builtin_amdgcn_readfirstlane(s_rng[1]);
         const int jlo = __builtin_amdgcn_readfirstlane(s_rng[2]);
         const int jhi = __builtin_amdgcn_readfirstlane(s_rng[3]);
         const int bn = __builtin_amdgcn_readfirstlane(s_tile);
-        if (bn < a.part.n_tiles) {     // the next tile's loads overlap this tile's stores
-            const float* xb = xs + (long long)bn * TILE;
-            const float* lb = a.lw + (long long)bn * TILE;
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                const int loc0 = (v * BLOCK + tid) * VEC;
-                load_vec<VEC>(xb + loc0, &xv[v * VEC]);
-                if (a.lw) load_vec<VEC>(lb + loc0, &lv[v * VEC]);
-            }
-        }
 
-        // Offspring by merge path.  The owned unwrapped targets t_ilo..t_ihi-1
-        // and the in-tile CDF c_0..c_last are both non-decreasing, and output
-        // i goes to the first parent j with c_j >= t_i (lower_bound; every
-        // owned target is <= c_last).  Merging the two sequences -- emit output
-        // a when t_a <= c_j, else pass parent j -- takes n_out + P steps; lane
-        // tid takes the tid-th slice of D steps and finds where its slice
-        // starts by a binary search along the merge path, so every lane does
-        // the same work whatever the weight degeneracy.
+        // Offspring.  Output i goes to the first parent j with c_j >= t_i
+        // (lower_bound; every owned unwrapped target is <= c_last), and both
+        // sequences are non-decreasing, so ancestors are too.
+        // Lane tid owns a contiguous run of ceil(n_out / BLOCK) outputs: it
+        // finds the first one's parent by a binary search over the tile CDF
+        // and each next one by galloping forward from the previous parent
+        // (O(log gap) whatever the weight degeneracy).  The Fisher variant
+        // keeps the merge path below, which moves the three extra vectors.
         const int n_out = ihi - ilo;
         if (n_out > 0) {
-            const int P = last + 1;
-            const int Lm = n_out + P;
-            const int D = (Lm + BLOCK - 1) / BLOCK;
-            const int d0 = min(tid * D, Lm), d1 = min(d0 + D, Lm);
-            // outputs among the first d0 merge steps: output m comes before
-            // parent d0 - m - 1 is passed iff t_m <= c_(d0-m-1)
-            int lo = max(0, d0 - P), hi = min(d0, n_out);
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                const double tm = (double)target_unwrapped(u, ilo + mid, n);
-                if (tm <= s_c[pad16(d0 - mid - 1)]) lo = mid + 1; else hi = mid;
-            }
-            int oa = lo, j = d0 - lo;
-            double cj = (j < P) ? s_c[pad16(j)] : __builtin_inf();
-            double tv = (oa < n_out) ? (double)target_unwrapped(u, ilo + oa, n) : 0.0;
-            for (int d = d0; d < d1; ++d) {
-                if (oa < n_out && tv <= cj) {
-                    const int jp = pad16(j);
-                    xd[ilo + oa] = s_x[jp];
-                    if constexpr (FISH) {
+            if constexpr (!FISH) {
+                const int per = (n_out + BLOCK - 1) / BLOCK;
+                const int o0 = min(tid * per, n_out), o1 = min(o0 + per, n_out);
+                if (o0 < o1) {
+                    double t = (double)target_unwrapped(u, ilo + o0, n);
+                    int lo = 0, hi = last;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (s_c[pad16(mid)] < t) lo = mid + 1; else hi = mid;
+                    }
+                    int j = lo;
+                    xd[ilo + o0] = s_x[pad16(j)];
+                    for (int o = o0 + 1; o < o1; ++o) {
+                        t = (double)target_unwrapped(u, ilo + o, n);
+                        if (s_c[pad16(j)] < t) {
+                            // the parent lies in (j, last]: gallop, then search
+                            int g0 = j + 1, g1 = g0, step = 1;
+                            while (g1 < last && s_c[pad16(g1)] < t) {
+                                g0 = g1 + 1;
+                                step <<= 1;
+                                g1 = min(g0 + step - 1, last);
+                            }
+                            while (g0 < g1) {
+                                const int mid = (g0 + g1) >> 1;
+                                if (s_c[pad16(mid)] < t) g0 = mid + 1; else g1 = mid;
+                            }
+                            j = g0;
+                        }
+                        xd[ilo + o] = s_x[pad16(j)];
+                    }
+                }
+            } else {
+                // merge path over the owned targets and the tile CDF:
+                // emit output a when t_a <= c_j, else pass parent j; lane tid
+                // takes the tid-th slice of the n_out + P steps and finds
+                // where it starts by a binary search along the path
+                const int P = last + 1;
+                const int Lm = n_out + P;
+                const int D = (Lm + BLOCK - 1) / BLOCK;
+                const int d0 = min(tid * D, Lm), d1 = min(d0 + D, Lm);
+                // outputs among the first d0 merge steps: output m comes before
+                // parent d0 - m - 1 is passed iff t_m <= c_(d0-m-1)
+                int lo = max(0, d0 - P), hi = min(d0, n_out);
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    const double tm = (double)target_unwrapped(u, ilo + mid, n);
+                    if (tm <= s_c[pad16(d0 - mid - 1)]) lo = mid + 1; else hi = mid;
+                }
+                int oa = lo, j = d0 - lo;
+                double cj = (j < P) ? s_c[pad16(j)] : __builtin_inf();
+                double tv = (oa < n_out) ? (double)target_unwrapped(u, ilo + oa, n) : 0.0;
+                for (int d = d0; d < d1; ++d) {
+                    if (oa < n_out && tv <= cj) {
+                        const int jp = pad16(j);
+                        xd[ilo + oa] = s_x[jp];
 #pragma unroll
                         for (int c = 0; c < 3; ++c) ald[c][ilo + oa] = s_al[c][jp];
+                        ++oa;
+                        tv = (oa < n_out) ? (double)target_unwrapped(u, ilo + oa, n) : 0.0;
+                    } else {
+                        ++j;
+                        cj = (j < P) ? s_c[pad16(j)] : __builtin_inf();
                     }
-                    ++oa;
-                    tv = (oa < n_out) ? (double)target_unwrapped(u, ilo + oa, n) : 0.0;
-                } else {
-                    ++j;
-                    cj = (j < P) ? s_c[pad16(j)] : __builtin_inf();
                 }
             }
         }

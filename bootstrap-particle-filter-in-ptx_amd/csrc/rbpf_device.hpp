@@ -134,31 +134,41 @@ __device__ inline void predict(float rho, float sz, float mub, float sgb, float 
 __device__ inline void ocsn_update(float y, float mp, float vp, const Model& md, float& mu_post,
                                    float& var_post, float& lnorm) {
     const float v = fmaxf(vp, 1e-6f);
-    float val[kOcsnK], inn[kOcsnK], inv[kOcsnK];
+    // per component: log-likelihood, posterior mean m_k and second moment
+    // var_k + m_k^2 (the same expressions as the reference's second loop,
+    // formed here so only three values per component stay live)
+    float val[kOcsnK], mk[kOcsnK], s2[kOcsnK];
     float mx = -__builtin_inff();
     const float d = y - mp;
 #pragma unroll
     for (int k = 0; k < kOcsnK; ++k) {
         const float S = v + md.ov[k];
-        inv[k] = 1.0f / S;
-        inn[k] = d - md.om[k];
-        const float q = inn[k] * inn[k] * inv[k];
+        const float inv = 1.0f / S;
+        const float inn = d - md.om[k];
+        const float q = inn * inn * inv;
         val[k] = md.olw[k] - 0.5f * (logf_cw(S) + q);
         mx = fmaxf(mx, val[k]);
+        const float K = v * inv;
+        mk[k] = mp + K * inn;
+        const float vk = (1.0f - K) * v;
+        s2[k] = vk + mk[k] * mk[k];
     }
+    // component weights e_k / sum e, e_k = exp(val_k - max) (the reference
+    // recomputes exp(val_k - lnorm); the quotient is the same weight)
     float se = 0.0f;
 #pragma unroll
-    for (int k = 0; k < kOcsnK; ++k) se += expf_cw(val[k] - mx);
+    for (int k = 0; k < kOcsnK; ++k) {
+        val[k] = expf_cw(val[k] - mx);
+        se += val[k];
+    }
     lnorm = mx + logf_cw(se);
+    const float rse = 1.0f / se;
     float mo = 0.0f, e2 = 0.0f;
 #pragma unroll
     for (int k = 0; k < kOcsnK; ++k) {
-        const float w = expf_cw(val[k] - lnorm);
-        const float K = v * inv[k];
-        const float mk = mp + K * inn[k];
-        const float vk = (1.0f - K) * v;
-        mo += w * mk;
-        e2 += w * (vk + mk * mk);
+        const float w = val[k] * rse;
+        mo += w * mk[k];
+        e2 += w * s2[k];
     }
     mu_post = mo;
     var_post = fmaxf(e2 - mo * mo, 1e-6f);

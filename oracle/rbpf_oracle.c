@@ -152,12 +152,17 @@ static inline void rb_ocsn(float y, float mp, float vp, const RbModel* md, float
         val[k] = md->olw[k] - 0.5f * (orc_logf(S) + q);
         mx = fmaxf(mx, val[k]);
     }
+    float e[RB_K];
     float se = 0.0f;
-    for (int k = 0; k < RB_K; ++k) se += orc_expf(val[k] - mx);
+    for (int k = 0; k < RB_K; ++k) {
+        e[k] = orc_expf(val[k] - mx);
+        se += e[k];
+    }
     *lnorm = mx + orc_logf(se);
+    const float rse = 1.0f / se;
     float mo = 0.0f, e2 = 0.0f;
     for (int k = 0; k < RB_K; ++k) {
-        const float w = orc_expf(val[k] - *lnorm);
+        const float w = e[k] * rse;
         const float K = v * inv[k];
         const float mk = mp + K * inn[k];
         const float vk = (1.0f - K) * v;
