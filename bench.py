@@ -311,6 +311,24 @@ def rbpf_workload(oracle):
         "theta_sd": [float(v) for v in sd],
     }
     log("RBPF learn", rows["learn_1024x512_t1200"])
+    # fixed-lag against full-history replays at T = 5000
+    # (smc2/test_smc2_rbpf.cu:343-410)
+    y5, _, _ = oracle.rb_dgp(5000, *truth, seed=12345)
+    for lag in (0, 200):
+        s = smc2_rbpf.Smc2Rbpf(1024, 512, seed=12345, noise_capacity=5128, fixed_lag=lag)
+        t0 = time.perf_counter()
+        s.run(y5)
+        wall = time.perf_counter() - t0
+        m, sd, c = s.theta_mean(), s.theta_std(), s.counters()
+        s.close()
+        rows[f"lag_{lag}_1024x512_t5000"] = {
+            "wall_ms": 1e3 * wall,
+            "resamples": c["n_resamples"],
+            "accept_rate": c["n_rejuv_accepts"] / max(c["n_rejuv_total"], 1),
+            "theta_mean": [float(v) for v in m],
+            "theta_sd": [float(v) for v in sd],
+        }
+        log("RBPF lag", lag, rows[f"lag_{lag}_1024x512_t5000"])
     # CPU oracle (OpenMP over theta-particles) on a bounded sample
     cfg = oracle.rb_cfg(64, 512, seed=42, lag=200, T=T)
     t0 = time.perf_counter()

@@ -147,3 +147,23 @@ def test_parameter_recovery():
     ok = sum(abs(m[q] - TRUTH[q]) <= 2 * s[q] for q in range(4))
     assert c["n_resamples"] >= 3 and c["n_rejuv_accepts"] > 0
     assert ok >= 3, (m, s)
+
+
+def test_high_volatility_regime():
+    """smc2/test_smc2_rbpf.cu:466-531: truth (0.98, 0.08, 2.0, 0.20), T 1000,
+    widened prior (mu_base 0 +- 2, bounds [-5, 8]), fixed lag 200, seed 999;
+    passes when at least 2 of 4 parameters lie within 2 posterior sd."""
+    import oracle
+    truth = (0.98, 0.08, 2.0, 0.20)
+    y, _, _ = oracle.rb_dgp(1000, *truth, seed=999)
+    say("rbpf high vol")
+    prior = (0.96, 0.02, 0.10, 0.10, 0.0, 2.0, 0.15, 0.10)
+    bounds = (0.8, 0.999, 0.01, 1.0, -5.0, 8.0, 0.01, 1.0)
+    f = smc2_rbpf.Smc2Rbpf(1024, 512, seed=999, fixed_lag=200, prior=prior, bounds=bounds)
+    f.run(y)
+    m, s = f.theta_mean(), f.theta_std()
+    f.close()
+    say("mean", m, "std", s)
+    assert sum(abs(m[q] - truth[q]) <= 2 * s[q] for q in range(4)) >= 2, (m, s)
+    assert abs(m[2] - truth[2]) < 1.0
+
