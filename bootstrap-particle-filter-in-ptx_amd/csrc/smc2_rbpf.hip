@@ -536,6 +536,8 @@ struct SMC2StateCUDA {
     SVBounds bounds{};
     SVCurve tcurve{};
     SVFixedCurves fixed{};
+    SVPrior run_prior{};      // the prior and bounds of the current run (copied at init)
+    SVBounds run_bounds{};
     bool adaptive = true;
     uint64_t seed = 12345;
     uint64_t host_rng = 0;
@@ -792,8 +794,8 @@ int launch_move_t(SMC2StateCUDA* s, int k, bool use_cp) {
     a.cscale = __builtin_sqrtf(1.0f - s->crho * s->crho);
     std::memcpy(a.chol, s->chol, sizeof(a.chol));
     std::memcpy(a.pstd, s->pstd, sizeof(a.pstd));
-    a.prior = s->prior;
-    a.bounds = s->bounds;
+    a.prior = s->run_prior;
+    a.bounds = s->run_bounds;
     a.md = s->md;
     hipLaunchKernelGGL(k_rb_move<NI>, dim3(s->nt), dim3(NI), 0, s->st, a);
     HIP_TRY(hipGetLastError());
@@ -1046,6 +1048,8 @@ void smc2_cuda_init_from_prior(SMC2StateCUDA* state) {
     SMC2StateCUDA* s = state;
     build_model(s);
     initial_chol(s);
+    s->run_prior = s->prior;
+    s->run_bounds = s->bounds;
     s->tb = s->ib = s->nb = s->cb = 0;
     s->n_res = s->n_acc = s->n_tot = 0;
     s->t_cur = -1;
