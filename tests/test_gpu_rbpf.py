@@ -127,7 +127,21 @@ def test_errors_and_model_access():
     th = g.particles()["theta"]
     assert abs(th[0].mean() - 0.9) < 0.01 and abs(th[2].mean() - 0.5) < 0.1
     assert g.counters()["t_current"] == -1 and g.outer_ess() == pytest.approx(16.0)
+    with pytest.raises(Exception, match="K_rejuv"):
+        smc2_rbpf.Smc2Rbpf(8, 64, K_rejuv=-1)
     g.close()
+
+
+def test_seed_zero_is_the_default_seed(orc):
+    """smc2_cuda_set_seed(s, 0) selects the fixed seed 12345 (the reference
+    seeds from the clock there)."""
+    y, _, _ = orc.rb_dgp(20, *TRUTH, seed=3)
+    say("rbpf seed 0")
+    a = gpu_run(8, 64, y, seed=0, ess_thr=0.95, K=1)
+    b = gpu_run(8, 64, y, seed=12345, ess_thr=0.95, K=1)
+    assert np.array_equal(a["theta"], b["theta"]) and np.array_equal(a["inner"], b["inner"])
+    c = gpu_run(8, 64, y, seed=7, ess_thr=0.95, K=1)
+    assert not np.array_equal(a["inner"], c["inner"])
 
 
 def test_parameter_recovery():
