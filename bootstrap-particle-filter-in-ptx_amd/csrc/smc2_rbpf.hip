@@ -49,7 +49,7 @@ template <int NI>
 struct Lds {
     float z[NI], mu[NI], var[NI];
     double c[NI];
-    uint64_t k[NI];
+    uint64_t k[2][NI];
     double red[2][NI / kWave];
     float fred[NI / kWave];
     float u0;
@@ -103,23 +103,39 @@ __device__ inline double blk_scan(double v, double* s, int lane, int wid, double
     return incl + P;
 }
 
-// Ascending bitonic sort of NI distinct 64-bit keys (LDS, one key per lane).
+// Ascending bitonic sort of NI distinct 64-bit keys, one per lane, held in
+// a register: stages whose partner lane (tid ^ j) lies in the same wave
+// exchange keys by lane shuffles, the others through LDS.  Lane tid ends with
+// the tid-th smallest key.  The keys are distinct, so the result is the
+// sorted order whatever the network.
+__device__ inline uint64_t shfl_xor_u64(uint64_t v, int m) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m, kWave);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m, kWave);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 template <int NI>
-__device__ inline void bitonic(uint64_t* k, int tid) {
+__device__ inline uint64_t bitonic(uint64_t key, uint64_t (*k)[NI], int tid) {
+    int buf = 0;     // LDS stages alternate buffers: one barrier per stage
+#pragma unroll
     for (int s = 2; s <= NI; s <<= 1) {
+#pragma unroll
         for (int j = s >> 1; j > 0; j >>= 1) {
-            const int p = tid ^ j;
-            if (p > tid) {
-                const uint64_t a = k[tid], b = k[p];
-                const bool asc = (tid & s) == 0;
-                if (asc ? (a > b) : (a < b)) {
-                    k[tid] = b;
-                    k[p] = a;
-                }
+            uint64_t p;
+            if (j < kWave) {
+                p = shfl_xor_u64(key, j);
+            } else {
+                k[buf][tid] = key;
+                __syncthreads();
+                p = k[buf][tid ^ j];
+                buf ^= 1;
             }
-            __syncthreads();
+            const bool asc = (tid & s) == 0, lower = (tid & j) == 0;
+            const uint64_t mn = key < p ? key : p, mx = key < p ? p : key;
+            key = (asc == lower) ? mn : mx;
         }
     }
+    return key;
 }
 
 // One RBPF tick of one theta-particle (smc2/smc2_rbpf_batch.cu:305-393; the
@@ -164,10 +180,9 @@ __device__ inline void tick(Lds<NI>& L, float& zt, float& mu, float& var, float&
         L.z[tid] = zt;
         L.mu[tid] = mu;
         L.var[tid] = var;
-        L.k[tid] = ((uint64_t)sort_key(mu) << 32) | (uint32_t)tid;
         __syncthreads();
-        bitonic<NI>(L.k, tid);
-        const int src = (int)(uint32_t)L.k[tid];
+        const uint64_t kk = bitonic<NI>(((uint64_t)sort_key(mu) << 32) | (uint32_t)tid, L.k, tid);
+        const int src = (int)(uint32_t)kk;
         zt = L.z[src];
         mu = L.mu[src];
         var = L.var[src];
