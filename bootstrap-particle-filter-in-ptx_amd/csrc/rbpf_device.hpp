@@ -24,12 +24,17 @@ constexpr float kOcsnM[kOcsnK] = {1.92677f,  1.34744f,  0.73504f,  0.02266f,  -0
                                   -1.97278f, -3.46788f, -5.55246f, -8.68384f, -14.65000f};
 constexpr float kOcsnV[kOcsnK] = {0.11265f, 0.17788f, 0.26768f, 0.40611f, 0.62699f,
                                   0.98583f, 1.57469f, 2.54498f, 4.16591f, 7.33342f};
+// (float)(log((double)kOcsnP[k]) - log(2 pi) / 2), exactly (the host checks
+// its libm against these at set-up; compile-time constants keep the replay
+// loop's scalar registers free)
+constexpr float kOcsnLogW[kOcsnK] = {-0x1.81486ep+2f, -0x1.faf8b4p+1f, -0x1.7a3660p+1f, -0x1.3f63c2p+1f,
+                                     -0x1.3356b2p+1f, -0x1.4b4440p+1f, -0x1.84847cp+1f, -0x1.e6c716p+1f,
+                                     -0x1.44787ap+2f, -0x1.ebf6b0p+2f};
 
 // Fixed part of the model, passed by value to every kernel.
 struct Model {
     float tb, ts, tr;          // theta(z) curve
     float ms, mr, ss, sr;      // shapes of mu(z) and sigma_h(z)
-    float om[kOcsnK], ov[kOcsnK], olw[kOcsnK];   // OCSN means, variances, log-weights
     float lw0;                 // -log(N_inner), the log-weight after a resample
 };
 
@@ -142,11 +147,11 @@ __device__ inline void ocsn_update(float y, float mp, float vp, const Model& md,
     const float d = y - mp;
 #pragma unroll
     for (int k = 0; k < kOcsnK; ++k) {
-        const float S = v + md.ov[k];
+        const float S = v + kOcsnV[k];
         const float inv = 1.0f / S;
-        const float inn = d - md.om[k];
+        const float inn = d - kOcsnM[k];
         const float q = inn * inn * inv;
-        val[k] = md.olw[k] - 0.5f * (logf_cw(S) + q);
+        val[k] = kOcsnLogW[k] - 0.5f * (logf_cw(S) + q);
         mx = fmaxf(mx, val[k]);
         const float K = v * inv;
         mk[k] = mp + K * inn;

@@ -592,7 +592,7 @@ void default_model(SMC2StateCUDA* s) {
     s->tcurve = {0.02f, 0.08f, 1.5f};
 }
 
-void build_model(SMC2StateCUDA* s) {
+int build_model(SMC2StateCUDA* s) {
     Model& m = s->md;
     m.tb = s->tcurve.base;
     m.ts = s->tcurve.scale;
@@ -603,12 +603,14 @@ void build_model(SMC2StateCUDA* s) {
     m.sr = s->fixed.sigma_rate;
     const double half_log_2pi = 0.5 * std::log(2.0 * 3.14159265358979323846);
     for (int k = 0; k < kOcsnK; ++k) {
-        m.om[k] = kOcsnM[k];
-        m.ov[k] = kOcsnV[k];
-        m.olw[k] = (float)(std::log((double)kOcsnP[k]) - half_log_2pi);
+        if ((float)(std::log((double)kOcsnP[k]) - half_log_2pi) != kOcsnLogW[k]) {
+            set_err("OCSN log-weight %d differs from this host's libm", k);
+            return -1;
+        }
     }
     // log-weight of every inner particle after a resample
     m.lw0 = -(float)std::log((double)s->ni);
+    return 0;
 }
 
 uint64_t base_key(const SMC2StateCUDA* s, Purpose p) { return key(s->seed, (uint64_t)p); }
@@ -1018,7 +1020,10 @@ SMC2StateCUDA* smc2_cuda_alloc(int N_theta, int N_inner) {
         release(s);
         return nullptr;
     }
-    build_model(s);
+    if (build_model(s)) {
+        release(s);
+        return nullptr;
+    }
     initial_chol(s);
     return s;
 }
@@ -1061,7 +1066,7 @@ void smc2_cuda_init_from_prior(SMC2StateCUDA* state) {
     if (!state) return;
     clear_err();
     SMC2StateCUDA* s = state;
-    build_model(s);
+    if (build_model(s)) return;
     initial_chol(s);
     s->run_prior = s->prior;
     s->run_bounds = s->bounds;
