@@ -520,6 +520,7 @@ using namespace bpf::rb;
 struct SMC2StateCUDA {
     int nt = 0, ni = 0;
     hipStream_t st = nullptr;
+    hipEvent_t ev_tick = nullptr;
     Theta th[2] = {};
     int tb = 0;
     Inner in[2] = {};
@@ -967,6 +968,7 @@ void release(SMC2StateCUDA* s) {
     if (s->h_anc) (void)hipHostFree(s->h_anc);
     if (s->h_acc) (void)hipHostFree(s->h_acc);
     if (s->h_th) (void)hipHostFree(s->h_th);
+    if (s->ev_tick) (void)hipEventDestroy(s->ev_tick);
     if (s->st) (void)hipStreamDestroy(s->st);
     delete s;
 }
@@ -974,6 +976,7 @@ void release(SMC2StateCUDA* s) {
 int alloc_all(SMC2StateCUDA* s) {
     const size_t n = (size_t)s->nt, tot = n * (size_t)s->ni;
     HIP_TRY(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&s->ev_tick, hipEventDisableTiming));
     for (int b = 0; b < 2; ++b) {
         for (int q = 0; q < kParams; ++q)
             if (dmalloc(&s->th[b].p[q], n)) return -1;
@@ -1099,8 +1102,17 @@ float smc2_cuda_update(SMC2StateCUDA* state, float y_obs) {
     if (t >= s->cap && grow_noise(s, 2 * s->cap)) return nan;
     s->t_cur = t;
     if (launch_step(s, y_obs)) return nan;
-    if (hipStreamSynchronize(s->st) != hipSuccess) {
-        set_err("smc2_cuda_update: tick failed: %s", hipGetErrorString(hipGetLastError()));
+    // the tick's increments are read right away: wait by polling an event
+    // (a blocking synchronisation adds the host thread's wake-up latency to
+    // every update)
+    hipError_t we = hipEventRecord(s->ev_tick, s->st);
+    if (we == hipSuccess) {
+        do {
+            we = hipEventQuery(s->ev_tick);
+        } while (we == hipErrorNotReady);
+    }
+    if (we != hipSuccess) {
+        set_err("smc2_cuda_update: tick failed: %s", hipGetErrorString(we));
         return nan;
     }
     for (int m = 0; m < s->nt; ++m) s->lw[m] += s->h_lli[m];
@@ -1115,10 +1127,10 @@ float smc2_cuda_update(SMC2StateCUDA* state, float y_obs) {
     if (s->lag > 0 && t > 0 && t % s->lag == 0 && t > s->t_cp) {
         if (save_checkpoint(s)) return nan;
         s->t_cp = t;
-    }
-    if (hipStreamSynchronize(s->st) != hipSuccess) {
-        set_err("smc2_cuda_update: %s", hipGetErrorString(hipGetLastError()));
-        return nan;
+        if (hipStreamSynchronize(s->st) != hipSuccess) {
+            set_err("smc2_cuda_update: checkpoint: %s", hipGetErrorString(hipGetLastError()));
+            return nan;
+        }
     }
     return ess;
 }
