@@ -311,6 +311,13 @@ def rbpf_workload(oracle):
         "theta_sd": [float(v) for v in sd],
     }
     log("RBPF learn", rows["learn_1024x512_t1200"])
+    s = smc2_rbpf.Smc2Rbpf(1024, 512, seed=12345, noise_capacity=1328, fixed_lag=200, noise_bits=16)
+    t0 = time.perf_counter()
+    s.run(yl)
+    rows["learn_1024x512_t1200_fp16_noise"] = {"wall_ms": 1e3 * (time.perf_counter() - t0),
+                                                "theta_mean": [float(v) for v in s.theta_mean()]}
+    s.close()
+    log("RBPF learn fp16 noise", rows["learn_1024x512_t1200_fp16_noise"])
     # fixed-lag against full-history replays at T = 5000
     # (smc2/test_smc2_rbpf.cu:343-410)
     y5, _, _ = oracle.rb_dgp(5000, *truth, seed=12345)

@@ -138,6 +138,30 @@ __device__ inline uint64_t bitonic(uint64_t key, uint64_t (*k)[NI], int tid) {
     return key;
 }
 
+// Noise history elements: FP32 (default) or IEEE half (the reference's
+// SMC2_NOISE_FP16, smc2/smc2_noise_precision.cuh:23-58).  A stored noise is
+// always used as stored, so a replay repeats the forward tick.
+template <bool H>
+struct NoiseT {
+    using type = float;
+};
+template <>
+struct NoiseT<true> {
+    using type = _Float16;
+};
+template <bool H>
+__device__ inline float nstore(void* p, long long i, float v) {
+    using T = typename NoiseT<H>::type;
+    const T h = (T)v;
+    static_cast<T*>(p)[i] = h;
+    return (float)h;
+}
+template <bool H>
+__device__ inline float nload(const void* p, long long i) {
+    using T = typename NoiseT<H>::type;
+    return (float)static_cast<const T*>(p)[i];
+}
+
 // One RBPF tick of one theta-particle (smc2/smc2_rbpf_batch.cu:305-393; the
 // same tick is replayed by CPMMH, :786-875).  Lane tid holds inner particle
 // tid; zn is its regime noise for this tick and u0 the block's resampling
@@ -232,8 +256,8 @@ struct Inner {
 struct RbInitArgs {
     Theta th;
     Inner in;
-    float* zn;
-    float* un;
+    void* zn;
+    void* un;
     uint64_t kprior, kz, ku;
     long long zrow;
     int ucap1;
@@ -245,7 +269,7 @@ struct RbInitArgs {
 // smc2_rbpf_batch.cu:166-230.  Lane 0 draws theta from the prior, up to 1000
 // attempts until all four lie within the bounds (the last attempt is kept
 // otherwise); every lane draws its regime from the stationary law.
-template <int NI>
+template <int NI, bool H>
 __global__ __launch_bounds__(NI) void k_rb_init(RbInitArgs a) {
     __shared__ float s_th[kParams];
     const int m = blockIdx.x, tid = threadIdx.x;
@@ -275,11 +299,10 @@ __global__ __launch_bounds__(NI) void k_rb_init(RbInitArgs a) {
         }
         a.th.ll[m] = 0.0;
         a.th.essi[m] = (float)NI;
-        a.un[(long long)m * a.ucap1] = normal(key(a.ku, (uint64_t)m), 0u);
+        nstore<H>(a.un, (long long)m * a.ucap1, normal(key(a.ku, (uint64_t)m), 0u));
     }
     __syncthreads();
-    const float n0 = normal(key(a.kz, (uint64_t)g), 0u);
-    a.zn[(long long)m * a.zrow + tid] = n0;
+    const float n0 = nstore<H>(a.zn, (long long)m * a.zrow + tid, normal(key(a.kz, (uint64_t)g), 0u));
     float zt, mu, var;
     init_particle(s_th[0], s_th[1], s_th[2], s_th[3], n0, a.md, zt, mu, var);
     a.in.a[0][g] = zt;
@@ -291,8 +314,8 @@ __global__ __launch_bounds__(NI) void k_rb_init(RbInitArgs a) {
 struct RbStepArgs {
     Theta th;
     Inner in;
-    float* zn;
-    float* un;
+    void* zn;
+    void* un;
     float* y_hist;            // the observation is stored at index t
     double* ll_incr;          // mapped pinned host memory
     uint64_t kz, ku;
@@ -306,18 +329,18 @@ struct RbStepArgs {
 // One forward tick at observation t (smc2_rbpf_batch.cu:245-398): draws and
 // stores this tick's noise (index t + 1), runs the tick in place, adds the
 // increment to the pseudo-marginal log-likelihood.
-template <int NI>
+template <int NI, bool H>
 __global__ __launch_bounds__(NI) void k_rb_step(RbStepArgs a) {
     __shared__ Lds<NI> L;
     const int m = blockIdx.x, tid = threadIdx.x;
     const long long g = (long long)m * NI + tid;
     const float rho = a.th.p[0][m], sz = a.th.p[1][m], mub = a.th.p[2][m], sgb = a.th.p[3][m];
     float zt = a.in.a[0][g], mu = a.in.a[1][g], var = a.in.a[2][g], lw = a.in.a[3][g];
-    const float zn = normal(key(a.kz, (uint64_t)g), (uint32_t)(a.t + 1));
-    a.zn[(long long)m * a.zrow + (long long)(a.t + 1) * NI + tid] = zn;
+    const float zn = nstore<H>(a.zn, (long long)m * a.zrow + (long long)(a.t + 1) * NI + tid,
+                               normal(key(a.kz, (uint64_t)g), (uint32_t)(a.t + 1)));
     if (tid == 0) {
-        const float n = normal(key(a.ku, (uint64_t)m), (uint32_t)(a.t + 1));
-        a.un[(long long)m * a.ucap1 + a.t + 1] = n;
+        const float n = nstore<H>(a.un, (long long)m * a.ucap1 + a.t + 1,
+                                  normal(key(a.ku, (uint64_t)m), (uint32_t)(a.t + 1)));
         L.u0 = u0_of(n);
         if (m == 0) a.y_hist[a.t] = a.y;
     }
@@ -338,10 +361,10 @@ __global__ __launch_bounds__(NI) void k_rb_step(RbStepArgs a) {
 struct RbMoveArgs {
     Theta th;
     Inner in;
-    const float* zc;
-    float* zo;
-    const float* uc;
-    float* uo;
+    const void* zc;
+    void* zo;
+    const void* uc;
+    void* uo;
     const float* y;
     int* acc;
     Inner cp;                 // checkpoint state (used when t_cp >= 0)
@@ -366,7 +389,7 @@ struct RbMoveArgs {
 // the fixed-lag checkpoint; lane 0 accepts with probability
 // min(1, exp(log-likelihood and prior difference)).  An accepted move
 // overwrites theta, its log-likelihood, inner ESS and inner state.
-template <int NI>
+template <int NI, bool H>
 __global__ __launch_bounds__(NI) void k_rb_move(RbMoveArgs a) {
     __shared__ Lds<NI> L;
     __shared__ float s_th[kParams];
@@ -418,8 +441,7 @@ __global__ __launch_bounds__(NI) void k_rb_move(RbMoveArgs a) {
         lw = a.cp.a[3][g];
         t0 = a.t_cp + 1;
     } else {
-        const float zp = a.crho * a.zc[zb + tid] + a.cscale * normal(kz, 0u);
-        a.zo[zb + tid] = zp;
+        const float zp = nstore<H>(a.zo, zb + tid, a.crho * nload<H>(a.zc, zb + tid) + a.cscale * normal(kz, 0u));
         init_particle(rho, sz, mub, sgb, zp, a.md, zt, mu, var);
         lw = a.md.lw0;
         t0 = 0;
@@ -427,11 +449,10 @@ __global__ __launch_bounds__(NI) void k_rb_move(RbMoveArgs a) {
     double llacc = 0.0, ess = (double)NI;
     for (int t = t0; t <= a.t_cur; ++t) {
         const long long zi = zb + (long long)(t + 1) * NI + tid;
-        const float zp = a.crho * a.zc[zi] + a.cscale * normal(kz, (uint32_t)(t + 1));
-        a.zo[zi] = zp;
+        const float zp = nstore<H>(a.zo, zi, a.crho * nload<H>(a.zc, zi) + a.cscale * normal(kz, (uint32_t)(t + 1)));
         if (tid == 0) {
-            const float up = a.crho * a.uc[ub + t + 1] + a.cscale * normal(ku, (uint32_t)(t + 1));
-            a.uo[ub + t + 1] = up;
+            const float up = nstore<H>(a.uo, ub + t + 1,
+                                       a.crho * nload<H>(a.uc, ub + t + 1) + a.cscale * normal(ku, (uint32_t)(t + 1)));
             L.u0 = u0_of(up);
         }
         __syncthreads();
@@ -462,16 +483,18 @@ __global__ __launch_bounds__(NI) void k_rb_move(RbMoveArgs a) {
     }
 }
 
-// dst row m <- src row anc[m] (row length len, float4 when VEC).
-template <bool VEC>
-__global__ void k_rows_gather(const float* __restrict__ src, float* __restrict__ dst,
+// dst row m <- src row anc[m] (row length len elements; 16-byte vectors
+// when VEC).
+template <class T, bool VEC>
+__global__ void k_rows_gather(const T* __restrict__ src, T* __restrict__ dst,
                               const int* __restrict__ anc, long long stride, long long len) {
+    constexpr int W = 16 / (int)sizeof(T);
     const int m = blockIdx.y;
     const long long s = (long long)anc[m] * stride, d = (long long)m * stride;
     const long long step = (long long)gridDim.x * blockDim.x;
     if constexpr (VEC) {
-        const long long n4 = len / 4;
-        for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < n4; j += step)
+        const long long nv = len / W;
+        for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < nv; j += step)
             reinterpret_cast<float4*>(dst + d)[j] = reinterpret_cast<const float4*>(src + s)[j];
     } else {
         for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < len; j += step)
@@ -481,17 +504,18 @@ __global__ void k_rows_gather(const float* __restrict__ src, float* __restrict__
 
 // dst row m <- src row m over [off, off + len) where acc[m] (commit of the
 // accepted proposals' noise).
-template <bool VEC>
-__global__ void k_rows_commit(const float* __restrict__ src, float* __restrict__ dst,
+template <class T, bool VEC>
+__global__ void k_rows_commit(const T* __restrict__ src, T* __restrict__ dst,
                               const int* __restrict__ acc, long long stride, long long off,
                               long long len) {
+    constexpr int W = 16 / (int)sizeof(T);
     const int m = blockIdx.y;
     if (!acc[m]) return;
     const long long b = (long long)m * stride + off;
     const long long step = (long long)gridDim.x * blockDim.x;
     if constexpr (VEC) {
-        const long long n4 = len / 4;
-        for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < n4; j += step)
+        const long long nv = len / W;
+        for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < nv; j += step)
             reinterpret_cast<float4*>(dst + b)[j] = reinterpret_cast<const float4*>(src + b)[j];
     } else {
         for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < len; j += step)
@@ -525,8 +549,9 @@ struct SMC2StateCUDA {
     int tb = 0;
     Inner in[2] = {};
     int ib = 0;
-    float* zn[2] = {nullptr, nullptr};
-    float* un[2] = {nullptr, nullptr};
+    void* zn[2] = {nullptr, nullptr};   // noise histories, elements of es bytes
+    void* un[2] = {nullptr, nullptr};
+    int es = 4;                          // 4: FP32, 2: IEEE half
     int nb = 0;
     int cap = 0;
     Inner cp[2] = {};
@@ -685,16 +710,17 @@ void initial_chol(SMC2StateCUDA* s) {
             s->chol[i * kParams + j] = (i == j) ? (float)((double)s->pstd[i] * sc) : 0.0f;
 }
 
-int alloc_noise(SMC2StateCUDA* s, int cap, float* z[2], float* u[2]) {
-    const size_t nz = (size_t)s->nt * (size_t)(cap + 1) * (size_t)s->ni;
-    const size_t nu = (size_t)s->nt * (size_t)(cap + 1);
+int alloc_noise(SMC2StateCUDA* s, int cap, void* z[2], void* u[2]) {
+    const size_t nz = (size_t)s->nt * (size_t)(cap + 1) * (size_t)s->ni * (size_t)s->es;
+    const size_t nu = (size_t)s->nt * (size_t)(cap + 1) * (size_t)s->es;
     for (int b = 0; b < 2; ++b) {
-        if (dmalloc(&z[b], nz) || dmalloc(&u[b], nu)) return -1;
+        HIP_TRY(hipMalloc(&z[b], nz));
+        HIP_TRY(hipMalloc(&u[b], nu));
     }
     return 0;
 }
 
-void free_noise(float* z[2], float* u[2]) {
+void free_noise(void* z[2], void* u[2]) {
     for (int b = 0; b < 2; ++b) {
         if (z[b]) (void)hipFree(z[b]);
         if (u[b]) (void)hipFree(u[b]);
@@ -703,19 +729,18 @@ void free_noise(float* z[2], float* u[2]) {
 }
 
 int grow_noise(SMC2StateCUDA* s, int cap) {
-    float* z[2] = {nullptr, nullptr};
-    float* u[2] = {nullptr, nullptr};
+    void* z[2] = {nullptr, nullptr};
+    void* u[2] = {nullptr, nullptr};
     if (alloc_noise(s, cap, z, u)) {
         free_noise(z, u);
         return -1;
     }
     // keep every theta's rows of the current buffer (strides change)
-    const size_t ni = (size_t)s->ni;
-    HIP_TRY(hipMemcpy2DAsync(z[s->nb], (size_t)(cap + 1) * ni * 4, s->zn[s->nb],
-                             (size_t)(s->cap + 1) * ni * 4, (size_t)(s->cap + 1) * ni * 4,
-                             (size_t)s->nt, hipMemcpyDeviceToDevice, s->st));
-    HIP_TRY(hipMemcpy2DAsync(u[s->nb], (size_t)(cap + 1) * 4, s->un[s->nb], (size_t)(s->cap + 1) * 4,
-                             (size_t)(s->cap + 1) * 4, (size_t)s->nt, hipMemcpyDeviceToDevice, s->st));
+    const size_t zr = (size_t)s->ni * (size_t)s->es, ur = (size_t)s->es;
+    HIP_TRY(hipMemcpy2DAsync(z[s->nb], (size_t)(cap + 1) * zr, s->zn[s->nb], (size_t)(s->cap + 1) * zr,
+                             (size_t)(s->cap + 1) * zr, (size_t)s->nt, hipMemcpyDeviceToDevice, s->st));
+    HIP_TRY(hipMemcpy2DAsync(u[s->nb], (size_t)(cap + 1) * ur, s->un[s->nb], (size_t)(s->cap + 1) * ur,
+                             (size_t)(s->cap + 1) * ur, (size_t)s->nt, hipMemcpyDeviceToDevice, s->st));
     HIP_TRY(hipStreamSynchronize(s->st));
     free_noise(s->zn, s->un);
     for (int b = 0; b < 2; ++b) {
@@ -747,7 +772,7 @@ int ensure_y(SMC2StateCUDA* s, int need) {
     return 0;
 }
 
-template <int NI>
+template <int NI, bool H>
 int launch_init_t(SMC2StateCUDA* s) {
     RbInitArgs a{};
     a.th = s->th[s->tb];
@@ -762,12 +787,12 @@ int launch_init_t(SMC2StateCUDA* s) {
     a.prior = s->prior;
     a.bounds = s->bounds;
     a.md = s->md;
-    hipLaunchKernelGGL(k_rb_init<NI>, dim3(s->nt), dim3(NI), 0, s->st, a);
+    hipLaunchKernelGGL((k_rb_init<NI, H>), dim3(s->nt), dim3(NI), 0, s->st, a);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
-template <int NI>
+template <int NI, bool H>
 int launch_step_t(SMC2StateCUDA* s, float y) {
     RbStepArgs a{};
     a.th = s->th[s->tb];
@@ -783,12 +808,12 @@ int launch_step_t(SMC2StateCUDA* s, float y) {
     a.t = s->t_cur;
     a.y = y;
     a.md = s->md;
-    hipLaunchKernelGGL(k_rb_step<NI>, dim3(s->nt), dim3(NI), 0, s->st, a);
+    hipLaunchKernelGGL((k_rb_step<NI, H>), dim3(s->nt), dim3(NI), 0, s->st, a);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
-template <int NI>
+template <int NI, bool H>
 int launch_move_t(SMC2StateCUDA* s, int k, bool use_cp) {
     RbMoveArgs a{};
     a.th = s->th[s->tb];
@@ -815,24 +840,29 @@ int launch_move_t(SMC2StateCUDA* s, int k, bool use_cp) {
     a.prior = s->run_prior;
     a.bounds = s->run_bounds;
     a.md = s->md;
-    hipLaunchKernelGGL(k_rb_move<NI>, dim3(s->nt), dim3(NI), 0, s->st, a);
+    hipLaunchKernelGGL((k_rb_move<NI, H>), dim3(s->nt), dim3(NI), 0, s->st, a);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
-#define RB_DISPATCH(ni, fn, ...)                        \
-    switch (ni) {                                       \
-        case 64: return fn<64>(__VA_ARGS__);            \
-        case 128: return fn<128>(__VA_ARGS__);          \
-        case 256: return fn<256>(__VA_ARGS__);          \
-        case 512: return fn<512>(__VA_ARGS__);          \
-        case 1024: return fn<1024>(__VA_ARGS__);        \
-        default: set_err("N_inner %d", ni); return -1;  \
+#define RB_DISPATCH_H(ni, H, fn, ...)                      \
+    switch (ni) {                                          \
+        case 64: return fn<64, H>(__VA_ARGS__);            \
+        case 128: return fn<128, H>(__VA_ARGS__);          \
+        case 256: return fn<256, H>(__VA_ARGS__);          \
+        case 512: return fn<512, H>(__VA_ARGS__);          \
+        case 1024: return fn<1024, H>(__VA_ARGS__);        \
+        default: set_err("N_inner %d", ni); return -1;     \
     }
+#define RB_DISPATCH(s, fn, ...)                                      \
+    do {                                                             \
+        if ((s)->es == 2) { RB_DISPATCH_H((s)->ni, true, fn, __VA_ARGS__) } \
+        RB_DISPATCH_H((s)->ni, false, fn, __VA_ARGS__)               \
+    } while (0)
 
-int launch_init(SMC2StateCUDA* s) { RB_DISPATCH(s->ni, launch_init_t, s) }
-int launch_step(SMC2StateCUDA* s, float y) { RB_DISPATCH(s->ni, launch_step_t, s, y) }
-int launch_move(SMC2StateCUDA* s, int k, bool cp) { RB_DISPATCH(s->ni, launch_move_t, s, k, cp) }
+int launch_init(SMC2StateCUDA* s) { RB_DISPATCH(s, launch_init_t, s); }
+int launch_step(SMC2StateCUDA* s, float y) { RB_DISPATCH(s, launch_step_t, s, y); }
+int launch_move(SMC2StateCUDA* s, int k, bool cp) { RB_DISPATCH(s, launch_move_t, s, k, cp); }
 
 dim3 row_grid(long long len, int rows) {
     long long b = (len / 4 + 255) / 256;
@@ -841,22 +871,50 @@ dim3 row_grid(long long len, int rows) {
     return dim3((unsigned)b, (unsigned)rows);
 }
 
-int gather_rows(SMC2StateCUDA* s, const float* src, float* dst, long long stride, long long len) {
+template <class T>
+int gather_rows_t(SMC2StateCUDA* s, const void* src, void* dst, long long stride, long long len) {
     if (len <= 0) return 0;
-    const bool vec = (stride % 4 == 0) && (len % 4 == 0);
-    if (vec) hipLaunchKernelGGL(k_rows_gather<true>, row_grid(len, s->nt), dim3(256), 0, s->st, src, dst, s->d_anc, stride, len);
-    else hipLaunchKernelGGL(k_rows_gather<false>, row_grid(len * 4, s->nt), dim3(256), 0, s->st, src, dst, s->d_anc, stride, len);
+    constexpr int W = 16 / (int)sizeof(T);
+    const T* a = static_cast<const T*>(src);
+    T* b = static_cast<T*>(dst);
+    if (stride % W == 0 && len % W == 0)
+        hipLaunchKernelGGL((k_rows_gather<T, true>), row_grid(len * sizeof(T) / 4, s->nt), dim3(256), 0, s->st, a, b,
+                           s->d_anc, stride, len);
+    else
+        hipLaunchKernelGGL((k_rows_gather<T, false>), row_grid(len * 4, s->nt), dim3(256), 0, s->st, a, b, s->d_anc,
+                           stride, len);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
-int commit_rows(SMC2StateCUDA* s, const float* src, float* dst, long long stride, long long off, long long len) {
+template <class T>
+int commit_rows_t(SMC2StateCUDA* s, const void* src, void* dst, long long stride, long long off, long long len) {
     if (len <= 0) return 0;
-    const bool vec = (stride % 4 == 0) && (off % 4 == 0) && (len % 4 == 0);
-    if (vec) hipLaunchKernelGGL(k_rows_commit<true>, row_grid(len, s->nt), dim3(256), 0, s->st, src, dst, s->d_acc, stride, off, len);
-    else hipLaunchKernelGGL(k_rows_commit<false>, row_grid(len * 4, s->nt), dim3(256), 0, s->st, src, dst, s->d_acc, stride, off, len);
+    constexpr int W = 16 / (int)sizeof(T);
+    const T* a = static_cast<const T*>(src);
+    T* b = static_cast<T*>(dst);
+    if (stride % W == 0 && off % W == 0 && len % W == 0)
+        hipLaunchKernelGGL((k_rows_commit<T, true>), row_grid(len * sizeof(T) / 4, s->nt), dim3(256), 0, s->st, a, b,
+                           s->d_acc, stride, off, len);
+    else
+        hipLaunchKernelGGL((k_rows_commit<T, false>), row_grid(len * 4, s->nt), dim3(256), 0, s->st, a, b,
+                           s->d_acc, stride, off, len);
     HIP_TRY(hipGetLastError());
     return 0;
+}
+
+// rows of 4-byte elements (inner states, checkpoint log-likelihood halves)
+int gather_rows(SMC2StateCUDA* s, const void* src, void* dst, long long stride, long long len) {
+    return gather_rows_t<float>(s, src, dst, stride, len);
+}
+// rows of noise-history elements
+int gather_noise(SMC2StateCUDA* s, const void* src, void* dst, long long stride, long long len) {
+    return s->es == 2 ? gather_rows_t<uint16_t>(s, src, dst, stride, len)
+                      : gather_rows_t<float>(s, src, dst, stride, len);
+}
+int commit_noise(SMC2StateCUDA* s, const void* src, void* dst, long long stride, long long off, long long len) {
+    return s->es == 2 ? commit_rows_t<uint16_t>(s, src, dst, stride, off, len)
+                      : commit_rows_t<float>(s, src, dst, stride, off, len);
 }
 
 int download_theta(SMC2StateCUDA* s) {
@@ -896,8 +954,8 @@ int resample_and_move(SMC2StateCUDA* s) {
     for (int q = 0; q < 4; ++q)
         if (gather_rows(s, s->in[s->ib].a[q], s->in[1 - s->ib].a[q], ni, ni)) return -1;
     const long long zrow = (long long)(s->cap + 1) * ni;
-    if (gather_rows(s, s->zn[s->nb], s->zn[1 - s->nb], zrow, (long long)(t + 2) * ni)) return -1;
-    if (gather_rows(s, s->un[s->nb], s->un[1 - s->nb], s->cap + 1, t + 2)) return -1;
+    if (gather_noise(s, s->zn[s->nb], s->zn[1 - s->nb], zrow, (long long)(t + 2) * ni)) return -1;
+    if (gather_noise(s, s->un[s->nb], s->un[1 - s->nb], s->cap + 1, t + 2)) return -1;
     s->tb = 1 - s->tb;
     s->ib = 1 - s->ib;
     s->nb = 1 - s->nb;
@@ -923,9 +981,9 @@ int resample_and_move(SMC2StateCUDA* s) {
     const long long t_start = use_cp ? (long long)s->t_cp + 1 : 0;
     for (int k = 0; k < s->K; ++k) {
         if (launch_move(s, k, use_cp)) return -1;
-        if (commit_rows(s, s->zn[1 - s->nb], s->zn[s->nb], zrow, t_start * ni, ((long long)t + 2 - t_start) * ni))
+        if (commit_noise(s, s->zn[1 - s->nb], s->zn[s->nb], zrow, t_start * ni, ((long long)t + 2 - t_start) * ni))
             return -1;
-        if (commit_rows(s, s->un[1 - s->nb], s->un[s->nb], s->cap + 1, t_start, (long long)t + 2 - t_start))
+        if (commit_noise(s, s->un[1 - s->nb], s->un[s->nb], s->cap + 1, t_start, (long long)t + 2 - t_start))
             return -1;
         HIP_TRY(hipMemcpyAsync(s->h_acc, s->d_acc, (size_t)n * 4, hipMemcpyDeviceToHost, s->st));
         HIP_TRY(hipStreamSynchronize(s->st));
@@ -1198,6 +1256,26 @@ int smc2_cuda_set_rejuv(SMC2StateCUDA* state, int K_rejuv, float ess_threshold_o
     }
     state->K = K_rejuv;
     state->ess_thr = ess_threshold_outer;
+    return 0;
+}
+
+int smc2_cuda_set_noise_precision(SMC2StateCUDA* state, int bits) {
+    clear_err();
+    if (!state || (bits != 16 && bits != 32)) {
+        set_err("smc2_cuda_set_noise_precision: bits must be 16 or 32");
+        return -1;
+    }
+    SMC2StateCUDA* s = state;
+    const int es = bits / 8;
+    if (es == s->es) return 0;
+    if (hipStreamSynchronize(s->st) != hipSuccess) {
+        set_err("smc2_cuda_set_noise_precision: %s", hipGetErrorString(hipGetLastError()));
+        return -1;
+    }
+    free_noise(s->zn, s->un);
+    s->es = es;
+    if (alloc_noise(s, s->cap, s->zn, s->un)) return -1;
+    s->inited = false;       // the histories restart with the next init
     return 0;
 }
 

@@ -6,7 +6,8 @@ The calls follow the reference host API one for one
 set_noise_capacity / set_fixed_lag / set_cpmmh_rho / set_proposal_std /
 init_from_prior / update / get_theta_mean / get_theta_std / get_outer_ess);
 the reference's struct-field access is replaced by the set_model / get_model /
-set_rejuv / set_adaptive / get_counters / get_particles / get_inner calls.
+set_rejuv / set_adaptive / get_counters / get_particles / get_inner calls, and
+its compile-time SMC2_NOISE_FP16 by set_noise_precision (noise_bits=16).
 The compute path is the HIP library _build/libgpubpf.so; there is no other.
 """
 import ctypes
@@ -72,6 +73,7 @@ def load():
         "smc2_cuda_get_model": (I, [P, P, P, P, P]),
         "smc2_cuda_set_rejuv": (I, [P, I, F]),
         "smc2_cuda_set_adaptive": (I, [P, I]),
+        "smc2_cuda_set_noise_precision": (I, [P, I]),
         "smc2_cuda_get_counters": (I, [P, ctypes.POINTER(SMC2Counters)]),
         "smc2_cuda_get_particles": (I, [P, P, P, P, P, P, P, P]),
         "smc2_cuda_get_inner": (I, [P, P, P, P, P]),
@@ -99,7 +101,7 @@ class Smc2Rbpf:
     def __init__(self, n_theta, n_inner, seed=12345, noise_capacity=None, fixed_lag=0,
                  cpmmh_rho=None, proposal_std=None, K_rejuv=None, ess_threshold=None,
                  adaptive=True, prior=None, bounds=None, theta_curve=None, fixed_curves=None,
-                 init=True):
+                 noise_bits=32, init=True):
         self._L = load()
         self._s = self._L.smc2_cuda_alloc(n_theta, n_inner)
         if not self._s:
@@ -121,6 +123,8 @@ class Smc2Rbpf:
                 raise _err("smc2_cuda_set_rejuv")
         if L.smc2_cuda_set_adaptive(s, 1 if adaptive else 0) != 0:
             raise _err("smc2_cuda_set_adaptive")
+        if L.smc2_cuda_set_noise_precision(s, noise_bits) != 0:
+            raise _err("smc2_cuda_set_noise_precision")
         if any(v is not None for v in (prior, bounds, theta_curve, fixed_curves)):
             args = [SVPrior(*prior) if prior is not None else None,
                     SVBounds(*bounds) if bounds is not None else None,

@@ -125,6 +125,12 @@ int smc2_cuda_get_model(SMC2StateCUDA* state, SVPrior* prior, SVBounds* bounds,
                         SVCurve* theta_curve, SVFixedCurves* fixed_curves);
 /* state->K_rejuv (>= 0) and state->ess_threshold_outer (fraction of N_theta). */
 int smc2_cuda_set_rejuv(SMC2StateCUDA* state, int K_rejuv, float ess_threshold_outer);
+/* Compile-time SMC2_NOISE_FP16 of smc2/smc2_noise_precision.cuh as a setting:
+ * 16 stores the noise histories as IEEE half (half the memory and copy
+ * traffic; every noise is used as stored, rounded to nearest even), 32 (the
+ * default) as FP32.  Reallocates the histories; call smc2_cuda_init_from_prior
+ * afterwards. */
+int smc2_cuda_set_noise_precision(SMC2StateCUDA* state, int bits);
 /* state->use_adaptive_proposals (default on): 0 keeps the initial diagonal
  * proposal diag(proposal_std) * 2.38 / sqrt(4) instead of the scaled
  * Cholesky factor of the theta-particles' covariance. */

@@ -203,7 +203,7 @@ double orc_grid_loglik(const float* y, int n_ticks, double rho, double sigma_z,
 #endif
 /* ---- SMC^2 with the RBPF / OCSN inner filter (rbpf_oracle.c) ---------- */
 typedef struct {
-    int n_theta, n_inner, K_rejuv, lag, cap, adaptive;
+    int n_theta, n_inner, K_rejuv, lag, cap, adaptive, noise_bits;   /* noise_bits 32 or 16 */
     float ess_thr, crho;
     uint64_t seed;              /* nonzero */
     float pstd[4];
@@ -225,5 +225,7 @@ int orc_rb_inner_run(const OrcRbCfg* c, const float* theta, const float* y, int 
 void orc_rb_ocsn(float y, float mp, float vp, float* out3);
 void orc_rb_math(int fn, const float* in, float* out, int n);
 void orc_rb_ocsn_table(float* p, float* m, float* v);
+/* float -> IEEE half (round to nearest even) -> float, element-wise */
+void orc_rb_half_roundtrip(const float* in, float* out, int n);
 
 #endif

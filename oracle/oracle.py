@@ -73,6 +73,7 @@ TICK_DTYPE = np.dtype(
 class OrcRbCfg(ctypes.Structure):
     _fields_ = [("n_theta", ctypes.c_int), ("n_inner", ctypes.c_int), ("K_rejuv", ctypes.c_int),
                 ("lag", ctypes.c_int), ("cap", ctypes.c_int), ("adaptive", ctypes.c_int),
+                ("noise_bits", ctypes.c_int),
                 ("ess_thr", ctypes.c_float), ("crho", ctypes.c_float), ("seed", ctypes.c_uint64),
                 ("pstd", ctypes.c_float * 4), ("prior", ctypes.c_float * 8),
                 ("bounds", ctypes.c_float * 8), ("tcurve", ctypes.c_float * 3),
@@ -166,6 +167,8 @@ def lib():
         L.orc_rb_ocsn.argtypes = [f32, f32, f32, P]
         L.orc_rb_math.restype = None
         L.orc_rb_math.argtypes = [i32, P, P, i32]
+        L.orc_rb_half_roundtrip.restype = None
+        L.orc_rb_half_roundtrip.argtypes = [P, P, i32]
         L.orc_rb_ocsn_table.restype = None
         L.orc_rb_ocsn_table.argtypes = [P, P, P]
         L.orc_resample_ancestors.restype = None
@@ -461,8 +464,9 @@ def grid_loglik(y, rho=0.97, sigma_z=0.15, mu=-1.0, n_grid=2048):
 
 def rb_cfg(n_theta, n_inner, seed=12345, K_rejuv=5, lag=0, cap=None, ess_thr=0.5, crho=0.99,
            adaptive=True, pstd=RB_PSTD, prior=RB_PRIOR, bounds=RB_BOUNDS, tcurve=RB_TCURVE,
-           fixed=RB_FIXED, T=0):
+           fixed=RB_FIXED, T=0, noise_bits=32):
     c = OrcRbCfg()
+    c.noise_bits = noise_bits
     c.n_theta, c.n_inner, c.K_rejuv, c.lag = n_theta, n_inner, K_rejuv, lag
     c.cap = cap if cap is not None else max(2048, T)
     c.adaptive = 1 if adaptive else 0
@@ -556,3 +560,10 @@ def rb_dgp(T, rho=0.95, sigma_z=0.10, mu_base=-1.0, sigma_base=0.15, fixed=RB_FI
         r2 = max((np.exp(h / 2.0) * rng.standard_normal()) ** 2, 1e-30)
         y[t], hs[t], zs[t] = np.log(r2), h, z
     return y.astype(np.float32), hs, zs
+
+
+def rb_half_roundtrip(x):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(x)
+    lib().orc_rb_half_roundtrip(_ptr(x), _ptr(out), len(x))
+    return out

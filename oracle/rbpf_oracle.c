@@ -80,6 +80,56 @@ static inline float runif(uint64_t k, uint32_t ctr) {
 static inline float rnorm(uint64_t k, uint32_t ctr) { return orc_icdf_f32(runif(k, ctr)); }
 enum { P_PRIOR = 1, P_Z = 2, P_U0 = 3, P_MOVE = 4, P_FZ = 5, P_FU = 6 };
 
+/* ---- noise storage ------------------------------------------------------ */
+/* float -> IEEE binary16 with round-to-nearest-even (overflow to inf, NaN
+ * kept, subnormal halves) -> float: the value a half-precision noise history
+ * holds (SMC2_NOISE_FP16 of smc2/smc2_noise_precision.cuh). */
+static float half_rt(float x) {
+    uint32_t b;
+    memcpy(&b, &x, 4);
+    const uint32_t sign = (b >> 16) & 0x8000u;
+    const uint32_t ab = b & 0x7FFFFFFFu;
+    uint32_t h;
+    if (ab > 0x7F800000u) {
+        h = sign | 0x7E00u;
+    } else if (ab >= 0x47800000u) {               /* >= 65536: infinity */
+        h = sign | 0x7C00u;
+    } else if (ab >= 0x38800000u) {               /* normal half range, >= 2^-14 */
+        const uint32_t e = (ab >> 23) - 127u + 15u;
+        const uint32_t mant = ab & 0x7FFFFFu;
+        uint32_t hv = (e << 10) | (mant >> 13);
+        const uint32_t rem = mant & 0x1FFFu;
+        if (rem > 0x1000u || (rem == 0x1000u && (hv & 1u))) hv += 1u;   /* may carry to inf */
+        h = sign | hv;
+    } else {                                      /* subnormal half or zero: units of 2^-24 */
+        float f;
+        memcpy(&f, &ab, 4);
+        h = sign | (uint32_t)rintf(f * 16777216.0f);
+    }
+    const uint32_t e = (h >> 10) & 31u, m = h & 0x3FFu;
+    uint32_t ob;
+    if (e == 0u) {
+        float v = (float)m * (1.0f / 16777216.0f);
+        memcpy(&ob, &v, 4);
+        ob |= sign << 16;
+    } else if (e == 31u) {
+        ob = (sign << 16) | 0x7F800000u | (m << 13);
+    } else {
+        ob = (sign << 16) | ((e - 15u + 127u) << 23) | (m << 13);
+    }
+    float o;
+    memcpy(&o, &ob, 4);
+    return o;
+}
+
+void orc_rb_half_roundtrip(const float* in, float* out, int n) {
+    for (int i = 0; i < n; ++i) out[i] = half_rt(in[i]);
+}
+
+/* the value a noise history stores and the filter uses */
+static int g_noise16 = 0;
+static inline float nstore(float v) { return g_noise16 ? half_rt(v) : v; }
+
 /* ---- model functions (FP32) ------------------------------------------- */
 static inline float rb_tanh(float x) {
     float ax = fabsf(x);
@@ -450,10 +500,10 @@ static void rb_init(RbState* S, const OrcRbCfg* c, const RbModel* md) {
         for (int q = 0; q < RB_NP; ++q) S->th[0][q][m] = th[q];
         S->ll[0][m] = 0.0;
         S->essi[0][m] = (float)ni;
-        S->un[0][(long long)m * (S->cap + 1)] = rnorm(rkey(ku, (uint64_t)m), 0);
+        S->un[0][(long long)m * (S->cap + 1)] = nstore(rnorm(rkey(ku, (uint64_t)m), 0));
         for (int i = 0; i < ni; ++i) {
             const long long g = (long long)m * ni + i;
-            const float n0 = rnorm(rkey(kz, (uint64_t)g), 0);
+            const float n0 = nstore(rnorm(rkey(kz, (uint64_t)g), 0));
             S->zn[0][m * zrow + i] = n0;
             rb_init_particle(th[0], th[1], th[2], th[3], n0, md, &S->in[0][0][g], &S->in[0][1][g],
                              &S->in[0][2][g]);
@@ -474,8 +524,9 @@ static void rb_step(RbState* S, const OrcRbCfg* c, const RbModel* md, int t, flo
         RbScratch sc;
         scratch_alloc(&sc, ni);
         float* zn = S->zn[S->nb] + m * zrow + (long long)(t + 1) * ni;
-        for (int i = 0; i < ni; ++i) zn[i] = rnorm(rkey(kz, (uint64_t)((long long)m * ni + i)), (uint32_t)(t + 1));
-        const float n = rnorm(rkey(ku, (uint64_t)m), (uint32_t)(t + 1));
+        for (int i = 0; i < ni; ++i)
+            zn[i] = nstore(rnorm(rkey(kz, (uint64_t)((long long)m * ni + i)), (uint32_t)(t + 1)));
+        const float n = nstore(rnorm(rkey(ku, (uint64_t)m), (uint32_t)(t + 1)));
         S->un[S->nb][(long long)m * (S->cap + 1) + t + 1] = n;
         const float u0 = rb_u0(n);
         float th[RB_NP];
@@ -547,7 +598,7 @@ static void rb_move(RbState* S, const OrcRbCfg* c, const RbModel* md, const floa
             t0 = t_cp + 1;
         } else {
             for (int i = 0; i < ni; ++i) {
-                const float v = crho * zc[zb + i] + cscale * rnorm(rkey(kfz, (uint64_t)(g0 + i)), 0);
+                const float v = nstore(crho * zc[zb + i] + cscale * rnorm(rkey(kfz, (uint64_t)(g0 + i)), 0));
                 zo[zb + i] = v;
                 rb_init_particle(tp[0], tp[1], tp[2], tp[3], v, md, &zt[i], &mu[i], &var[i]);
                 lw[i] = md->lw0;
@@ -559,10 +610,10 @@ static void rb_move(RbState* S, const OrcRbCfg* c, const RbModel* md, const floa
         for (int t = t0; t <= t_cur; ++t) {
             const long long zi = zb + (long long)(t + 1) * ni;
             for (int i = 0; i < ni; ++i) {
-                zp[i] = crho * zc[zi + i] + cscale * rnorm(rkey(kfz, (uint64_t)(g0 + i)), (uint32_t)(t + 1));
+                zp[i] = nstore(crho * zc[zi + i] + cscale * rnorm(rkey(kfz, (uint64_t)(g0 + i)), (uint32_t)(t + 1)));
                 zo[zi + i] = zp[i];
             }
-            const float up = crho * uc[ub + t + 1] + cscale * rnorm(ku, (uint32_t)(t + 1));
+            const float up = nstore(crho * uc[ub + t + 1] + cscale * rnorm(ku, (uint32_t)(t + 1)));
             uo[ub + t + 1] = up;
             double l;
             rb_tick(ni, zt, mu, var, lw, zp, rb_u0(up), y[t], t, tp, md, &sc, &l, &ess);
@@ -605,6 +656,8 @@ int orc_rb_smc2_run(const OrcRbCfg* c, const float* y, int T, float* ess_out, fl
                     int* resample_ticks, OrcRbCount* cnt) {
     const int nt = c->n_theta, ni = c->n_inner;
     if (nt < 2 || ni < 64 || ni % 64 || T < 0 || c->cap < T) return -1;
+    if (c->noise_bits != 16 && c->noise_bits != 32) return -1;
+    g_noise16 = c->noise_bits == 16;
     RbModel md;
     rb_model(c, &md);
     RbState S;
@@ -724,6 +777,8 @@ int orc_rb_inner_run(const OrcRbCfg* c, const float* theta, const float* y, int 
                      double* ll_incr, double* ess, float* state_out) {
     const int ni = c->n_inner;
     if (ni < 64 || ni % 64 || T < 0) return -1;
+    if (c->noise_bits != 16 && c->noise_bits != 32) return -1;
+    g_noise16 = c->noise_bits == 16;
     RbModel md;
     rb_model(c, &md);
     const uint64_t kz = rkey(rkey(c->seed, P_Z), 0), ku = rkey(rkey(c->seed, P_U0), 0);
@@ -735,13 +790,13 @@ int orc_rb_inner_run(const OrcRbCfg* c, const float* theta, const float* y, int 
     RbScratch sc;
     scratch_alloc(&sc, ni);
     for (int i = 0; i < ni; ++i) {
-        rb_init_particle(theta[0], theta[1], theta[2], theta[3], rnorm(rkey(kz, (uint64_t)i), 0), &md,
+        rb_init_particle(theta[0], theta[1], theta[2], theta[3], nstore(rnorm(rkey(kz, (uint64_t)i), 0)), &md,
                          &zt[i], &mu[i], &var[i]);
         lw[i] = md.lw0;
     }
     for (int t = 0; t < T; ++t) {
-        for (int i = 0; i < ni; ++i) zn[i] = rnorm(rkey(kz, (uint64_t)i), (uint32_t)(t + 1));
-        const float u0 = rb_u0(rnorm(rkey(ku, 0), (uint32_t)(t + 1)));
+        for (int i = 0; i < ni; ++i) zn[i] = nstore(rnorm(rkey(kz, (uint64_t)i), (uint32_t)(t + 1)));
+        const float u0 = rb_u0(nstore(rnorm(rkey(ku, 0), (uint32_t)(t + 1))));
         double l, e;
         rb_tick(ni, zt, mu, var, lw, zn, u0, y[t], t, theta, &md, &sc, &l, &e);
         if (ll_incr) ll_incr[t] = l;

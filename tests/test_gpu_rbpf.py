@@ -22,9 +22,10 @@ def say(*a):
     print("[case]", *a, flush=True)
 
 
-def gpu_run(nt, ni, y, seed=12345, K=5, ess_thr=0.5, lag=0, crho=None, adaptive=True, pstd=None):
+def gpu_run(nt, ni, y, seed=12345, K=5, ess_thr=0.5, lag=0, crho=None, adaptive=True, pstd=None,
+            noise_bits=32):
     f = smc2_rbpf.Smc2Rbpf(nt, ni, seed=seed, fixed_lag=lag, cpmmh_rho=crho, proposal_std=pstd,
-                           K_rejuv=K, ess_threshold=ess_thr, adaptive=adaptive)
+                           K_rejuv=K, ess_threshold=ess_thr, adaptive=adaptive, noise_bits=noise_bits)
     ess = f.run(y)
     out = f.particles()
     out["ess"] = ess
@@ -36,10 +37,10 @@ def gpu_run(nt, ni, y, seed=12345, K=5, ess_thr=0.5, lag=0, crho=None, adaptive=
 
 
 def orc_run(orc, nt, ni, y, seed=12345, K=5, ess_thr=0.5, lag=0, crho=0.99, adaptive=True,
-            pstd=None):
+            pstd=None, noise_bits=32):
     cfg = orc.rb_cfg(nt, ni, seed=seed, K_rejuv=K, ess_thr=ess_thr, lag=lag,
                      crho=0.99 if crho is None else crho, adaptive=adaptive,
-                     pstd=orc.RB_PSTD if pstd is None else pstd, T=len(y))
+                     pstd=orc.RB_PSTD if pstd is None else pstd, T=len(y), noise_bits=noise_bits)
     return orc.run_rb_smc2(cfg, y)
 
 
@@ -181,3 +182,18 @@ def test_high_volatility_regime():
     assert sum(abs(m[q] - truth[q]) <= 2 * s[q] for q in range(4)) >= 2, (m, s)
     assert abs(m[2] - truth[2]) < 1.0
 
+
+
+@pytest.mark.parametrize("nt, ni, T, lag", [(16, 64, 48, 0), (8, 256, 40, 6)])
+def test_half_precision_noise_parity(orc, nt, ni, T, lag):
+    """SMC2_NOISE_FP16 (smc2/smc2_noise_precision.cuh) as noise_bits = 16:
+    histories stored as IEEE half, every noise used as stored; bit-exact with
+    the oracle's half rounding, through resampling, moves and checkpoints."""
+    say("rbpf fp16 noise", nt, ni, T, lag)
+    y, _, _ = orc.rb_dgp(T, *TRUTH, seed=nt + 1)
+    g = gpu_run(nt, ni, y, K=2, ess_thr=0.95, lag=lag, noise_bits=16)
+    o = orc_run(orc, nt, ni, y, K=2, ess_thr=0.95, lag=lag, noise_bits=16)
+    assert o["n_resamples"] >= 2
+    assert_parity(g, o)
+    f = gpu_run(nt, ni, y, K=2, ess_thr=0.95, lag=lag)        # FP32 histories differ
+    assert not np.array_equal(f["inner"], g["inner"])

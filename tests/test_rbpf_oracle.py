@@ -170,3 +170,18 @@ def test_outer_resampling_and_moves_happen(orc):
     assert r["n_resamples"] >= 1 and 0 < r["n_acc"] < r["n_tot"]
     assert r["t_cp"] == 70
     assert np.all(np.isfinite(r["ll"])) and np.all(r["ess_inner"] > 0)
+
+
+def test_half_rounding_is_ieee(orc):
+    """The oracle's half-precision noise store rounds as NumPy's float16 (IEEE
+    round to nearest even), including every halfway point and the overflow,
+    subnormal and special cases."""
+    rng = np.random.default_rng(0)
+    h = np.arange(0, 0x7C00, dtype=np.uint16).view(np.float16).astype(np.float64)
+    mid = ((h[:-1] + h[1:]) / 2).astype(np.float32)
+    x = np.concatenate([rng.standard_normal(200_000).astype(np.float32) * 4, mid, -mid,
+                        h.astype(np.float32), np.float32([65504, 65519.99, 65520, 1e6, -1e6, np.inf, -np.inf,
+                                                          0.0, -0.0, 5.96e-8, 2.98e-8, 1e-9])])
+    with np.errstate(over="ignore"):
+        ref = x.astype(np.float16).astype(np.float32)
+    assert np.array_equal(orc.rb_half_roundtrip(x), ref)
