@@ -503,6 +503,25 @@ def main():
         res["roofline"]["traffic_carry_tick"] = tr["traffic_bytes_per_launch_carry_tick"]
         res["roofline"]["algorithmic_bytes_per_launch"] = bytes_per_launch
         res["roofline"]["traffic_source"] = tr["source"]
+    # a plain device copy moving the same bytes as one propagate launch (half
+    # read, half written), timed the same way: the streaming rate this access
+    # pattern can reach on this GPU, beside the 8 TB/s peak
+    half = int(bytes_per_launch // 8)
+    src = torch.ones(half, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(20):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_bw = 8.0 * half * 20 / (e0.elapsed_time(e1) * 1e-3)
+    del src, dst
+    res["roofline"]["copy_same_bytes_GBps"] = copy_bw / 1e9
+    res["roofline"]["frac_of_copy"] = achieved / copy_bw
+    log("copy of the same bytes", copy_bw / 1e9, "GB/s")
     if not args.no_cpu:
         log("cpu baseline", args.cpu_ticks, "ticks x", n, "threads", args.cpu_threads)
         v, dt = cpu_baseline(y, mu_t, n, args.cpu_ticks, args.cpu_threads)
