@@ -299,7 +299,8 @@ def test_ess_threshold_semantics(orc, sv_data):
 
 def test_fp32_vs_fp64_oracle(orc, sv_data):
     """Arithmetic error of the FP32 pipeline: FP64 oracle on the same genealogy.
-    Loglik error <= 1e-4 relative; measured ~1e-7."""
+    Loglik error <= 1e-4 relative (the floating-point tolerance of the filter's
+    log-likelihood); measured ~1e-7."""
     _, y = sv_data
     for thr in (0.0, 0.5):
         o32 = orc.run_filter(y, 16384, ess_threshold=thr, threads=8)[0]
