@@ -1105,22 +1105,36 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
                         if (s_c[pad16(mid)] < t) lo = mid + 1; else hi = mid;
                     }
                     int j = lo;
+                    // c_j and c_(j+1) stay in registers: another child of j
+                    // reads no LDS, a child of j + 1 waits on none (the
+                    // refill of c_(j+2) is needed only by the next output)
+                    double cj = s_c[pad16(j)];
+                    double cn = s_c[pad16(min(j + 1, last))];
                     xd[ilo + o0] = s_x[pad16(j)];
                     for (int o = o0 + 1; o < o1; ++o) {
                         t = (double)target_unwrapped(u, ilo + o, n);
-                        if (s_c[pad16(j)] < t) {
-                            // the parent lies in (j, last]: gallop, then search
-                            int g0 = j + 1, g1 = g0, step = 1;
-                            while (g1 < last && s_c[pad16(g1)] < t) {
-                                g0 = g1 + 1;
-                                step <<= 1;
-                                g1 = min(g0 + step - 1, last);
+                        if (cj < t) {
+                            // t <= c_last, so j < last here
+                            if (t <= cn) {
+                                ++j;
+                                cj = cn;
+                            } else {
+                                // the parent lies in (j + 1, last]: gallop,
+                                // then search
+                                int g0 = j + 2, g1 = g0, step = 1;
+                                while (g1 < last && s_c[pad16(g1)] < t) {
+                                    g0 = g1 + 1;
+                                    step <<= 1;
+                                    g1 = min(g0 + step - 1, last);
+                                }
+                                while (g0 < g1) {
+                                    const int mid = (g0 + g1) >> 1;
+                                    if (s_c[pad16(mid)] < t) g0 = mid + 1; else g1 = mid;
+                                }
+                                j = g0;
+                                cj = s_c[pad16(j)];
                             }
-                            while (g0 < g1) {
-                                const int mid = (g0 + g1) >> 1;
-                                if (s_c[pad16(mid)] < t) g0 = mid + 1; else g1 = mid;
-                            }
-                            j = g0;
+                            cn = s_c[pad16(min(j + 1, last))];
                         }
                         xd[ilo + o] = s_x[pad16(j)];
                     }
