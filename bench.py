@@ -6,17 +6,27 @@ particles, T = 5,000 ticks of matched-DGP data (phi = 0.97, sigma_eta = 0.15,
 PCG32 DGP seed 42) whose mu switches -1.5 / -0.5 / +0.5 every 1,250 ticks (the
 fourth block repeats -1.5); the filter is told each switch through
 gpu_bpf_set_mu; resampling when ESS < 0.5 N; FP64 CDF scan and cumulative
-log-likelihood.  A step is one tick.  The observations sit in HBM before the
-timed region starts.
+log-likelihood.
+
+A step is one batch of --ticks-per-step observations (default 250), so the
+default 20 timed steps are the workload's T = 5,000 ticks from tick 0.  The
+warm-up steps run on the same filter instance, which is then restarted at
+tick 0 (gpu_bpf_restart, no allocation) before the timed region.  The
+observations sit in HBM before the timed region starts.
 
 Prints one JSON line: particle-ticks/s, the HBM roofline of the dominant
-kernel (propagate+weight), and the CPU oracle's throughput on a bounded
-sample of the same workload.  Under "workloads" the same line carries the
-other configurations (unless --no-extra): the always-resample N=65,536
-filter with its log-likelihood error against the FP64 oracle, the
-differentiable N=2^20 filter (forward + adjoint score), and the batched SMC^2
-runs (CPMMH, and MALA with Student-t observations) next to the plain filter
-looped over the same parameter sets.
+kernel (propagate+weight, HIP events on every tenth tick of the timed region),
+and the CPU oracle's throughput on a bounded sample of the same workload.
+Under "workloads" the same line carries the other configurations (unless
+--no-extra): the always-resample N=65,536 filter, the differentiable N=2^20
+filter (forward + adjoint score), the batched SMC^2 runs (CPMMH, and MALA with
+Student-t observations) next to the plain filter looped over the same
+parameter sets, each with its log-likelihood error against the FP64 oracle on
+a sample the oracle covers, and the RBPF SMC^2 learner.
+
+The observation series come from sv_data (the reference's data generator);
+the CPU oracle is used only by the CPU-baseline and error-check legs
+(cpu_* functions below).
 """
 import argparse
 import json
@@ -30,16 +40,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "bootstrap-particle-filter-in-ptx_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
+import sv_data  # noqa: E402
+
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 METRIC = "particle-ticks/s + %HBM roofline; plain vs batched/adjoint at fixed loglik err"
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
-
-
-def mu_schedule(T, block=1250, levels=(-1.5, -0.5, 0.5)):
-    return np.array([levels[(t // block) % 3] for t in range(T)], np.float32)
 
 
 def segments(mu_t):
@@ -61,18 +69,59 @@ def enqueue_run(f, L, y_dev_ptr, mu_t, t0, t1):
             raise RuntimeError(f.L.gpu_bpf_last_error().decode())
 
 
-def cpu_baseline(y, mu_t, n, ticks, threads):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(y, mu_t, n, ticks, ticks_1, threads):
+    """The oracle's FP32 filter (the reference's kernel sequence restated in
+    C, OpenMP over particles) on the first ticks of the same workload, on
+    `threads` threads and on one; returns (rate, seconds, rate 1, seconds 1)."""
     import oracle
     oracle.lib()
-    t0 = time.perf_counter()
-    oracle.run_filter(y[:0], n, rho=0.97, sigma_z=0.15, mu=float(mu_t[0]),
-                      ess_threshold=0.5, tile=4096, threads=threads)
-    t_init = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    oracle.run_filter(y[:ticks], n, rho=0.97, sigma_z=0.15, mu=float(mu_t[0]),
-                      ess_threshold=0.5, tile=4096, threads=threads, mu_t=mu_t[:ticks])
-    dt = time.perf_counter() - t0 - t_init
-    return n * ticks / dt, dt
+    out = []
+    for thr, tk in ((threads, ticks), (1, ticks_1)):
+        t0 = time.perf_counter()
+        oracle.run_filter(y[:0], n, rho=0.97, sigma_z=0.15, mu=float(mu_t[0]),
+                          ess_threshold=0.5, tile=4096, threads=thr)
+        t_init = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        oracle.run_filter(y[:tk], n, rho=0.97, sigma_z=0.15, mu=float(mu_t[0]),
+                          ess_threshold=0.5, tile=4096, threads=thr, mu_t=mu_t[:tk])
+        dt = time.perf_counter() - t0 - t_init
+        out += [n * tk / dt, dt]
+    return tuple(out)
+
+
+def cpu_ll_error(y, ll_gpu, n, tile, threads, **kw):
+    """Relative error of a GPU cumulative log-likelihood against the FP64
+    oracle run on the FP32 restatement's genealogy (same streams)."""
+    import oracle
+    ref = oracle.run_filter(y, n, tile=tile, mode=oracle.MODE_F64_COUPLED, threads=threads, **kw)[0]
+    return abs(ll_gpu - ref["ll_cum"][-1]) / abs(ref["ll_cum"][-1])
+
+
+def host_uniforms(seed, k):
+    """First k uniforms (r >> 9) / 2^23 of the reference's host PCG32 stream:
+    state seed * 67890 + 12345, increment 1442695040888963407, XSH-RR of the
+    old state (gpu_bpf/gpu_bpf_ptx_full.cu:241-251,353)."""
+    mask = (1 << 64) - 1
+    st = (seed * 67890 + 12345) & mask
+    out = []
+    for _ in range(k):
+        old = st
+        st = (old * 6364136223846793005 + 1442695040888963407) & mask
+        xs = (((old >> 18) ^ old) >> 27) & 0xFFFFFFFF
+        rot = old >> 59
+        r = ((xs >> rot) | (xs << ((-rot) & 31))) & 0xFFFFFFFF
+        out.append((r >> 9) / 8388608.0)
+    return out
 
 
 def log_uniform_counts(rng, M, lo, hi):
@@ -89,7 +138,8 @@ def prior_draws(rng, M):
 
 def looped_plain(gpu_bpf, n, mu, phi, sig, seeds, y, ticks, nu=0.0, score=False):
     """The plain filter looped over parameter sets (the baseline the batch
-    replaces): particle-ticks/s over `ticks` ticks of each filter."""
+    replaces), one filter after another: particle-ticks/s over `ticks` ticks
+    of each filter.  With score, each filter runs the adjoint pair."""
     tot = 0.0
     pt = 0
     for m in range(len(n)):
@@ -104,7 +154,7 @@ def looped_plain(gpu_bpf, n, mu, phi, sig, seeds, y, ticks, nu=0.0, score=False)
             f.run(y[:2])
             f.synchronize()
             t0 = time.perf_counter()
-            f.run(y[2:2 + ticks])
+            f.run(y[2:2 + ticks], stats=False)
             f.synchronize()
             tot += time.perf_counter() - t0
         f.close()
@@ -112,11 +162,38 @@ def looped_plain(gpu_bpf, n, mu, phi, sig, seeds, y, ticks, nu=0.0, score=False)
     return pt / tot
 
 
-def extra_workloads(args, gpu_bpf, oracle):
+def looped_plain_streams(gpu_bpf, torch, n, mu, phi, sig, seeds, y, ticks, nu=0.0):
+    """The plain filters over parameter sets on one stream each, all enqueued
+    before any is waited on (the reference's IMM loop, gpu_bpf/gpu_bpf.cu:
+    1033-1038): each filter's ticks replay its captured tick graphs, and the
+    filters' streams overlap on the GPU."""
+    y_dev = torch.from_numpy(np.ascontiguousarray(y[:2 + ticks], np.float32)).to("cuda:0")
+    torch.cuda.synchronize()
+    fs = []
+    for m in range(len(n)):
+        f = gpu_bpf.GpuBpf(int(n[m]), float(phi[m]), float(sig[m]), float(mu[m]), nu_obs=nu,
+                           seed=int(seeds[m]), ess_threshold=0.0)
+        f.run_device(y_dev.data_ptr(), 2)
+        fs.append(f)
+    for f in fs:
+        f.synchronize()
+    t0 = time.perf_counter()
+    for f in fs:
+        f.run_device(y_dev.data_ptr() + 8, ticks)
+    for f in fs:
+        f.synchronize()
+    dt = time.perf_counter() - t0
+    for f in fs:
+        f.close()
+    return float(np.sum(n[:len(fs)])) * ticks / dt
+
+
+def extra_workloads(args, gpu_bpf, torch):
     import gpu_bpf_batch
     out = {}
+    thr = args.cpu_threads
     # always-resample N = 65,536, T = 1,000 (the reference's CPU-runnable case)
-    _, y0 = oracle.dgp(1000, 0.97, 0.15, -1.0, seed=42)
+    _, y0 = sv_data.sv_series(1000, 0.97, 0.15, -1.0, seed=42)
     y0 = y0.astype(np.float32)
     f = gpu_bpf.GpuBpf(65536, 0.97, 0.15, -1.0, seed=42, ess_threshold=0.0)
     tile, _ = f.layout()
@@ -124,24 +201,26 @@ def extra_workloads(args, gpu_bpf, oracle):
     _, st = f.run(y0)
     wall = time.perf_counter() - t0
     f.close()
-    ref = oracle.run_filter(y0, 65536, ess_threshold=0.0, tile=tile, mode=oracle.MODE_F64_COUPLED)[0]
-    t0 = time.perf_counter()
-    oracle.run_filter(y0[:200], 65536, ess_threshold=0.0, tile=tile, threads=args.cpu_threads)
-    cpu = 65536 * 200 / (time.perf_counter() - t0)
-    out["plain_n65536_t1000"] = {
+    row = {
         "particle_ticks_per_s": 65536 * 1000 / wall,
         "ms_per_tick": wall,
         "ll_fp64": float(st["ll_cum"][-1]),
-        "ll_rel_err_vs_fp64_oracle": abs(st["ll_cum"][-1] - ref["ll_cum"][-1]) / abs(ref["ll_cum"][-1]),
-        "cpu_oracle_particle_ticks_per_s": cpu,
-        "cpu_threads": args.cpu_threads,
     }
-    log("C0", out["plain_n65536_t1000"])
-    # differentiable filter: N = 2^20, T = 2,000, checkpoint every 64 ticks
-    hs = np.array(oracle.host_stream(42, 3), np.uint64)
-    u = (hs >> np.uint64(9)).astype(np.float64) / 8388608.0
+    if not args.no_cpu:
+        import oracle
+        row["ll_rel_err_vs_fp64_oracle"] = cpu_ll_error(y0, st["ll_cum"][-1], 65536, tile, thr,
+                                                        ess_threshold=0.0)
+        t0 = time.perf_counter()
+        oracle.run_filter(y0[:200], 65536, ess_threshold=0.0, tile=tile, threads=thr)
+        row["cpu_oracle_particle_ticks_per_s"] = 65536 * 200 / (time.perf_counter() - t0)
+        row["cpu_threads"] = thr
+    out["plain_n65536_t1000"] = row
+    log("C0", row)
+    # differentiable filter: N = 2^20, T = 2,000, checkpoint every 64 ticks;
+    # theta from the host stream of seed 42 (uniform prior box)
+    u = host_uniforms(42, 3)
     th = (-2.0 + 2.0 * u[0], 0.90 + 0.095 * u[1], 0.05 + 0.35 * u[2])
-    _, y2 = oracle.dgp(2000, th[1], th[2], th[0], seed=42)
+    _, y2 = sv_data.sv_series(2000, th[1], th[2], th[0], seed=42)
     y2 = y2.astype(np.float32)
     n2 = 1 << 20
     f = gpu_bpf.GpuBpf(n2, th[1], th[2], th[0], seed=42)
@@ -151,7 +230,7 @@ def extra_workloads(args, gpu_bpf, oracle):
     g2 = f.score_backward(True)
     t2 = time.perf_counter()
     f.close()
-    out["adjoint_n1M_t2000"] = {
+    row = {
         "theta": [float(v) for v in th],
         "loglik": ll2,
         "score": [float(v) for v in g2],
@@ -160,106 +239,130 @@ def extra_workloads(args, gpu_bpf, oracle):
         "particle_ticks_per_s": n2 * 2000 / (t2 - t0),
         "checkpoint_every": 64,
     }
-    log("C2", out["adjoint_n1M_t2000"])
+    if not args.no_cpu:
+        # the forward pass is the always-resample filter: its log-likelihood
+        # over the first 200 ticks against the FP64 oracle
+        f = gpu_bpf.GpuBpf(n2, th[1], th[2], th[0], seed=42, ess_threshold=0.0)
+        tile2, _ = f.layout()
+        _, s2 = f.run(y2[:200])
+        f.close()
+        row["ll_rel_err_vs_fp64_oracle"] = cpu_ll_error(y2[:200], s2["ll_cum"][-1], n2, tile2, thr,
+                                                        rho=th[1], sigma_z=th[2], mu=th[0],
+                                                        ess_threshold=0.0)
+        row["ll_err_sample"] = "N=2^20, first 200 ticks"
+    out["adjoint_n1M_t2000"] = row
+    log("C2", row)
     # batched SMC^2: CPMMH, M = 512, N log-uniform in [256, 8192], T = 2,000
     T3 = 2000
-    mu3 = mu_schedule(T3)
-    _, y3 = oracle.dgp_regimes(mu3.astype(np.float64), rho=0.97, sigma_z=0.15, seed=42)
+    mu3 = sv_data.mu_schedule(T3)
+    _, y3 = sv_data.sv_series(T3, 0.97, 0.15, mu3.astype(np.float64), seed=42)
     y3 = y3.astype(np.float32)
     rng = np.random.default_rng(42)
     M = 512
     n3 = log_uniform_counts(rng, M, 256, 8192)
     mu, phi, sig, seeds = prior_draws(rng, M)
-    b = gpu_bpf_batch.GpuBpfBatch(n3, mu, phi, sig, seeds)
-    blk = b.autotune(y3, 64)
-    b.close()
-    b = gpu_bpf_batch.GpuBpfBatch(n3, mu, phi, sig, seeds, block=blk)
-    b.run(y3[:32], stats=False)
-    b.synchronize()
-    t0 = time.perf_counter()
-    b.run(y3[32:T3], stats=False)
-    b.synchronize()
-    batch_pt = float(n3.sum()) * (T3 - 32) / (time.perf_counter() - t0)
-    b.close()
-    loop_pt = looped_plain(gpu_bpf, n3[:64], mu, phi, sig, seeds, y3, 40)
-    t0 = time.perf_counter()
-    s = gpu_bpf_batch.GpuSmc2(M, 256, 8192, seed=42, capacity=T3, block=blk)
-    s.run(y3)
-    wall = time.perf_counter() - t0
-    c = s.counters()
-    tm, tsd = s.theta_mean_std()
-    s.close()
-    out["smc2_cpmmh_m512_t2000"] = {
-        "inner_batch_particle_ticks_per_s": batch_pt,
-        "looped_plain_particle_ticks_per_s": loop_pt,
-        "batch_over_looped": batch_pt / loop_pt,
-        "block": blk,
-        "sum_n": int(n3.sum()),
-        "smc2_wall_s": wall,
-        "smc2_useful_particle_ticks_per_s": c["pt_forward"] / wall,
-        "smc2_total_particle_ticks_per_s": (c["pt_forward"] + c["pt_replay"]) / wall,
-        "counters": c,
-        "theta_mean": [float(v) for v in tm],
-        "theta_sd": [float(v) for v in tsd],
-    }
+    out["smc2_cpmmh_m512_t2000"] = batch_and_smc2(args, gpu_bpf, gpu_bpf_batch, torch, y3, n3, mu,
+                                                  phi, sig, seeds, 256, 8192, nu=0.0, kind="cpmmh",
+                                                  moves=5, ticks_batch=T3 - 32)
     log("C3", out["smc2_cpmmh_m512_t2000"])
     # batched SMC^2: MALA from per-filter scores, M = 1024, N log-uniform in
     # [512, 16384], Student-t (nu = 5) observations, T = 5,000
     T4 = 5000
-    mu4 = mu_schedule(T4)
-    _, y4 = oracle.dgp_regimes(mu4.astype(np.float64), rho=0.97, sigma_z=0.15, nu_obs=5.0, seed=42)
+    mu4 = sv_data.mu_schedule(T4)
+    _, y4 = sv_data.sv_series(T4, 0.97, 0.15, mu4.astype(np.float64), nu_obs=5.0, seed=42)
     y4 = y4.astype(np.float32)
     M = 1024
     n4 = log_uniform_counts(rng, M, 512, 16384)
     mu, phi, sig, seeds = prior_draws(rng, M)
-    b = gpu_bpf_batch.GpuBpfBatch(n4, mu, phi, sig, seeds, nu_obs=5.0)
-    blk4 = b.autotune(y4, 64)
+    out["smc2_mala_m1024_t5000_student_t"] = batch_and_smc2(args, gpu_bpf, gpu_bpf_batch, torch, y4,
+                                                            n4, mu, phi, sig, seeds, 512, 16384,
+                                                            nu=5.0, kind="mala", moves=3,
+                                                            ticks_batch=1024)
+    log("C4", out["smc2_mala_m1024_t5000_student_t"])
+    out["smc2_rbpf_ntheta1024_t1000"] = rbpf_workload(args)
+    log("RBPF", out["smc2_rbpf_ntheta1024_t1000"])
+    return out
+
+
+def batch_and_smc2(args, gpu_bpf, gpu_bpf_batch, torch, y, n, mu, phi, sig, seeds, nmin, nmax, nu,
+                   kind, moves, ticks_batch):
+    """Inner-batch throughput (autotuned), the looped plain filters, the
+    SMC^2 run, and the inner filters' log-likelihood error on a sample."""
+    score = kind == "mala"
+    M = len(n)
+    b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=nu)
+    blk = b.autotune(y, 64)
     b.close()
-    b = gpu_bpf_batch.GpuBpfBatch(n4, mu, phi, sig, seeds, nu_obs=5.0, block=blk4)
-    b.enable_score()
-    b.run(y4[:32], stats=False)
+    b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=nu, block=blk)
+    if score:
+        b.enable_score()
+    b.run(y[:32], stats=False)
     b.synchronize()
     t0 = time.perf_counter()
-    b.run(y4[32:1056], stats=False)
+    b.run(y[32:32 + ticks_batch], stats=False)
     b.synchronize()
-    batch4 = float(n4.sum()) * 1024 / (time.perf_counter() - t0)
+    batch_pt = float(n.sum()) * ticks_batch / (time.perf_counter() - t0)
     b.close()
-    loop4 = looped_plain(gpu_bpf, n4[:16], mu, phi, sig, seeds, y4, 100, nu=5.0, score=True)
+    k = 16 if score else 64
+    loop_pt = looped_plain(gpu_bpf, n[:k], mu, phi, sig, seeds, y, 100 if score else 40, nu=nu,
+                           score=score)
+    row = {
+        ("inner_batch_with_score_particle_ticks_per_s" if score else
+         "inner_batch_particle_ticks_per_s"): batch_pt,
+        ("looped_plain_score_particle_ticks_per_s" if score else
+         "looped_plain_particle_ticks_per_s"): loop_pt,
+        "batch_over_looped": batch_pt / loop_pt,
+    }
+    if not score:
+        strm = looped_plain_streams(gpu_bpf, torch, n[:M], mu, phi, sig, seeds, y, 200, nu=nu)
+        row["looped_plain_streams_particle_ticks_per_s"] = strm
+        row["batch_over_looped_streams"] = batch_pt / strm
+        row["looped_streams_sample"] = f"all {M} filters, one stream each, 200 ticks"
+    row.update({"block": blk, "sum_n": int(n.sum())})
+    T = len(y)
     t0 = time.perf_counter()
-    s = gpu_bpf_batch.GpuSmc2(M, 512, 16384, nu_obs=5.0, seed=42, kind="mala", moves=3,
-                              mala_step=0.5, block=blk4)
-    s.run(y4)
+    s = gpu_bpf_batch.GpuSmc2(M, nmin, nmax, nu_obs=nu, seed=42, kind=kind, moves=moves,
+                              mala_step=0.5, capacity=T, block=blk)
+    s.run(y)
     wall = time.perf_counter() - t0
     c = s.counters()
     tm, tsd = s.theta_mean_std()
     s.close()
-    out["smc2_mala_m1024_t5000_student_t"] = {
-        "inner_batch_with_score_particle_ticks_per_s": batch4,
-        "looped_plain_score_particle_ticks_per_s": loop4,
-        "batch_over_looped": batch4 / loop4,
-        "block": blk4,
-        "sum_n": int(n4.sum()),
+    row.update({
         "smc2_wall_s": wall,
         "smc2_useful_particle_ticks_per_s": c["pt_forward"] / wall,
         "smc2_total_particle_ticks_per_s": (c["pt_forward"] + c["pt_replay"]) / wall,
         "counters": c,
         "theta_mean": [float(v) for v in tm],
         "theta_sd": [float(v) for v in tsd],
-    }
-    log("C4", out["smc2_mala_m1024_t5000_student_t"])
-    out["smc2_rbpf_ntheta1024_t1000"] = rbpf_workload(oracle)
-    log("RBPF", out["smc2_rbpf_ntheta1024_t1000"])
-    return out
+    })
+    if not args.no_cpu:
+        # 8 inner filters spread over the particle-count range, 500 ticks,
+        # against the FP64 oracle on the same streams
+        pick = np.argsort(n)[np.linspace(0, M - 1, 8).astype(int)]
+        errs = []
+        bs = gpu_bpf_batch.GpuBpfBatch(n[pick], mu[pick], phi[pick], sig[pick], seeds[pick], nu_obs=nu,
+                                       block=blk)
+        bs.run(y[:500], stats=False)
+        ll = bs.loglik()
+        bs.close()
+        for q, m in enumerate(pick):
+            errs.append(cpu_ll_error(y[:500], ll[q], int(n[m]), 256, args.cpu_threads,
+                                     rho=float(phi[m]), sigma_z=float(sig[m]), mu=float(mu[m]),
+                                     nu_obs=nu, ess_threshold=0.0, seed=int(seeds[m])))
+        row["ll_rel_err_vs_fp64_oracle_max"] = float(max(errs))
+        row["ll_err_sample"] = "8 inner filters across the N range, first 500 ticks"
+    return row
 
 
-def rbpf_workload(oracle):
+def rbpf_workload(args):
     """SMC^2 with the RBPF / OCSN inner filter, the reference's throughput and
     latency cases (smc2/test_smc2_rbpf.cu:412-470,600-680): N_theta = 1024,
     N_inner in {64, 128, 256, 512}, T = 1000 observations of the
     regime-switching SV model, fixed lag 200, seed 42."""
     import smc2_rbpf
     T = 1000
-    y, _, _ = oracle.rb_dgp(T, 0.95, 0.10, -1.0, 0.15, seed=12345)
+    y, _, _ = sv_data.rbpf_series(T, 0.95, 0.10, -1.0, 0.15, seed=12345)
     rows = {}
     for ni in (64, 128, 256, 512):
         s = smc2_rbpf.Smc2Rbpf(1024, ni, seed=42, fixed_lag=200)
@@ -293,7 +396,7 @@ def rbpf_workload(oracle):
     # 512, T = 1200, fixed lag 200; its published time is 300 ms per 1200
     # ticks on an RTX 5080 (smc2/multi_scale_smc2_architecture.md:34)
     truth = (0.95, 0.10, -1.0, 0.15)
-    yl, _, _ = oracle.rb_dgp(1200, *truth, seed=12345)
+    yl, _, _ = sv_data.rbpf_series(1200, *truth, seed=12345)
     s = smc2_rbpf.Smc2Rbpf(1024, 512, seed=12345, noise_capacity=1328, fixed_lag=200)
     t0 = time.perf_counter()
     s.run(yl)
@@ -320,7 +423,7 @@ def rbpf_workload(oracle):
     log("RBPF learn fp16 noise", rows["learn_1024x512_t1200_fp16_noise"])
     # fixed-lag against full-history replays at T = 5000
     # (smc2/test_smc2_rbpf.cu:343-410)
-    y5, _, _ = oracle.rb_dgp(5000, *truth, seed=12345)
+    y5, _, _ = sv_data.rbpf_series(5000, *truth, seed=12345)
     for lag in (0, 200):
         s = smc2_rbpf.Smc2Rbpf(1024, 512, seed=12345, noise_capacity=5128, fixed_lag=lag)
         t0 = time.perf_counter()
@@ -336,24 +439,28 @@ def rbpf_workload(oracle):
             "theta_sd": [float(v) for v in sd],
         }
         log("RBPF lag", lag, rows[f"lag_{lag}_1024x512_t5000"])
-    # CPU oracle (OpenMP over theta-particles) on a bounded sample
-    cfg = oracle.rb_cfg(64, 512, seed=42, lag=200, T=T)
-    t0 = time.perf_counter()
-    oracle.run_rb_smc2(cfg, y[:40])
-    rows["cpu_oracle_forward_inner_particle_ticks_per_s"] = 64 * 512 * 40 / (time.perf_counter() - t0)
-    rows["cpu_sample"] = "oracle, N_theta=64, N_inner=512, first 40 observations"
+    if not args.no_cpu:
+        # CPU oracle (OpenMP over theta-particles) on a bounded sample
+        import oracle
+        cfg = oracle.rb_cfg(64, 512, seed=42, lag=200, T=T)
+        t0 = time.perf_counter()
+        oracle.run_rb_smc2(cfg, y[:40])
+        rows["cpu_oracle_forward_inner_particle_ticks_per_s"] = 64 * 512 * 40 / (time.perf_counter() - t0)
+        rows["cpu_sample"] = "oracle, N_theta=64, N_inner=512, first 40 observations"
     return rows
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000, help="timed ticks")
-    ap.add_argument("--warmup", type=int, default=100, help="untimed ticks")
+    ap.add_argument("--steps", type=int, default=20, help="timed steps")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed steps (same instance)")
+    ap.add_argument("--ticks-per-step", type=int, default=250,
+                    help="observations per step; 20 x 250 = the workload's T = 5000")
     ap.add_argument("--particles", type=int, default=51_200_000)
-    ap.add_argument("--ticks", type=int, default=5000, help="length of the observation series")
     ap.add_argument("--items", type=int, default=0, help="particles per lane (0 = auto)")
     ap.add_argument("--cpu-ticks", type=int, default=100)
+    ap.add_argument("--cpu-ticks-1", type=int, default=8, help="ticks of the one-thread CPU run")
     ap.add_argument("--timing-stride", type=int, default=10,
                     help="HIP-event timing of every n-th tick of the timed region")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
@@ -376,8 +483,7 @@ def main():
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 5}))
         return
     if args.rbpf_only:
-        import oracle
-        print(json.dumps({"smc2_rbpf_ntheta1024_t1000": rbpf_workload(oracle)}), flush=True)
+        print(json.dumps({"smc2_rbpf_ntheta1024_t1000": rbpf_workload(args)}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
@@ -386,71 +492,70 @@ def main():
 
     import torch
     import gpu_bpf
-    import oracle
 
     n = args.particles
-    T = max(args.ticks, args.steps)
-    mu_t = mu_schedule(T)
-    log("generating observations", T)
-    _, y64 = oracle.dgp_regimes(mu_t.astype(np.float64), rho=0.97, sigma_z=0.15, seed=42)
+    tps = args.ticks_per_step
+    K = args.steps
+    T = K * tps                      # timed ticks, from tick 0 of the series
+    W = args.warmup * tps            # warm-up ticks on the same instance
+    n_series = max(T, W)
+    mu_t = sv_data.mu_schedule(n_series)
+    log("generating observations", n_series)
+    _, y64 = sv_data.sv_series(n_series, 0.97, 0.15, mu_t.astype(np.float64), seed=42)
     y = y64.astype(np.float32)
     dev = torch.device("cuda:0")
     y_dev = torch.from_numpy(y).to(dev)
     torch.cuda.synchronize()
     L = gpu_bpf.load()
 
-    # warm-up on a separate filter instance
-    if args.warmup > 0:
-        log("warmup", args.warmup)
-        w = gpu_bpf.GpuBpf(n, 0.97, 0.15, float(mu_t[0]), seed=42, ess_threshold=0.5,
-                           items_per_thread=args.items or None)
-        enqueue_run(w, L, y_dev.data_ptr(), mu_t, 0, args.warmup)
-        w.synchronize()
-        w.close()
-
     f = gpu_bpf.GpuBpf(n, 0.97, 0.15, float(mu_t[0]), seed=42, ess_threshold=0.5,
                        items_per_thread=args.items or None)
+    f.set_stats_capacity(T)          # every timed tick's statistics stay on the device
     tile, n_tiles = f.layout()
+    if W > 0:
+        log("warmup", W, "ticks on the timed instance")
+        enqueue_run(f, L, y_dev.data_ptr(), mu_t, 0, W)
+        f.synchronize()
+        f.restart()                  # back to tick 0, no allocation
+        f.set_mu(float(mu_t[0]))
     # HIP events around the three kernels of every timing_stride-th tick
     # (each event adds ~2 us to the stream, so not every tick is timed)
     f.enable_kernel_timing(True, stride=args.timing_stride)
-    K = args.steps
-    log("timed", K, "ticks, N =", n, "tile", tile)
+    log("timed", K, "steps =", T, "ticks, N =", n, "tile", tile)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    enqueue_run(f, L, y_dev.data_ptr(), mu_t, 0, K)
+    enqueue_run(f, L, y_dev.data_ptr(), mu_t, 0, T)
     f.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    tick_ms = f.kernel_tick_times(K)
-    ms, cnt = f.kernel_times()
-    st = np.zeros(K, gpu_bpf.STATS_DTYPE)
-    ring = min(K, 1024)
-    rc = L.gpu_bpf_copy_stats(f.h, st[K - ring:].ctypes.data, ring)
-    rs_mask = st["resampled"][K - ring:] == 1
-    tm = tick_ms[K - ring:]
+    tick_ms = f.kernel_tick_times(T)
+    st = f.stats_tail(T)
     rcount = f.resample_count()
     f.close()
-    if rc < 0:
-        raise RuntimeError(gpu_bpf.last_error())
-    log(f"wall {wall:.3f}s  kernel ms {ms.tolist()}  launches {cnt.tolist()}  resamples {rcount}")
+    if len(tick_ms) != T or not np.array_equal(st["tick"], np.arange(T)):
+        raise RuntimeError("timed ticks were not all recorded")
+    rs = st["resampled"] == 1
+    log(f"wall {wall:.3f}s  resamples {rcount}")
 
-    pt = float(n) * K
+    pt = float(n) * T
     value = pt / wall
-    # algorithmic bytes of the propagate kernel: read x, write x', write l,
-    # plus read l on ticks that carry weights (every tick after a tick that
-    # did not resample); tick 0 reads x and writes l only
-    n_prop = K - 1
-    last_rs = int(st["resampled"][-1])
-    n_carry = (K - 1) - (rcount - last_rs)     # ticks 1..K-1 whose predecessor kept weights
-    bytes_prop = n * (8.0 + 12.0 * n_prop + 4.0 * n_carry)
-    # the timed launches are propagating ticks (tick 0 is not sampled): their
-    # mean algorithmic bytes are those of ticks 1..K-1
-    avg_prop_s = ms[0] / 1e3 / max(cnt[0], 1)
-    bytes_per_launch = n * (12.0 * n_prop + 4.0 * n_carry) / n_prop
+    # Algorithmic bytes per tick (4 B per read or write of a per-particle FP32
+    # array): propagate reads x and writes x' and l (12 B), plus reads l on a
+    # tick whose predecessor kept its weights (16 B); tick 0 reads x and
+    # writes l (8 B); a resampling tick adds read x', l and write x (12 B).
+    carry = np.zeros(T, bool)
+    carry[1:] = ~rs[:-1]
+    prop_bytes = np.where(carry, 16.0, 12.0) * n
+    prop_bytes[0] = 8.0 * n
+    tick_bytes = prop_bytes + np.where(rs, 12.0 * n, 0.0)
+    timed = np.isfinite(tick_ms[:, 0])
+    timed[0] = False                 # tick 0 propagates nothing
+    prop_us = tick_ms[timed, 0] * 1e3
+    avg_prop_s = float(np.mean(tick_ms[timed, 0])) / 1e3
+    bytes_per_launch = float(np.mean(prop_bytes[timed]))
     achieved = bytes_per_launch / avg_prop_s
-    # whole-tick algorithmic traffic incl. resampling (read x', l; write x)
-    bytes_total = bytes_prop + 12.0 * n * rcount
+    fin_ms = tick_ms[np.isfinite(tick_ms[:, 1]), 1]
+    rs_t = timed & rs
     res = {
         "metric": METRIC,
         "value": value,
@@ -465,23 +570,32 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "large single SV bootstrap filter: N=51.2M FP32 particles, T=5000 ticks, "
-                        "mu regime shifts -1.5/-0.5/+0.5 every 1250 ticks, ESS<0.5N resampling, "
-                        "FP64 CDF and log-likelihood",
+            "workload": f"large single SV bootstrap filter: N={n} FP32 particles, {T} ticks from "
+                        f"tick 0 ({K} steps of {tps} observations; warm-up {W} ticks on the same "
+                        f"instance, then gpu_bpf_restart), mu -1.5/-0.5/+0.5 switching every 1250 "
+                        f"ticks via gpu_bpf_set_mu, ESS<0.5N resampling ({rcount} resampling "
+                        f"ticks), FP64 CDF and log-likelihood",
             "n_particles": n,
-            "ticks": K,
+            "ticks": T,
+            "ticks_per_step": tps,
+            "warmup_ticks": W,
             "ess_threshold": 0.5,
             "tile": tile,
             "resamples": rcount,
             "ll_cum_fp64": float(st["ll_cum"][-1]),
-            "whole_tick_hbm_frac": bytes_total / wall / HBM_PEAK,
-            "kernel_ms_per_launch": {
-                "propagate": ms[0] / max(cnt[0], 1),
-                "finalize": ms[1] / max(cnt[1], 1),
-                "resample": ms[2] / max(cnt[2], 1),
-                "resample_on_resample_ticks": nanmean_or_none(tm[rs_mask, 2]),
-                "resample_on_other_ticks": nanmean_or_none(tm[~rs_mask, 2]),
-                "timed_ticks": int(cnt[0]),
+            "whole_tick_hbm_frac": float(tick_bytes.sum()) / wall / HBM_PEAK,
+            "algorithmic_bytes_per_tick_mean": float(tick_bytes.mean()),
+            "kernel_us": {
+                "propagate_mean": float(prop_us.mean()),
+                "propagate_p5_p50_p95": [float(v) for v in np.percentile(prop_us, [5, 50, 95])],
+                "propagate_min_max": [float(prop_us.min()), float(prop_us.max())],
+                "finalize_mean": float(fin_ms.mean()) * 1e3,
+                "resample_on_resample_ticks": (float(np.mean(tick_ms[rs_t, 2])) * 1e3
+                                               if rs_t.any() else None),
+                "resample_on_other_ticks": float(np.mean(tick_ms[timed & ~rs, 2])) * 1e3,
+                "timed_ticks": int(timed.sum()),
+                "timing_stride": args.timing_stride,
+                "propagate_per_timed_tick": [round(float(v), 1) for v in prop_us],
             },
         },
         "roofline": {
@@ -492,6 +606,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
             "traffic": None,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
         },
     }
     # measured HBM bytes per launch of the same kernel (rocprofv3 FETCH_SIZE /
@@ -501,7 +616,6 @@ def main():
         tr = json.load(open(tpath))
         res["roofline"]["traffic"] = tr["traffic_bytes_per_launch_mean"]
         res["roofline"]["traffic_carry_tick"] = tr["traffic_bytes_per_launch_carry_tick"]
-        res["roofline"]["algorithmic_bytes_per_launch"] = bytes_per_launch
         res["roofline"]["traffic_source"] = tr["source"]
     # a plain device copy moving the same bytes as one propagate launch (half
     # read, half written), timed the same way: the streaming rate this access
@@ -524,17 +638,27 @@ def main():
     log("copy of the same bytes", copy_bw / 1e9, "GB/s")
     if not args.no_cpu:
         log("cpu baseline", args.cpu_ticks, "ticks x", n, "threads", args.cpu_threads)
-        v, dt = cpu_baseline(y, mu_t, n, args.cpu_ticks, args.cpu_threads)
+        v, dt, v1, dt1 = cpu_baseline(y, mu_t, n, args.cpu_ticks, args.cpu_ticks_1, args.cpu_threads)
+        try:
+            aff = len(os.sched_getaffinity(0))
+        except AttributeError:
+            aff = None
         res["cpu_baseline"] = {
             "value": v,
             "unit": "particle-ticks/s",
             "cores": args.cpu_threads,
             "kind": "port",
-            "sample": f"oracle FP32 filter (OpenMP), N={n}, first {args.cpu_ticks} ticks of the "
-                      f"same workload, {dt:.1f} s",
+            "sample": f"oracle FP32 filter (the reference's kernel sequence in C, OpenMP over "
+                      f"particles), N={n}, first {args.cpu_ticks} ticks of the same workload on "
+                      f"{args.cpu_threads} threads ({dt:.1f} s) and first {args.cpu_ticks_1} ticks "
+                      f"on 1 thread ({dt1:.1f} s)",
+            "single_core_value": v1,
+            "cpu_model": cpu_model(),
+            "nproc": os.cpu_count(),
+            "cpus_allowed": aff,
         }
     if not args.no_extra:
-        res["workloads"] = extra_workloads(args, gpu_bpf, oracle)
+        res["workloads"] = extra_workloads(args, gpu_bpf, torch)
     print(json.dumps(res), flush=True)
 
 

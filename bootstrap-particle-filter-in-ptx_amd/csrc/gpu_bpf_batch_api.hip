@@ -295,8 +295,9 @@ int gpu_bpf_batch_set_block(GpuBpfBatch* b, int threads) {
 int gpu_bpf_batch_get_block(GpuBpfBatch* b) { return b ? b->block : -1; }
 
 int gpu_bpf_batch_autotune(GpuBpfBatch* b, const float* y, int ticks) {
-    if (!b || !y || ticks <= 0) {
-        set_err("gpu_bpf_batch_autotune: bad arguments");
+    // two untimed ticks, then ticks - 2 timed ones: y holds `ticks` values
+    if (!b || !y || ticks < 3) {
+        set_err("gpu_bpf_batch_autotune: need a batch and at least 3 observations");
         return -1;
     }
     float best_ms = 0.0f;
@@ -314,7 +315,7 @@ int gpu_bpf_batch_autotune(GpuBpfBatch* b, const float* y, int ticks) {
         int rc = -1;
         if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
             append_y(s, y, ticks, hipMemcpyHostToDevice) == 0 && run_ticks(s, 2) == 0 &&
-            hipEventRecord(e0, s->stream) == hipSuccess && run_ticks(s, ticks - 2 > 0 ? ticks - 2 : 1) == 0 &&
+            hipEventRecord(e0, s->stream) == hipSuccess && run_ticks(s, ticks - 2) == 0 &&
             hipEventRecord(e1, s->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
             hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
             rc = 0;

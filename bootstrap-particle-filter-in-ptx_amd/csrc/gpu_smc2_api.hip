@@ -105,6 +105,8 @@ struct GpuSmc2 {
     double last_ess = 0.0;
     long long ticks = 0, outer_resamples = 0, proposed = 0, accepted = 0;
     long long replay_ticks = 0, pt_forward = 0, pt_replay = 0;
+    std::vector<int> hist_tick;        // per outer resample: ticks done when it happened
+    std::vector<long long> hist_sum_n; // and the particle count of the resampled population
     bool ready = false;
 };
 
@@ -346,6 +348,34 @@ static double log_alpha(int kind, const Precond& P, double h, const double th[3]
     return la;
 }
 
+// Systematic resampling of n log-weights with one uniform u in (0, 1):
+// slot m takes the first j whose running weight sum reaches (u + m) / n of the
+// total.  Weights are exp(lw - max) over the finite entries; a non-finite
+// log-weight (NaN from a NaN observation, -inf from a zero likelihood) has
+// weight zero and is never an ancestor.  Returns -1 when no entry is finite.
+static int systematic_ancestors(const double* lw, int n, double u, int* anc) {
+    double mx = -INFINITY;
+    for (int m = 0; m < n; ++m)
+        if (std::isfinite(lw[m]) && lw[m] > mx) mx = lw[m];
+    if (!std::isfinite(mx)) return -1;
+    std::vector<double> cdf(n);
+    double acc = 0.0;
+    int last = 0;               // last entry of positive weight: the walk never passes it
+    for (int m = 0; m < n; ++m) {
+        const double w = std::isfinite(lw[m]) ? exp(lw[m] - mx) : 0.0;
+        acc += w;
+        cdf[m] = acc;
+        if (w > 0.0) last = m;
+    }
+    int j = 0;
+    for (int m = 0; m < n; ++m) {
+        const double t = (u + m) / n * acc;
+        while (j < last && cdf[j] < t) ++j;     // a zero weight never reaches t > 0 first
+        anc[m] = j;
+    }
+    return 0;
+}
+
 // Outer systematic resampling after `rows` ticks: slot m takes slot anc[m]'s
 // whole filter; future ticks of every slot draw from fresh streams.
 static int outer_resample(GpuSmc2* s, int rows) {
@@ -355,21 +385,12 @@ static int outer_resample(GpuSmc2* s, int rows) {
     hipStream_t st = mb->stream;
     std::vector<double> lw(M);
     HIP_TRY(hipMemcpy(lw.data(), s->d_logw, M * sizeof(double), hipMemcpyDeviceToHost));
-    double mx = -INFINITY;
-    for (double v : lw) mx = std::max(mx, v);
-    std::vector<double> cdf(M);
-    double acc = 0.0;
-    for (int m = 0; m < M; ++m) {
-        acc += std::isfinite(mx) ? exp(lw[m] - mx) : 1.0;
-        cdf[m] = acc;
-    }
     std::vector<int> anc(M);
     const double u = s->rng.uniform();
-    int j = 0;
-    for (int m = 0; m < M; ++m) {
-        const double t = (u + m) / M * acc;
-        while (j < M - 1 && cdf[j] < t) ++j;
-        anc[m] = j;
+    if (systematic_ancestors(lw.data(), M, u, anc.data())) {
+        set_err("gpu_smc2: every outer log-weight is non-finite at tick %d (NaN or impossible "
+                "observations); the theta-particles cannot be resampled", rows);
+        return -1;
     }
     HIP_TRY(hipMemcpyAsync(s->d_anc, anc.data(), M * sizeof(int), hipMemcpyHostToDevice, st));
     GatherArgs g;
@@ -433,6 +454,10 @@ static int outer_resample(GpuSmc2* s, int rows) {
     HIP_TRY(hipMemcpy(mb->d_ctl, c.data(), M * sizeof(Ctl), hipMemcpyHostToDevice));
     HIP_TRY(hipMemsetAsync(s->d_logw, 0, M * sizeof(double), st));
     s->outer_resamples += 1;
+    long long sum_n = 0;
+    for (int m = 0; m < M; ++m) sum_n += mb->n[m];
+    s->hist_tick.push_back(rows);
+    s->hist_sum_n.push_back(sum_n);
     return 0;
 }
 
@@ -822,11 +847,14 @@ static int theta_stats(GpuSmc2* s, float* mean3, float* std3) {
     std::vector<double> lw(s->M);
     HIP_TRY(hipStreamSynchronize(s->main->stream));
     HIP_TRY(hipMemcpy(lw.data(), s->d_logw, s->M * sizeof(double), hipMemcpyDeviceToHost));
+    // non-finite log-weights count as zero weight, as in outer resampling;
+    // with none finite every theta-particle counts equally
     double mx = -INFINITY;
-    for (double v : lw) mx = std::max(mx, v);
+    for (double v : lw)
+        if (std::isfinite(v) && v > mx) mx = v;
     double W = 0.0, m1[3] = {0, 0, 0}, m2[3] = {0, 0, 0};
     for (int m = 0; m < s->M; ++m) {
-        const double w = std::isfinite(mx) ? exp(lw[m] - mx) : 1.0;
+        const double w = !std::isfinite(mx) ? 1.0 : (std::isfinite(lw[m]) ? exp(lw[m] - mx) : 0.0);
         const double th[3] = {s->main->mu[m], s->main->phi[m], s->main->sig[m]};
         W += w;
         for (int k = 0; k < 3; ++k) {
@@ -853,6 +881,31 @@ int gpu_smc2_get_counters(GpuSmc2* s, long long* out) {
     const long long v[7] = {s->ticks, s->outer_resamples, s->proposed, s->accepted,
                             s->replay_ticks, s->pt_forward, s->pt_replay};
     memcpy(out, v, sizeof v);
+    return 0;
+}
+
+int gpu_smc2_get_history(GpuSmc2* s, int* ticks, long long* sum_n, int max_entries) {
+    if (!s || max_entries < 0) {
+        set_err("gpu_smc2_get_history: null state or negative size");
+        return -1;
+    }
+    const int k = (int)s->hist_tick.size();
+    for (int i = 0; i < k && i < max_entries; ++i) {
+        if (ticks) ticks[i] = s->hist_tick[i];
+        if (sum_n) sum_n[i] = s->hist_sum_n[i];
+    }
+    return k;
+}
+
+int gpu_smc2_systematic_ancestors(const double* log_w, int n, double u, int* anc) {
+    if (!log_w || !anc || n <= 0 || !(u > 0.0 && u < 1.0)) {
+        set_err("gpu_smc2_systematic_ancestors: need n > 0 entries and u in (0, 1)");
+        return -1;
+    }
+    if (systematic_ancestors(log_w, n, u, anc)) {
+        set_err("gpu_smc2_systematic_ancestors: no finite log-weight");
+        return -1;
+    }
     return 0;
 }
 

@@ -122,6 +122,8 @@ def load(path=LIB_PATH):
         "gpu_bpf_set_items_per_thread": (I, [P, I]),
         "gpu_bpf_synchronize": (I, [P]),
         "gpu_bpf_get_timestep": (I, [P]),
+        "gpu_bpf_restart": (I, [P]),
+        "gpu_bpf_set_stats_capacity": (I, [P, I]),
         "gpu_bpf_enable_kernel_timing": (I, [P, I]),
         "gpu_bpf_score_forward": (I, [P, P, I, I, ctypes.POINTER(D)]),
         "gpu_bpf_kernel_tick_times": (I, [P, P, I]),
@@ -262,6 +264,19 @@ class GpuBpf:
     def timestep(self):
         return self.L.gpu_bpf_get_timestep(self.h)
 
+    def set_stats_capacity(self, ticks):
+        self._check(self.L.gpu_bpf_set_stats_capacity(self.h, int(ticks)))
+
+    def stats_tail(self, n):
+        """Statistics of the last n ticks (n <= the ring capacity)."""
+        st = np.zeros(n, STATS_DTYPE)
+        self._check(self.L.gpu_bpf_copy_stats(self.h, _ptr(st), int(n)))
+        return st
+
+    def restart(self):
+        """Back to tick 0 with the current parameters (no allocation)."""
+        self._check(self.L.gpu_bpf_restart(self.h))
+
     def set_mu_schedule_run(self, y, mu_t, stats=False):
         """Run y with mu changed (set_mu) wherever mu_t changes value."""
         y = np.ascontiguousarray(y, np.float32)
@@ -276,6 +291,10 @@ class GpuBpf:
                 yd = y[a:b]
                 self._check(self.L.gpu_bpf_run(self.h, _ptr(yd), len(yd), None, None))
         return np.concatenate(out) if stats else None
+
+    def run_device(self, y_dev_ptr, n_ticks):
+        """Enqueue n_ticks ticks on observations already in device memory."""
+        self._check(self.L.gpu_bpf_run_device(self.h, ctypes.c_void_p(int(y_dev_ptr)), int(n_ticks)))
 
     def enqueue(self, y):
         """Enqueue len(y) ticks without waiting (observations staged once)."""

@@ -168,6 +168,8 @@ def _smc2_lib():
             "gpu_smc2_set_block": (I, [P, I]),
             "gpu_smc2_set_mala_step": (I, [P, F]),
             "gpu_smc2_selftest_mcmc": (I, [I, I, I, I, P]),
+            "gpu_smc2_get_history": (I, [P, P, P, I]),
+            "gpu_smc2_systematic_ancestors": (I, [P, I, ctypes.c_double, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -255,6 +257,26 @@ class GpuSmc2:
         c = np.zeros(7, np.int64)
         self._check(self.L.gpu_smc2_get_counters(self.h, _ptr(c)))
         return dict(zip(COUNTER_NAMES, c.tolist()))
+
+    def history(self):
+        """Outer resamples so far: (ticks done at each, total particle count of
+        the resampled population)."""
+        k = self._check(self.L.gpu_smc2_get_history(self.h, None, None, 0))
+        t = np.zeros(max(k, 1), np.int32)
+        s = np.zeros(max(k, 1), np.int64)
+        self._check(self.L.gpu_smc2_get_history(self.h, _ptr(t), _ptr(s), k))
+        return t[:k], s[:k]
+
+
+def systematic_ancestors(log_w, u):
+    """The outer layer's systematic resampling (host code of the library, no
+    GPU): ancestors of len(log_w) slots for the uniform u."""
+    lw = np.ascontiguousarray(log_w, np.float64)
+    anc = np.zeros(lw.size, np.int32)
+    rc = _smc2_lib().gpu_smc2_systematic_ancestors(_ptr(lw), lw.size, float(u), _ptr(anc))
+    if rc < 0:
+        raise gpu_bpf.GpuBpfError(gpu_bpf.last_error())
+    return anc
 
 
 def selftest_mcmc(kind, iters, seed=1, fault=0):

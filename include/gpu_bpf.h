@@ -182,6 +182,12 @@ int gpu_bpf_run_device(GpuBpfState* state, const float* y_dev, int n_ticks);
 /* Copy the statistics of the last n ticks (n <= ring capacity) to the host. */
 int gpu_bpf_copy_stats(GpuBpfState* state, BpfTickStats* stats, int n);
 
+/* Grow the device ring of per-tick statistics to hold at least `ticks`
+ * ticks (default 1024; gpu_bpf_run grows it to its run length).  Records
+ * already held are kept.  Allocates; call it before a run of
+ * gpu_bpf_run_device whose every tick is to be read back. */
+int gpu_bpf_set_stats_capacity(GpuBpfState* state, int ticks);
+
 /* Latest tick's full statistics (synchronises). */
 int gpu_bpf_get_stats(GpuBpfState* state, BpfTickStats* stats);
 
@@ -201,6 +207,15 @@ int gpu_bpf_synchronize(GpuBpfState* state);
 
 /* Number of ticks processed so far. */
 int gpu_bpf_get_timestep(GpuBpfState* state);
+
+/* Restart at tick 0 without allocating: waits for pending work, redraws x_0
+ * from draw 0 of the particle streams with the current mu, rho and sigma_z,
+ * and resets the host resampling stream, the resample count, the cumulative
+ * log-likelihood, the learner's counters and the band regime.  The ESS
+ * threshold, options and buffers stay.  The next ticks then repeat those of a
+ * filter created with the same arguments.  Returns 0, or -1 with
+ * gpu_bpf_last_error(). */
+int gpu_bpf_restart(GpuBpfState* state);
 
 /* Pathwise score of the log-likelihood with the ancestors held fixed
  * (replaces the reference's autograd path, tests/diff_bpf.py:186-233, and its

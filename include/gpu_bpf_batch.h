@@ -36,8 +36,9 @@ void gpu_bpf_batch_destroy(GpuBpfBatch* batch);
  * autotuner at the first run.  Returns 0 or -1. */
 int gpu_bpf_batch_set_block(GpuBpfBatch* batch, int threads);
 int gpu_bpf_batch_get_block(GpuBpfBatch* batch);
-/* Times each block size over `ticks` ticks of a scratch batch of the same
- * filters on y and keeps the fastest; returns it, or -1. */
+/* Times each block size on a scratch batch of the same filters: two untimed
+ * ticks on y[0..1], then ticks - 2 timed ones on the rest of y (ticks >= 3
+ * values); keeps the fastest and returns it, or -1. */
 int gpu_bpf_batch_autotune(GpuBpfBatch* batch, const float* y, int ticks);
 
 /* Advance every filter by n_ticks observations (host or device array);
@@ -111,6 +112,18 @@ int gpu_smc2_get_loglik(GpuSmc2* smc2, double* ll);
 /* [ticks, outer resamples, moves proposed, moves accepted, replayed ticks,
  *  forward particle-ticks, replayed particle-ticks] */
 int gpu_smc2_get_counters(GpuSmc2* smc2, long long* counters);
+/* Outer-resampling history: for each outer resample so far, the number of
+ * ticks done when it happened and the total particle count of the resampled
+ * theta-particles' filters (the reference prints its resample ticks,
+ * smc2/test_smc2_rbpf.cu).  Fills up to max_entries of each array (either
+ * may be NULL) and returns the number of outer resamples, or -1. */
+int gpu_smc2_get_history(GpuSmc2* smc2, int* ticks, long long* sum_n, int max_entries);
+/* Host only (no GPU): the outer layer's systematic resampling of n
+ * log-weights with uniform u in (0, 1) -- slot m takes the first j whose
+ * running sum of exp(log_w - max) reaches (u + m) / n of the total
+ * (smc2/smc2_rbpf_batch.cu:519-540).  Non-finite log-weights have weight zero
+ * and are never chosen; returns -1 when none is finite. */
+int gpu_smc2_systematic_ancestors(const double* log_w, int n, double u, int* anc);
 int gpu_smc2_set_block(GpuSmc2* smc2, int threads);
 /* Verification aid (host only, no GPU): the rejuvenation moves' proposal and
  * acceptance code on a closed-form 3-d Gaussian target; kind as above;

@@ -28,3 +28,50 @@ def test_moves_leave_gaussian_target_invariant(kind):
 def test_dropped_mala_correction_is_detected():
     _, cov, _ = gpu_bpf_batch.selftest_mcmc("mala", 400_000, seed=3, fault=1)
     assert np.max(np.abs(np.diag(cov) / np.diag(C0) - 1.0)) > 0.2
+
+
+def _brute_ancestors(lw, u):
+    """Systematic resampling by exhaustive search: slot m takes the smallest j
+    whose exact running weight sum reaches (u + m) / n of the total; entries
+    with non-finite log-weights weigh nothing."""
+    lw = np.asarray(lw, np.float64)
+    ok = np.isfinite(lw)
+    w = np.where(ok, np.exp(np.where(ok, lw - lw[ok].max(), 0.0)), 0.0)
+    c = np.cumsum(w)
+    n = lw.size
+    return np.array([int(np.argmax(c >= (u + m) / n * c[-1])) for m in range(n)])
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_outer_ancestors_match_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(1, 200))
+    lw = rng.normal(0.0, 3.0, n) - 500.0
+    if n > 3:
+        lw[rng.integers(0, n, 3)] = [np.nan, -np.inf, np.nan]
+    if not np.isfinite(lw).any():
+        lw[0] = 0.0
+    u = float(rng.uniform(1e-6, 1 - 1e-6))
+    anc = gpu_bpf_batch.systematic_ancestors(lw, u)
+    np.testing.assert_array_equal(anc, _brute_ancestors(lw, u))
+    assert np.all(np.isfinite(lw[anc]))
+    assert np.all(np.diff(anc) >= 0)
+
+
+def test_outer_ancestors_nan_and_inf_never_chosen():
+    lw = np.array([np.nan, 0.0, -np.inf, np.nan, 1.0, np.nan])
+    for u in (1e-9, 0.3, 0.999999):
+        anc = gpu_bpf_batch.systematic_ancestors(lw, u)
+        assert set(anc.tolist()) <= {1, 4}
+    # offspring counts are floor or ceil of n * w
+    w = np.exp(np.array([0.0, 1.0]))
+    w /= w.sum()
+    cnt = np.bincount(gpu_bpf_batch.systematic_ancestors(lw, 0.5), minlength=6)[[1, 4]]
+    assert np.all(np.abs(cnt - 6 * w) < 1.0)
+
+
+def test_outer_ancestors_without_finite_weight_fail():
+    import gpu_bpf
+
+    with pytest.raises(gpu_bpf.GpuBpfError):
+        gpu_bpf_batch.systematic_ancestors([np.nan, -np.inf, np.nan], 0.5)
