@@ -162,29 +162,29 @@ def looped_plain(gpu_bpf, n, mu, phi, sig, seeds, y, ticks, nu=0.0, score=False)
     return pt / tot
 
 
-def looped_plain_streams(gpu_bpf, torch, n, mu, phi, sig, seeds, y, ticks, nu=0.0):
+def looped_plain_streams(gpu_bpf, n, mu, phi, sig, seeds, y, ticks, nu=0.0):
     """The plain filters over parameter sets on one stream each, all enqueued
     before any is waited on (the reference's IMM loop, gpu_bpf/gpu_bpf.cu:
     1033-1038): each filter's ticks replay its captured tick graphs, and the
     filters' streams overlap on the GPU."""
-    y_dev = torch.from_numpy(np.ascontiguousarray(y[:2 + ticks], np.float32)).to("cuda:0")
-    torch.cuda.synchronize()
+    y_dev = gpu_bpf.DeviceArray(y[:2 + ticks])
     fs = []
     for m in range(len(n)):
         f = gpu_bpf.GpuBpf(int(n[m]), float(phi[m]), float(sig[m]), float(mu[m]), nu_obs=nu,
                            seed=int(seeds[m]), ess_threshold=0.0)
-        f.run_device(y_dev.data_ptr(), 2)
+        f.run_device(y_dev.ptr, 2)
         fs.append(f)
     for f in fs:
         f.synchronize()
     t0 = time.perf_counter()
     for f in fs:
-        f.run_device(y_dev.data_ptr() + 8, ticks)
+        f.run_device(y_dev.ptr + 8, ticks)
     for f in fs:
         f.synchronize()
     dt = time.perf_counter() - t0
     for f in fs:
         f.close()
+    y_dev.close()
     return float(np.sum(n[:len(fs)])) * ticks / dt
 
 
@@ -314,7 +314,7 @@ def batch_and_smc2(args, gpu_bpf, gpu_bpf_batch, torch, y, n, mu, phi, sig, seed
         "batch_over_looped": batch_pt / loop_pt,
     }
     if not score:
-        strm = looped_plain_streams(gpu_bpf, torch, n[:M], mu, phi, sig, seeds, y, 200, nu=nu)
+        strm = looped_plain_streams(gpu_bpf, n[:M], mu, phi, sig, seeds, y, 200, nu=nu)
         row["looped_plain_streams_particle_ticks_per_s"] = strm
         row["batch_over_looped_streams"] = batch_pt / strm
         row["looped_streams_sample"] = f"all {M} filters, one stream each, 200 ticks"

@@ -123,6 +123,9 @@ def load(path=LIB_PATH):
         "gpu_bpf_synchronize": (I, [P]),
         "gpu_bpf_get_timestep": (I, [P]),
         "gpu_bpf_restart": (I, [P]),
+        "gpu_bpf_device_upload": (P, [P, I]),
+        "gpu_bpf_device_free": (None, [P]),
+        "gpu_bpf_device_download": (I, [P, P, I]),
         "gpu_bpf_set_stats_capacity": (I, [P, I]),
         "gpu_bpf_enable_kernel_timing": (I, [P, I]),
         "gpu_bpf_score_forward": (I, [P, P, I, I, ctypes.POINTER(D)]),
@@ -339,6 +342,36 @@ class GpuBpf:
         cnt = np.zeros(3, np.int32)
         self._check(self.L.gpu_bpf_kernel_times(self.h, _ptr(ms), _ptr(cnt), 1 if reset else 0))
         return ms, cnt
+
+
+class DeviceArray:
+    """Floats copied to device memory by the library (gpu_bpf_device_upload);
+    .ptr is the device address for the *_run_device entry points."""
+
+    def __init__(self, host):
+        self.L = load()
+        a = np.ascontiguousarray(host, np.float32)
+        self.n = a.size
+        self.ptr = self.L.gpu_bpf_device_upload(_ptr(a), self.n)
+        if not self.ptr:
+            raise GpuBpfError(last_error())
+
+    @classmethod
+    def zeros(cls, n):
+        return cls(np.zeros(int(n), np.float32))
+
+    def download(self):
+        out = np.empty(self.n, np.float32)
+        if self.L.gpu_bpf_device_download(self.ptr, _ptr(out), self.n) < 0:
+            raise GpuBpfError(last_error())
+        return out
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.L.gpu_bpf_device_free(self.ptr)
+            self.ptr = None
+
+    __del__ = close
 
 
 class GpuApf:

@@ -179,6 +179,15 @@ int gpu_bpf_run(GpuBpfState* state, const float* y, int n_ticks,
  * and can be fetched with gpu_bpf_copy_stats. */
 int gpu_bpf_run_device(GpuBpfState* state, const float* y_dev, int n_ticks);
 
+/* Device copy of n host floats (an observation series for
+ * gpu_bpf_run_device or gpu_bpf_batch_run_device, for callers without a
+ * device allocator of their own); NULL on failure.  Free it with
+ * gpu_bpf_device_free (NULL-safe). */
+float* gpu_bpf_device_upload(const float* host, int n);
+void gpu_bpf_device_free(float* dev);
+/* Copy n floats of device memory to the host (synchronous); 0 or -1. */
+int gpu_bpf_device_download(const float* dev, float* host, int n);
+
 /* Copy the statistics of the last n ticks (n <= ring capacity) to the host. */
 int gpu_bpf_copy_stats(GpuBpfState* state, BpfTickStats* stats, int n);
 

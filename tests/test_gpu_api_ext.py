@@ -73,17 +73,15 @@ def test_ring_regrow_keeps_records():
 
 
 def test_stats_capacity_covers_a_device_run():
-    import torch
-
     y = _y(3000)
     say("stats capacity")
     f = gpu_bpf.GpuBpf(8192, 0.97, 0.15, -1.0, seed=4, ess_threshold=0.5)
     f.set_stats_capacity(3000)
-    yd = torch.from_numpy(y).to("cuda:0")
-    torch.cuda.synchronize()
-    f.run_device(yd.data_ptr(), 3000)
+    yd = gpu_bpf.DeviceArray(y)
+    f.run_device(yd.ptr, 3000)
     st = f.stats_tail(3000)
     f.close()
+    yd.close()
     g = gpu_bpf.GpuBpf(8192, 0.97, 0.15, -1.0, seed=4, ess_threshold=0.5)
     _, ref = g.run(y)
     g.close()
