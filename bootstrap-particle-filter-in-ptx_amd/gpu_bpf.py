@@ -15,6 +15,8 @@ import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 REPO_DIR = os.path.dirname(PKG_DIR)
+# largest particle count of gpu_bpf_run_device_noalloc (GPU_BPF_NOALLOC_MAX_N)
+NOALLOC_MAX_N = 4194304
 LIB_PATH = os.path.join(PKG_DIR, "_build", "libgpubpf.so")
 HEADER_PATH = os.path.join(REPO_DIR, "include", "gpu_bpf.h")
 
@@ -126,6 +128,7 @@ def load(path=LIB_PATH):
         "gpu_bpf_device_upload": (P, [P, I]),
         "gpu_bpf_device_free": (None, [P]),
         "gpu_bpf_device_download": (I, [P, P, I]),
+        "gpu_bpf_run_device_noalloc": (I, [I, F, F, F, F, I, F, P, I, P, P, P]),
         "gpu_bpf_set_stats_capacity": (I, [P, I]),
         "gpu_bpf_enable_kernel_timing": (I, [P, I]),
         "gpu_bpf_score_forward": (I, [P, P, I, I, ctypes.POINTER(D)]),

@@ -191,6 +191,27 @@ int gpu_bpf_device_download(const float* dev, float* host, int n);
 /* Copy the statistics of the last n ticks (n <= ring capacity) to the host. */
 int gpu_bpf_copy_stats(GpuBpfState* state, BpfTickStats* stats, int n);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the allocation-free variant of gpu_bpf_run_device.  It
+ * runs a fresh filter -- the one gpu_bpf_create(n_particles, rho, sigma_z,
+ * mu, 0, nu_obs, seed) with gpu_bpf_set_ess_threshold(ess_threshold) would
+ * be -- over n_ticks observations already in device memory, on `stream`
+ * (a hipStream_t; NULL = the default stream), and enqueues only.  Outputs, in
+ * caller-owned device memory: the n_particles final particle states in
+ * x_out_dev and, unless stats_out_dev is NULL, one BpfTickStats per tick.
+ * Results equal gpu_bpf_run_device's on such a filter bit for bit.  Working
+ * memory is one library-held device array of 52 MiB per GPU (held until the
+ * process ends), so n_particles is at most GPU_BPF_NOALLOC_MAX_N; nothing is
+ * allocated or freed per call.  Calls that overlap on one GPU share that
+ * array and race: serialise them.  Returns 0, or -1 with
+ * gpu_bpf_last_error() (bad sizes are rejected before anything is
+ * enqueued). */
+#define GPU_BPF_NOALLOC_MAX_N 4194304
+int gpu_bpf_run_device_noalloc(int n_particles, float rho, float sigma_z, float mu, float nu_obs,
+                               int seed, float ess_threshold, const float* y_dev, int n_ticks,
+                               float* x_out_dev, BpfTickStats* stats_out_dev, void* stream);
+
 /* Grow the device ring of per-tick statistics to hold at least `ticks`
  * ticks (default 1024; gpu_bpf_run grows it to its run length).  Records
  * already held are kept.  Allocates; call it before a run of
