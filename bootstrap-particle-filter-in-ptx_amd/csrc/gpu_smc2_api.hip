@@ -884,6 +884,21 @@ int gpu_smc2_get_counters(GpuSmc2* s, long long* out) {
     return 0;
 }
 
+int gpu_smc2_get_score(GpuSmc2* s, double* score) {
+    if (!s || !s->ready || !score) {
+        set_err("gpu_smc2_get_score: not initialised or null");
+        return -1;
+    }
+    if (!s->main->d_score) {
+        set_err("gpu_smc2_get_score: the theta-particles carry scores only with MALA rejuvenation");
+        return -1;
+    }
+    std::vector<double> g;
+    if (read_score(s->main, s->M, g)) return -1;
+    std::copy(g.begin(), g.end(), score);
+    return 0;
+}
+
 int gpu_smc2_get_history(GpuSmc2* s, int* ticks, long long* sum_n, int max_entries) {
     if (!s || max_entries < 0) {
         set_err("gpu_smc2_get_history: null state or negative size");

@@ -169,6 +169,7 @@ def _smc2_lib():
             "gpu_smc2_set_mala_step": (I, [P, F]),
             "gpu_smc2_selftest_mcmc": (I, [I, I, I, I, P]),
             "gpu_smc2_get_history": (I, [P, P, P, I]),
+            "gpu_smc2_get_score": (I, [P, P]),
             "gpu_smc2_systematic_ancestors": (I, [P, I, ctypes.c_double, P]),
         }
         for name, (res, args) in sig.items():
@@ -257,6 +258,12 @@ class GpuSmc2:
         c = np.zeros(7, np.int64)
         self._check(self.L.gpu_smc2_get_counters(self.h, _ptr(c)))
         return dict(zip(COUNTER_NAMES, c.tolist()))
+
+    def score(self):
+        """MALA runs: per theta-particle pathwise score [M, 3]."""
+        sc = np.zeros((self.M, 3), np.float64)
+        self._check(self.L.gpu_smc2_get_score(self.h, _ptr(sc)))
+        return sc
 
     def history(self):
         """Outer resamples so far: (ticks done at each, total particle count of

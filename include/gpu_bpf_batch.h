@@ -79,6 +79,17 @@ int gpu_bpf_batch_get_score(GpuBpfBatch* batch, double* score);
  * rho = 0.99; smc2_rbpf_batch.cu:693-981,1246-1294).  The inner noise of the
  * moves is stored on the device (capacity in ticks, default 2048). */
 typedef struct GpuSmc2 GpuSmc2;
+/* Random numbers of the outer layer (so that runs repeat and can be checked):
+ * PCG32 XSH-RR streams seeded (state 0, increment 2 id + 1, one step, + seed,
+ * one step) with id 0x5eed5c2 for the prior draws (per theta-particle:
+ * log-uniform particle count, mu, phi, sigma_eta, filter seed), the
+ * outer-resampling uniform and the fresh filter seeds after it, and id 0x30e5
+ * for the moves (per theta-particle: three proposal normals, a proposal seed;
+ * then one MH uniform each); uniforms ((r1 >> 5) 2^26 + (r2 >> 6) + 0.5) 2^-53,
+ * normals by Box-Muller (cosine branch).  A filter with seed s uses the
+ * particle streams and host resampling stream of gpu_bpf_create(..., s);
+ * CPMMH proposals draw their normals from the particle streams of the
+ * proposal seed and their uniform-walk steps from its stream index 16384. */
 
 GpuSmc2* gpu_smc2_create(int n_theta, int n_min, int n_max, float nu_obs, int seed);
 void gpu_smc2_destroy(GpuSmc2* smc2);
@@ -112,6 +123,10 @@ int gpu_smc2_get_loglik(GpuSmc2* smc2, double* ll);
 /* [ticks, outer resamples, moves proposed, moves accepted, replayed ticks,
  *  forward particle-ticks, replayed particle-ticks] */
 int gpu_smc2_get_counters(GpuSmc2* smc2, long long* counters);
+/* MALA runs: each theta-particle's pathwise score of its filter's
+ * log-likelihood with respect to (mu, phi, sigma_eta), n_theta x 3 doubles
+ * (the drift of the next move); -1 for CPMMH runs. */
+int gpu_smc2_get_score(GpuSmc2* smc2, double* score);
 /* Outer-resampling history: for each outer resample so far, the number of
  * ticks done when it happened and the total particle count of the resampled
  * theta-particles' filters (the reference prints its resample ticks,

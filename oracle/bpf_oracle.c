@@ -21,6 +21,10 @@ static inline float f32_bits(uint32_t b) { float f; memcpy(&f, &b, 4); return f;
  * on a plausible error (0 = none).  See orc_set_fault in bpf_oracle.h. */
 static int g_fault = 0;
 void orc_set_fault(int code) { g_fault = code; }
+
+/* Threads of orc_bpf_score (1 = sequential, the default). */
+static int g_threads = 1;
+void orc_set_threads(int n) { g_threads = n > 1 ? n : 1; }
 static inline uint32_t u32_of(float f) { uint32_t b; memcpy(&b, &f, 4); return b; }
 
 /* ========================================================================= */
@@ -687,6 +691,7 @@ static int bpf_run_impl(const OrcParams* p, int mode, const float* y, const floa
         if (do64) {
             const double mud = mu, rhod = rho, sigd = sig;
             double gm = -INFINITY;
+#pragma omp parallel for num_threads(n_threads) if (n_threads > 1) schedule(static) reduction(max:gm)
             for (int i = 0; i < n; i++) {
                 if (t > 0) {
                     double z = (double)normal_j(jt, (uint32_t)i);
@@ -697,6 +702,7 @@ static int bpf_run_impl(const OrcParams* p, int mode, const float* y, const floa
                 if (ld[i] > gm) gm = ld[i];
             }
             double* wt = do32 ? wd : w;
+#pragma omp parallel for num_threads(n_threads) if (n_threads > 1) schedule(static)
             for (int i = 0; i < n; i++) wt[i] = (gm == -INFINITY) ? 0.0 : exp(ld[i] - gm);
             M = gm;
             md_prev = gm;
@@ -731,6 +737,7 @@ static int bpf_run_impl(const OrcParams* p, int mode, const float* y, const floa
         double S32 = S, Q32 = Q;
         if (do64 && do32) {
             S32 = 0.0; Q32 = 0.0;
+#pragma omp parallel for num_threads(n_threads) if (n_threads > 1) schedule(static) reduction(+:S32,Q32)
             for (int i = 0; i < n; i++) { S32 += w[i]; Q32 += w[i] * w[i]; }
         }
 
@@ -810,6 +817,7 @@ static int bpf_run_impl(const OrcParams* p, int mode, const float* y, const floa
                 }
             }
             if (do64) {
+#pragma omp parallel for num_threads(n_threads) if (n_threads > 1) schedule(static)
                 for (int i = 0; i < n; i++) xdt[i] = xd[anc[i]];
                 double* sw = xd; xd = xdt; xdt = sw;
             }
@@ -1100,6 +1108,7 @@ int orc_bpf_score(const OrcParams* p, const float* y, int n_ticks, int x0_repara
     const int tile = p->tile;
     const int n_tiles = (n + tile - 1) / tile;
     const int replay = anc_in != NULL;
+    const int nt = g_threads;
     /* FP64 parameters: the FP32 ones, or theta64 (finite differences) */
     const double mud = theta64 ? theta64[0] : (double)p->mu;
     const double rhod = theta64 ? theta64[1] : (double)p->rho;
@@ -1144,6 +1153,7 @@ int orc_bpf_score(const OrcParams* p, const float* y, int n_ticks, int x0_repara
             dsd_dphi = 0.0;
         }
         PcgJump j0 = pcg_jump(p->seed, 0);
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static)
         for (int i = 0; i < n; i++) {
             float z = normal_j(j0, (uint32_t)i);
             x[i] = fmaf(sd, z, mu);
@@ -1161,6 +1171,7 @@ int orc_bpf_score(const OrcParams* p, const float* y, int n_ticks, int x0_repara
         const float ysq = yt * yt;
         const PcgJump jt = pcg_jump(p->seed, (uint32_t)t);
         /* propagate (FP32 restatement for the genealogy, FP64 + tangents) */
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static)
         for (int i = 0; i < n; i++) {
             if (t > 0) {
                 float z = normal_j(jt, (uint32_t)i);
@@ -1180,14 +1191,17 @@ int orc_bpf_score(const OrcParams* p, const float* y, int n_ticks, int x0_repara
         }
         /* FP64 weights, log-likelihood and score increment */
         double M = -INFINITY;
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static) reduction(max:M)
         for (int i = 0; i < n; i++) {
             lw[i] = lw_f64(xd[i], (double)yt, (double)nu_obs, c_obs_d);
             if (lw[i] > M) M = lw[i];
         }
         double S = 0.0;
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static) reduction(+:S)
         for (int i = 0; i < n; i++) S += exp(lw[i] - M);
         ll += M + log(S / (double)n);
         double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static) reduction(+:a0,a1,a2)
         for (int i = 0; i < n; i++) {
             double Wg = exp(lw[i] - M) / S * dlw_dx_f64(xd[i], (double)yt, (double)nu_obs);
             a0 += Wg * D[3 * i + 0];
@@ -1200,20 +1214,37 @@ int orc_bpf_score(const OrcParams* p, const float* y, int n_ticks, int x0_repara
         if (replay) {
             at = anc_in + (size_t)t * (size_t)n;
         } else {
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static)
             for (int i = 0; i < n; i++) l[i] = lw_f32(x[i], ysq, nu_obs, c_obs, nph);
-            float gm = tiled_weights(l, n, tile, tmax, tfd, w, 1);
+            float gm = tiled_weights(l, n, tile, tmax, tfd, w, nt);
             double S32 = 0.0;
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static) reduction(+:S32)
             for (int i = 0; i < n; i++) S32 += w[i];
             ll32 += (double)gm + log(S32 / (double)n);
             uint32_t hr = orc_pcg_host_next(&hrng);
             float u = ((float)(hr >> 9) * (1.0f / 8388608.0f)) / (float)n;
             build_cdf(w, n, cdf);
-            orc_resample_ancestors(cdf, n, u, anc);
+            if (nt > 1) {
+#pragma omp parallel for num_threads(nt) schedule(static)
+                for (int i = 0; i < n; i++) {
+                    const double tt = (double)orc_target(u, i, n);
+                    int lo = 0, hi = n - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (g_fault == 3 ? (cdf[mid] <= tt) : (cdf[mid] < tt)) lo = mid + 1; else hi = mid;
+                    }
+                    anc[i] = lo;
+                }
+            } else {
+                orc_resample_ancestors(cdf, n, u, anc);
+            }
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static)
             for (int i = 0; i < n; i++) xt[i] = x[anc[i]];
             float* sw = x; x = xt; xt = sw;
             at = anc;
         }
         if (anc_out) memcpy(anc_out + (size_t)t * (size_t)n, at, sizeof(int) * (size_t)n);
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static)
         for (int i = 0; i < n; i++) {
             int a = at[i];
             xdt[i] = xd[a];
@@ -1230,4 +1261,149 @@ int orc_bpf_score(const OrcParams* p, const float* y, int n_ticks, int x0_repara
     free(x); free(xt); free(l); free(w); free(cdf); free(anc); free(xd); free(xdt);
     free(D); free(Dt); free(lw); free(tmax); free(tfd);
     return 0;
+}
+
+/* ========================================================================= */
+/* Helpers shared with smc2_oracle.c (the batched inner filters' restatement) */
+/* ========================================================================= */
+
+void orc_pcg_jump(int seed, uint32_t k, uint64_t* a, uint64_t* g) {
+    PcgJump j = pcg_jump(seed, k);
+    *a = j.a;
+    *g = j.g;
+}
+
+float orc_lw(float x, float ysq, float nu_obs, float c_obs, float nph) {
+    return lw_f32(x, ysq, nu_obs, c_obs, nph);
+}
+
+float orc_tiled_weights(const float* l, int n, int tile, float* tmax, double* tf, double* w) {
+    return tiled_weights(l, n, tile, tmax, tf, w, 1);
+}
+
+void orc_build_cdf(const double* w, int n, double* cdf) { build_cdf(w, n, cdf); }
+
+double orc_c_obs(double nu) { return c_obs_f64(nu); }
+
+/* ========================================================================= */
+/* One tick from a given state (plain filter), for tick-by-tick checks of    */
+/* the GPU at sizes where whole-run parity would be broken by a near-tie     */
+/* ancestor (gpu_bpf/gpu_bpf_ptx_full.cu:591-748 for one tick).               */
+/* ========================================================================= */
+
+int orc_bpf_tick(const OrcParams* p, float mu, int t, float y, int carry, uint64_t* hrng, float* x,
+                 float* l, float* x_pre, int* anc, OrcTick* out, double* ll64, int n_threads) {
+    const int n = p->n_particles, tile = p->tile;
+    if (n <= 0 || tile <= 0 || t < 0 || !hrng || !x || !l || !x_pre || !anc || !out) return -1;
+#ifdef _OPENMP
+    if (n_threads < 1) n_threads = 1;
+#else
+    (void)n_threads;
+#endif
+    const int n_tiles = (n + tile - 1) / tile;
+    float* tmax = (float*)malloc(sizeof(float) * (size_t)n_tiles);
+    double* tf = (double*)malloc(sizeof(double) * (size_t)n_tiles);
+    double* w = (double*)malloc(sizeof(double) * (size_t)n);
+    double* cdf = (double*)malloc(sizeof(double) * (size_t)n);
+    if (!tmax || !tf || !w || !cdf) {
+        free(tmax); free(tf); free(w); free(cdf);
+        return -1;
+    }
+    const float rho = p->rho, sig = p->sigma_z, nu_obs = p->nu_obs;
+    const double c_obs_d = nu_obs > 0.0f ? c_obs_f64((double)nu_obs) : 0.0;
+    const float c_obs = (float)c_obs_d, nph = (nu_obs + 1.0f) * 0.5f;
+    const float ysq = y * y;
+    /* carried weights: m_prev = their max, s_prev = their tiled FP64 sum */
+    float m_prev = 0.0f;
+    double s_prev = 1.0, s_prev64 = 1.0;
+    if (carry) {
+        m_prev = tiled_weights(l, n, tile, tmax, tf, w, n_threads);
+        double a = 0.0, b = 0.0;
+#pragma omp parallel for num_threads(n_threads) schedule(static) reduction(+:a,b)
+        for (int i = 0; i < n; i++) {
+            a += w[i];
+            b += exp((double)l[i] - (double)m_prev);
+        }
+        s_prev = a;
+        s_prev64 = b;
+    }
+    const PcgJump jt = pcg_jump(p->seed, (uint32_t)t);
+    double gm64 = -INFINITY;
+#pragma omp parallel for num_threads(n_threads) schedule(static) reduction(max:gm64)
+    for (int i = 0; i < n; i++) {
+        float xi = x[i];
+        double xd = (double)xi;
+        if (t > 0) {
+            const float z = normal_j(jt, (uint32_t)i);
+            float d = xi - mu;
+            d = rho * d;
+            xi = mu + d;
+            xi = fmaf(sig, z, xi);
+            xd = (double)mu + (double)rho * (xd - (double)mu) + (double)sig * (double)z;
+        }
+        x_pre[i] = xi;
+        const float lw = lw_f32(xi, ysq, nu_obs, c_obs, nph);
+        const float lprev = l[i];
+        l[i] = carry ? lw + (lprev - m_prev) : lw;
+        /* FP64 log-weight of the same state and noise, carried on the FP32 weights */
+        double l64 = lw_f64(xd, (double)y, (double)nu_obs, c_obs_d);
+        if (carry) l64 += (double)lprev - (double)m_prev;
+        w[i] = l64;           /* staged: FP64 log-weights */
+        if (l64 > gm64) gm64 = l64;
+    }
+    double S64 = 0.0;
+#pragma omp parallel for num_threads(n_threads) schedule(static) reduction(+:S64)
+    for (int i = 0; i < n; i++) S64 += exp(w[i] - gm64);
+    if (ll64) *ll64 = carry ? (gm64 + log(S64)) - log(s_prev64) : gm64 + log(S64 / (double)n);
+    const float gm = tiled_weights(l, n, tile, tmax, tf, w, n_threads);
+    const double M = (double)gm;
+    double S = 0.0, SX = 0.0, SXX = 0.0, Q = 0.0;
+#pragma omp parallel for num_threads(n_threads) schedule(static) reduction(+:S,SX,SXX,Q)
+    for (int i = 0; i < n; i++) {
+        const double xi = (double)x_pre[i], wi = w[i];
+        S += wi;
+        SX += wi * xi;
+        SXX += wi * xi * xi;
+        Q += wi * wi;
+    }
+    OrcTick r;
+    memset(&r, 0, sizeof r);
+    r.tick = t;
+    const int ok = isfinite(S) && S > 0.0 && isfinite(M);
+    if (ok) {
+        r.mean = SX / S;
+        r.var = SXX / S - r.mean * r.mean;
+        r.ess = (S * S) / Q;
+        r.ll_incr = carry ? (M + log(S)) - log(s_prev) : M + log(S / (double)n);
+    } else {
+        r.mean = r.var = r.ess = r.ll_incr = NAN;
+    }
+    r.h_mean = (float)r.mean;
+    const int resample = ok && (p->ess_threshold <= 0.0f || (r.ess / (double)n) < (double)p->ess_threshold);
+    r.resampled = resample;
+    if (resample) {
+        const uint32_t hr = orc_pcg_host_next(hrng);
+        const float u = ((float)(hr >> 9) * (1.0f / 8388608.0f)) / (float)n;
+        build_cdf(w, n, cdf);
+#pragma omp parallel for num_threads(n_threads) schedule(static)
+        for (int i = 0; i < n; i++) {
+            const double tt = (double)orc_target(u, i, n);
+            int lo = 0, hi = n - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (cdf[mid] < tt) lo = mid + 1; else hi = mid;
+            }
+            anc[i] = lo;
+            x[i] = x_pre[lo];
+        }
+    } else {
+#pragma omp parallel for num_threads(n_threads) schedule(static)
+        for (int i = 0; i < n; i++) {
+            anc[i] = i;
+            x[i] = x_pre[i];
+        }
+    }
+    *out = r;
+    free(tmax); free(tf); free(w); free(cdf);
+    return resample;
 }

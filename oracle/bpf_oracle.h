@@ -194,6 +194,36 @@ int orc_bpf_score(const OrcParams* p, const float* y, int n_ticks, int x0_repara
  * Process-global. */
 void orc_set_fault(int code);
 
+/* Threads of orc_bpf_score's loops (1, the default: sequential, fixed
+ * summation order; more: OpenMP reductions). */
+void orc_set_threads(int n);
+
+/* One tick of the FP32 restatement (plain filter) from a given state: x (in:
+ * particles, out: after this tick's resampling if any), l (in: carried
+ * log-weights when carry, out: this tick's log-weights), x_pre (out: the
+ * propagated states before resampling), anc (out: ancestors, identity
+ * without resampling), *hrng the host resampling stream (advanced when
+ * resampling), draw t of the particle streams (tick 0 does not propagate).
+ * The carried maximum and sum come from l.  out receives the statistics
+ * (ll_ref and ll_cum are not set); ll64 (nullable) the FP64 evaluation of the
+ * tick's log-likelihood increment from the same state and normals.
+ * Returns 1 if the tick resampled, 0 if not, -1 on bad arguments. */
+int orc_bpf_tick(const OrcParams* p, float mu, int t, float y, int carry, uint64_t* hrng, float* x,
+                 float* l, float* x_pre, int* anc, OrcTick* out, double* ll64, int n_threads);
+
+/* ---- helpers shared with smc2_oracle.c ------------------------------------ */
+/* PCG32 jump constants of draw k: state of particle i = a + (2i + 1) g. */
+void   orc_pcg_jump(int seed, uint32_t k, uint64_t* a, uint64_t* g);
+/* FP32 observation log-weight (Gaussian, or Student-t with nu_obs > 0). */
+float  orc_lw(float x, float ysq, float nu_obs, float c_obs, float nph);
+/* Tile-relative FP64 weights w (tiles of `tile`, groups of 256 tiles);
+ * returns the global max log-weight. */
+float  orc_tiled_weights(const float* l, int n, int tile, float* tmax, double* tf, double* w);
+/* Normalised CDF of the weights (80-bit running sum, one FP64 rounding). */
+void   orc_build_cdf(const double* w, int n, double* cdf);
+/* Student-t log-density constant C_obs in FP64. */
+double orc_c_obs(double nu);
+
 /* Exact log p(y_{1:T}) of the Gaussian SV model by a point-mass filter. */
 double orc_grid_loglik(const float* y, int n_ticks, double rho, double sigma_z,
                        double mu, int n_grid, double* ll_t);

@@ -85,6 +85,19 @@ class OrcRbCount(ctypes.Structure):
                 ("t_cp", ctypes.c_int)]
 
 
+class OrcSmc2Cfg(ctypes.Structure):
+    _fields_ = [("n_theta", ctypes.c_int), ("n_min", ctypes.c_int), ("n_max", ctypes.c_int),
+                ("seed", ctypes.c_int), ("kind", ctypes.c_int), ("moves", ctypes.c_int),
+                ("nu_obs", ctypes.c_float), ("cn_rho", ctypes.c_float), ("u_step", ctypes.c_float),
+                ("prop_scale", ctypes.c_float), ("mala_h", ctypes.c_float),
+                ("outer_thr", ctypes.c_float), ("lo", ctypes.c_float * 3), ("hi", ctypes.c_float * 3)]
+
+
+class OrcSmc2Count(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_longlong) for k in ("ticks", "outer_resamples", "proposed", "accepted",
+                                                 "replay_ticks", "pt_forward", "pt_replay")]
+
+
 # Defaults of smc2/smc2_rbpf_batch.cu:1080-1110 (prior, bounds, curves, proposal std)
 RB_PRIOR = (0.95, 0.02, 0.1, 0.1, -1.0, 0.5, 0.15, 0.05)
 RB_BOUNDS = (0.8, 0.999, 0.01, 1.0, -10.0, 5.0, 0.01, 1.0)
@@ -177,6 +190,22 @@ def lib():
         L.orc_bpf_score.argtypes = [ctypes.POINTER(OrcParams), P, i32, i32, P, P, P, P, P, P]
         L.orc_set_fault.restype = None
         L.orc_set_fault.argtypes = [i32]
+        L.orc_smc2_set_fault.restype = None
+        L.orc_smc2_set_fault.argtypes = [i32]
+        L.orc_sv_smc2_run.restype = i32
+        L.orc_sv_smc2_run.argtypes = [ctypes.POINTER(OrcSmc2Cfg), P, i32, P, P, P, P, P, P, P, P, i32,
+                                      ctypes.POINTER(OrcSmc2Count)]
+        L.orc_set_threads.restype = None
+        L.orc_set_threads.argtypes = [i32]
+        L.orc_bpf_tick.restype = i32
+        L.orc_bpf_tick.argtypes = [ctypes.POINTER(OrcParams), f32, i32, f32, i32, ctypes.POINTER(u64), P, P,
+                                   P, P, P, ctypes.POINTER(f64), i32]
+        L.orc_smc2_ancestors.restype = i32
+        L.orc_smc2_ancestors.argtypes = [P, i32, f64, P]
+        L.orc_smc2_selftest.restype = i32
+        L.orc_smc2_selftest.argtypes = [i32, i32, i32, P]
+        L.orc_sv_inner_run.restype = f64
+        L.orc_sv_inner_run.argtypes = [i32, f32, f32, f32, f32, i32, P, i32, P, P]
         L.orc_math_array.restype = None
         L.orc_math_array.argtypes = [i32, P, P, i32]
         L.orc_grid_loglik.restype = f64
@@ -414,7 +443,7 @@ def learn_update(sums, k, rm_step, mu, rho, learn_rho=True, c=0.1, t0=10.0, gamm
 
 
 def score(y, n_particles, rho=0.97, sigma_z=0.15, mu=-1.0, nu_obs=0.0, seed=42, tile=256,
-          x0_reparam=True, theta64=None, anc_in=None, want_anc=False):
+          x0_reparam=True, theta64=None, anc_in=None, want_anc=False, threads=1):
     """Pathwise score of the always-resample filter with frozen ancestors.
     Returns (ll64, score[3], ll32, anc[T,N] or None)."""
     y = np.ascontiguousarray(y, np.float32)
@@ -426,8 +455,12 @@ def score(y, n_particles, rho=0.97, sigma_z=0.15, mu=-1.0, nu_obs=0.0, seed=42, 
     ll = ctypes.c_double()
     ll32 = ctypes.c_double()
     sc = np.zeros(3, np.float64)
-    rc = lib().orc_bpf_score(ctypes.byref(p), _ptr(y), T, 1 if x0_reparam else 0, _ptr(th),
-                             _ptr(ai), _ptr(ao), ctypes.byref(ll), _ptr(sc), ctypes.byref(ll32))
+    lib().orc_set_threads(int(threads))
+    try:
+        rc = lib().orc_bpf_score(ctypes.byref(p), _ptr(y), T, 1 if x0_reparam else 0, _ptr(th),
+                                 _ptr(ai), _ptr(ao), ctypes.byref(ll), _ptr(sc), ctypes.byref(ll32))
+    finally:
+        lib().orc_set_threads(1)
     if rc < 0:
         raise ValueError("oracle rejected the arguments")
     return ll.value, sc, ll32.value, ao
@@ -567,3 +600,100 @@ def rb_half_roundtrip(x):
     out = np.empty_like(x)
     lib().orc_rb_half_roundtrip(_ptr(x), _ptr(out), len(x))
     return out
+
+
+def run_sv_smc2(y, n_theta, n_min, n_max, seed=42, kind="cpmmh", moves=5, nu_obs=0.0, cn_rho=0.99,
+                u_step=0.05, theta_scale=1.0, mala_step=0.5, outer_threshold=0.5,
+                prior=((-2.0, 0.90, 0.05), (0.0, 0.995, 0.40)), fault=0):
+    """SMC^2 over batched SV bootstrap filters (smc2_oracle.c), with the
+    arguments of gpu_bpf_batch.GpuSmc2.  Returns a dict: theta [M, 3], n [M],
+    logw [M], ll [M], score [M, 3], ess [T], history (ticks, sum_n), counters."""
+    y = np.ascontiguousarray(y, np.float32)
+    T, M = len(y), int(n_theta)
+    c = OrcSmc2Cfg()
+    c.n_theta, c.n_min, c.n_max, c.seed = M, int(n_min), int(n_max), int(seed)
+    c.kind = 1 if kind == "mala" else 0
+    c.moves = int(moves)
+    c.nu_obs, c.cn_rho, c.u_step = float(nu_obs), float(cn_rho), float(u_step)
+    c.prop_scale, c.mala_h, c.outer_thr = float(theta_scale), float(mala_step), float(outer_threshold)
+    for k in range(3):
+        c.lo[k], c.hi[k] = float(prior[0][k]), float(prior[1][k])
+    th = np.zeros((M, 3), np.float32)
+    n = np.zeros(M, np.int32)
+    lw = np.zeros(M, np.float64)
+    ll = np.zeros(M, np.float64)
+    sc = np.zeros((M, 3), np.float64)
+    ess = np.zeros(max(T, 1), np.float64)
+    cap = max(T, 1)
+    ht = np.zeros(cap, np.int32)
+    hs = np.zeros(cap, np.int64)
+    cnt = OrcSmc2Count()
+    L = lib()
+    L.orc_smc2_set_fault(int(fault))
+    try:
+        k = L.orc_sv_smc2_run(ctypes.byref(c), _ptr(y), T, _ptr(th), _ptr(n), _ptr(lw), _ptr(ll), _ptr(sc),
+                              _ptr(ess), _ptr(ht), _ptr(hs), cap, ctypes.byref(cnt))
+    finally:
+        L.orc_smc2_set_fault(0)
+    if k < 0:
+        raise RuntimeError("orc_sv_smc2_run failed")
+    return {"theta": th, "n": n, "logw": lw, "ll": ll, "score": sc, "ess": ess[:T],
+            "history": (ht[:k], hs[:k]),
+            "counters": {f: getattr(cnt, f) for f, _ in OrcSmc2Count._fields_}}
+
+
+def sv_inner_run(y, n, mu, phi, sig, nu_obs=0.0, seed=42):
+    """One batched inner filter (tiles of 256, every tick resampled, PCG32
+    noise) with its FP32 tangents: (log-likelihood, pathwise score[3], x)."""
+    y = np.ascontiguousarray(y, np.float32)
+    sc = np.zeros(3, np.float64)
+    x = np.zeros(int(n), np.float32)
+    ll = lib().orc_sv_inner_run(int(n), float(mu), float(phi), float(sig), float(nu_obs), int(seed), _ptr(y),
+                                len(y), _ptr(sc), _ptr(x))
+    return ll, sc, x
+
+
+def smc2_ancestors(log_w, u):
+    lw = np.ascontiguousarray(log_w, np.float64)
+    anc = np.zeros(lw.size, np.int32)
+    if lib().orc_smc2_ancestors(_ptr(lw), lw.size, float(u), _ptr(anc)) < 0:
+        raise ValueError("no finite outer log-weight")
+    return anc
+
+
+def smc2_selftest(kind, iters, seed=1, fault=0):
+    """The oracle's move code on closed-form targets: kind 'rw' / 'mala' give
+    (mean[3], cov[3, 3], acceptance) of a chain on N(m, C); kind 'cn' gives
+    (mean, variance, lag-1 correlation) of the Crank-Nicolson noise chain."""
+    out = np.zeros(13, np.float64)
+    L = lib()
+    L.orc_smc2_set_fault(int(fault))
+    try:
+        rc = L.orc_smc2_selftest({"rw": 0, "mala": 1, "cn": 2}[kind], int(iters), int(seed), _ptr(out))
+    finally:
+        L.orc_smc2_set_fault(0)
+    if rc < 0:
+        raise ValueError("bad selftest arguments")
+    if kind == "cn":
+        return out[0], out[1], out[2]
+    return out[:3], out[3:12].reshape(3, 3), out[12]
+
+
+def filter_tick(x, l, t, y, carry, hrng, n_particles, mu, rho=0.97, sigma_z=0.15, nu_obs=0.0,
+                ess_threshold=0.5, seed=42, tile=4096, threads=1):
+    """One tick of the FP32 restatement from the state (x, l, carry, host
+    stream hrng).  x and l are updated in place; returns (stats record,
+    resampled, x_pre, anc, FP64 ll_incr, new hrng)."""
+    n = int(n_particles)
+    assert x.dtype == np.float32 and l.dtype == np.float32 and x.size == n and l.size == n
+    p = OrcParams(n, rho, sigma_z, mu, nu_obs, ess_threshold, seed, tile)
+    x_pre = np.empty(n, np.float32)
+    anc = np.empty(n, np.int32)
+    out = np.zeros(1, TICK_DTYPE)
+    h = ctypes.c_uint64(int(hrng))
+    ll64 = ctypes.c_double()
+    rc = lib().orc_bpf_tick(ctypes.byref(p), float(mu), int(t), float(y), int(carry), ctypes.byref(h), _ptr(x),
+                            _ptr(l), _ptr(x_pre), _ptr(anc), _ptr(out), ctypes.byref(ll64), int(threads))
+    if rc < 0:
+        raise ValueError("oracle rejected the arguments")
+    return out[0], bool(rc), x_pre, anc, ll64.value, h.value

@@ -372,3 +372,43 @@ def test_planted_resample_fault_is_caught(golden_dir, orc):
             if orc.resample_ancestors(c["cdf"], c["u"]).tolist() != brute_ancestors(np.array(c["cdf"]), c["u"]):
                 bad += 1
     assert bad > 0
+
+
+@pytest.mark.parametrize("nu,tile", [(0.0, 256), (5.0, 4096)])
+def test_tick_by_tick_stepping_repeats_the_whole_run(orc, nu, tile):
+    """orc_bpf_tick from its own state, tick after tick (carried weights,
+    host stream advanced per resample, a mu switch), repeats orc_bpf_run:
+    particles bit for bit, statistics to 1e-12; its FP64 increments agree
+    with the FP64-coupled run's to 1e-6 (same genealogy, FP32 vs FP64 state)."""
+    import sv_data
+
+    T, n = 80, 9000
+    mu_t = np.where(np.arange(T) < 40, -1.0, -0.4).astype(np.float32)
+    y = sv_data.sv_series(T, 0.97, 0.15, mu_t.astype(np.float64), nu_obs=nu, seed=3)[1].astype(np.float32)
+    ref, _, xo, _ = orc.run_filter(y, n, nu_obs=nu, ess_threshold=0.5, tile=tile, mu_t=mu_t, seed=9,
+                                   want_x=True)
+    ref64 = orc.run_filter(y, n, nu_obs=nu, ess_threshold=0.5, tile=tile, mu_t=mu_t, seed=9,
+                           mode=orc.MODE_F64_COUPLED, threads=4)[0]
+    # x_0 from the oracle itself (zero ticks)
+    _, _, x, _ = orc.run_filter(y[:0], n, ess_threshold=0.5, tile=tile, mu=float(mu_t[0]), seed=9, want_x=True)
+    l = np.zeros(n, np.float32)
+    h = int(orc.lib().orc_pcg_host_seed(9))
+    carry = False
+    for t in range(T):
+        rec, rs, _, _, ll64, h = orc.filter_tick(x, l, t, y[t], carry, h, n, float(mu_t[t]), nu_obs=nu,
+                                                 ess_threshold=0.5, seed=9, tile=tile, threads=4)
+        assert rs == bool(ref["resampled"][t]), t
+        for k in ("mean", "var", "ess", "ll_incr"):
+            assert abs(rec[k] - ref[k][t]) <= 1e-12 * max(1.0, abs(ref[k][t])), (t, k)
+        assert abs(ll64 - ref64["ll_incr"][t]) <= 1e-6 * max(1.0, abs(ref64["ll_incr"][t])), t
+        carry = not rs
+    np.testing.assert_array_equal(x, xo)
+
+
+def test_threaded_score_oracle_matches_sequential(orc):
+    _, y = orc.dgp(60, 0.97, 0.15, -1.0, seed=42)
+    y = y.astype(np.float32)
+    a = orc.score(y, 5000, tile=256)
+    b = orc.score(y, 5000, tile=256, threads=4)
+    assert abs(a[0] - b[0]) <= 1e-12 * abs(a[0])
+    np.testing.assert_allclose(b[1], a[1], rtol=1e-10)
