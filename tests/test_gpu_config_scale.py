@@ -8,15 +8,15 @@ tick must agree: log-weights bit for bit, particles bit for bit on ticks
 without resampling, and on resampling ticks every output equal except near
 ties (a target within the FP64 rounding of the CDF, ~N^2 1e-16 per tick at
 this size): at most 64 such outputs, each taken from a parent next to the
-oracle's; statistics to 1e-12.  The oracle's FP64 evaluation of each tick's
+oracle's; statistics to 1e-11 (FP64 sums of 51.2M terms in two orders).  The oracle's FP64 evaluation of each tick's
 log-likelihood increment from the same state bounds the GPU's FP32 pipeline
 error: 1e-4 relative over the window.  (Whole-run bit-exactness is not a
 property at this size: the first near tie changes one particle and, through
 the normalisation, every later ancestor.)
 
 C1 at full length on N = 2^20 (T = 5000, the workload's mu schedule): the
-whole run against the FP32 oracle (particles bit for bit, statistics to
-1e-12) and the FP64-coupled oracle (log-likelihood 1e-4 relative).
+same one-tick-ahead check on every one of the 5000 ticks, and the whole
+run's log-likelihood against the FP64-coupled oracle's (1e-4 relative).
 
 C2, N = 2^20, T = 2000, theta from the host stream of seed 42, checkpoints
 every 64 ticks: the reverse-mode score against the oracle's FP64 forward-mode
@@ -40,14 +40,13 @@ def say(*a):
     print("[case]", *a, flush=True)
 
 
-def test_c1_window_51M_tick_by_tick(orc):
-    n, T, block = 51_200_000, 48, 12
-    mu_t = sv_data.mu_schedule(T, block=block)
-    y = sv_data.sv_series(T, 0.97, 0.15, mu_t.astype(np.float64), seed=42)[1].astype(np.float32)
-    say("c1 window", n, T)
+def _tick_by_tick(orc, n, y, mu_t, stat_tol):
+    """Steps the GPU filter one tick at a time and the oracle one tick from
+    the GPU's state; returns (resampling ticks, near-tie outputs per
+    resampling tick, GPU sum of ll_incr, FP64 sum of ll_incr)."""
+    T = len(y)
     f = gpu_bpf.GpuBpf(n, 0.97, 0.15, float(mu_t[0]), seed=42, ess_threshold=0.5)
     tile, _ = f.layout()
-    assert tile == 4096
     x = f.particles()
     _, _, x0, _ = orc.run_filter(y[:0], n, mu=float(mu_t[0]), ess_threshold=0.5, tile=tile, seed=42,
                                  want_x=True, threads=THREADS)
@@ -57,7 +56,7 @@ def test_c1_window_51M_tick_by_tick(orc):
     h = int(orc.lib().orc_pcg_host_seed(42))
     carry = False
     sum_gpu = sum_64 = 0.0
-    n_rs = 0
+    ties = []
     for t in range(T):
         if t == 0 or mu_t[t] != mu_t[t - 1]:
             f.set_mu(float(mu_t[t]))
@@ -70,25 +69,37 @@ def test_c1_window_51M_tick_by_tick(orc):
         assert bool(st["resampled"]) == rs, t
         np.testing.assert_array_equal(lg, l)        # this tick's log-weights
         for k in ("mean", "var", "ess", "ll_incr"):
-            assert abs(st[k] - rec[k]) <= 1e-12 * max(1.0, abs(rec[k])), (t, k, st[k], rec[k])
+            assert abs(st[k] - rec[k]) <= stat_tol * max(1.0, abs(rec[k])), (t, k, st[k], rec[k])
         assert abs(st["ll_incr"] - ll64) <= 1e-4 * max(1.0, abs(ll64)), (t, st["ll_incr"], ll64)
         if rs:
-            n_rs += 1
             bad = np.flatnonzero(xg != x)
-            say("tick", t, "resampled; near-tie outputs", bad.size)
-            assert bad.size <= 64
+            if bad.size:
+                say("tick", t, "resampled; near-tie outputs", bad.size)
+            assert bad.size <= max(64, n // 1_000_000)
             for i in bad:
                 a = int(anc[i])
                 near = [x_pre[j] for j in (a - 1, a + 1) if 0 <= j < n]
                 assert xg[i] in near, (t, i)
+            ties.append(bad.size)
         else:
             np.testing.assert_array_equal(xg, x)
         sum_gpu += float(st["ll_incr"])
         sum_64 += ll64
         x, l, carry = xg, lg, not rs                 # continue from the GPU's state
     f.close()
+    return len(ties), ties, sum_gpu, sum_64
+
+
+def test_c1_window_51M_tick_by_tick(orc):
+    n, T = 51_200_000, 48
+    mu_t = sv_data.mu_schedule(T, block=12)
+    y = sv_data.sv_series(T, 0.97, 0.15, mu_t.astype(np.float64), seed=42)[1].astype(np.float32)
+    say("c1 window", n, T)
+    # FP64 sums of 51.2M terms in two association orders: 1e-11
+    n_rs, ties, sg, s64 = _tick_by_tick(orc, n, y, mu_t, 1e-11)
+    say("c1 window: resampling ticks", n_rs, "near-tie outputs", ties, "ll", sg, s64)
     assert n_rs >= 3
-    assert abs(sum_gpu - sum_64) <= 1e-4 * abs(sum_64)
+    assert abs(sg - s64) <= 1e-4 * abs(s64)
 
 
 def test_c1_full_length_2p20(orc):
@@ -96,23 +107,21 @@ def test_c1_full_length_2p20(orc):
     mu_t = sv_data.mu_schedule(T)
     y = sv_data.sv_series(T, 0.97, 0.15, mu_t.astype(np.float64), seed=42)[1].astype(np.float32)
     say("c1 full length", n, T)
+    n_rs, ties, sg, s64 = _tick_by_tick(orc, n, y, mu_t, 1e-12)
+    say("c1 2^20 x 5000: resampling ticks", n_rs, "with near ties", sum(1 for v in ties if v),
+        "near-tie outputs", sum(ties), "ll", sg, s64)
+    assert 100 <= n_rs < T
+    assert abs(sg - s64) <= 1e-4 * abs(s64)
+    # the whole run (no resynchronisation): its log-likelihood against the
+    # FP64-coupled oracle's, which follows the oracle's own genealogy
     f = gpu_bpf.GpuBpf(n, 0.97, 0.15, float(mu_t[0]), seed=42, ess_threshold=0.5)
     tile, _ = f.layout()
     st = f.set_mu_schedule_run(y, mu_t, stats=True)
-    x = f.particles()
     f.close()
-    ref, cnt, xo, _ = orc.run_filter(y, n, mu=float(mu_t[0]), ess_threshold=0.5, tile=tile, seed=42,
-                                     mu_t=mu_t, want_x=True, threads=THREADS)
-    np.testing.assert_array_equal(st["resampled"], ref["resampled"])
-    np.testing.assert_array_equal(x, xo)
-    for k in ("mean", "var", "ess", "ll_incr", "ll_cum"):
-        err = np.max(np.abs(st[k] - ref[k]) / np.maximum(np.abs(ref[k]), 1.0))
-        assert err <= 1e-12, (k, err)
-    assert 100 <= cnt < T
     r64 = orc.run_filter(y, n, mu=float(mu_t[0]), ess_threshold=0.5, tile=tile, seed=42, mu_t=mu_t,
                          mode=orc.MODE_F64_COUPLED, threads=THREADS)[0]
     rel = abs(st["ll_cum"][-1] - r64["ll_cum"][-1]) / abs(r64["ll_cum"][-1])
-    say("c1 2^20 x 5000: resamples", cnt, "ll rel err vs FP64", rel)
+    say("c1 2^20 x 5000 whole run: ll rel err vs FP64", rel)
     assert rel <= 1e-4
 
 
