@@ -64,23 +64,6 @@ __device__ inline void wave_normals(uint64_t pa, uint64_t pg, int base, int lane
     wave_lds_sync();
 }
 
-// Jump constants of draw k (the single filter's tick-k draw): particle
-// i's PCG32 state is seed * A + (2i + 1) * G with A = M^(k+2) and
-// G = sum_{j < k+2} M^j (mod 2^64), by binary powering of the LCG step.
-__device__ inline void pcg_draw_constants(uint32_t k, uint64_t& A, uint64_t& G) {
-    uint64_t am = 1, ap = 0, cm = kPcgMult, cpl = 1;
-    for (uint32_t d = k + 2; d > 0; d >>= 1) {
-        if (d & 1u) {
-            am *= cm;
-            ap = ap * cm + cpl;
-        }
-        cpl = (cm + 1) * cpl;
-        cm *= cm;
-    }
-    A = am;
-    G = ap;
-}
-
 // The normals of one tick of slot m under the batch's noise source.
 __device__ inline void batch_normals(const BatchArgs& a, int m, uint64_t pa, uint64_t jm,
                                      uint64_t jg, int base, int lane, int zc, size_t zoff,

@@ -104,6 +104,23 @@ __device__ inline void store_vec(float* p, const float* v) {
 }
 
 
+// Jump constants of draw k (the single filter's tick-k draw): particle
+// i's PCG32 state is seed * A + (2i + 1) * G with A = M^(k+2) and
+// G = sum_{j < k+2} M^j (mod 2^64), by binary powering of the LCG step.
+__device__ inline void pcg_draw_constants(uint32_t k, uint64_t& A, uint64_t& G) {
+    uint64_t am = 1, ap = 0, cm = kPcgMult, cpl = 1;
+    for (uint32_t d = k + 2; d > 0; d >>= 1) {
+        if (d & 1u) {
+            am *= cm;
+            ap = ap * cm + cpl;
+        }
+        cpl = (cm + 1) * cpl;
+        cm *= cm;
+    }
+    A = am;
+    G = ap;
+}
+
 __device__ inline float host_pcg_float(unsigned long long* st) {
     unsigned long long old = *st;
     *st = old * kPcgMult + kPcgHostInc;

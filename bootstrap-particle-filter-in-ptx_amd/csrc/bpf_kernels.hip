@@ -396,8 +396,21 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
     }
 
     ObsModel obs = a.obs;
-    {
-        float yv = a.y_ptr ? *a.y_ptr : a.y;
+    uint64_t pcg_a = a.pcg_a, pcg_g = a.pcg_g;
+    int do_prop = a.do_prop;
+    if (a.tick_from_ctl) {
+        // a captured tick: this tick's index, observation and draw constants
+        // from the control block
+        const int tick = a.ctl->tick;
+        const float yv = a.ctl->y_base[tick - a.ctl->y_tick0];
+        obs.ysq = yv * yv;
+        uint64_t A, G;
+        pcg_draw_constants((uint32_t)tick, A, G);
+        pcg_a = (uint64_t)(int64_t)a.seed * A;
+        pcg_g = G;
+        do_prop = tick > 0;
+    } else {
+        const float yv = a.y_ptr ? *a.y_ptr : a.y;
         obs.ysq = yv * yv;
     }
 
@@ -415,7 +428,7 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
     float dp[LEARN ? K : 1];
     if constexpr (LEARN) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) dp[k] = a.do_prop ? x[k] - a.mu : 0.0f;
+        for (int k = 0; k < K; ++k) dp[k] = do_prop ? x[k] - a.mu : 0.0f;
     }
     // auxiliary filter: predictive means mp = mu + phi (x - mu), the
     // transition without its noise term
@@ -434,9 +447,9 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
         for (int k = 0; k < K; ++k) l[k] = -log_weight(mp[k], obs);
     }
 
-    if (!APF1 && a.do_prop) {
+    if (!APF1 && do_prop) {
         float z[K];
-        lane_normals<kBlock, K>(a.pcg_a, a.pcg_g, base, tid, s_q, s_cnt, z);
+        lane_normals<kBlock, K>(pcg_a, pcg_g, base, tid, s_q, s_cnt, z);
         if constexpr (FISH) {
 #pragma unroll
             for (int k = 0; k < K; ++k) {
@@ -987,7 +1000,9 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
     const double Sg = ctl->S;
     ObsModel obs = a.obs;
     {
-        float yv = a.y_ptr ? *a.y_ptr : a.y;
+        // a captured tick runs after the finalise pass advanced ctl->tick
+        const float yv = a.tick_from_ctl ? ctl->y_base[ctl->tick - 1 - ctl->y_tick0]
+                                         : (a.y_ptr ? *a.y_ptr : a.y);
         obs.ysq = yv * yv;
     }
 
@@ -1448,6 +1463,19 @@ __global__ __launch_bounds__(kBlock) void k_device_math(int fn, const float* in,
 hipError_t launch_device_math(int fn, const float* in, float* out, int n, hipStream_t st) {
     hipLaunchKernelGGL(k_device_math, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, fn, in,
                        out, n);
+    return hipGetLastError();
+}
+
+// Points the control block at an observation series whose element 0 is the
+// observation of the next tick (one thread; a kernel, so that it may sit
+// in front of captured ticks).
+__global__ void k_set_observations(Ctl* ctl, const float* y_base) {
+    ctl->y_base = y_base;
+    ctl->y_tick0 = ctl->tick;
+}
+
+hipError_t launch_set_observations(Ctl* ctl, const float* y_base, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_observations, dim3(1), dim3(1), 0, st, ctl, y_base);
     return hipGetLastError();
 }
 

@@ -43,6 +43,10 @@ struct Ctl {
     double off;          // raw log-weight offset (reference log_lik semantics)
     double ll_cum;       // FP64 cumulative log-likelihood
     unsigned long long host_rng;   // host-side PCG32 stream of the reference
+    // observation source of ticks whose launch arguments come from this
+    // block (captured tick graphs): tick t reads y_base[t - y_tick0]
+    const float* y_base;
+    int y_tick0;
     BpfTickStats last;
 };
 
@@ -93,6 +97,11 @@ struct PropArgs {
     const Ctl* ctl;
     const float* y_ptr;   // observation in device memory, or null
     float y;              // observation by value when y_ptr is null
+    // 1: the tick's observation, draw constants and propagation flag come
+    // from the control block (ctl->tick, ctl->y_base), not from y_ptr / y /
+    // pcg_a / pcg_g / do_prop, so one captured launch serves every tick
+    int tick_from_ctl;
+    int seed;             // PCG32 seed of the particle streams (tick_from_ctl)
     uint64_t pcg_a;       // seed * M^(k+2)      (state = pcg_a + inc_i * pcg_g)
     uint64_t pcg_g;       // sum_{j<k+2} M^j
     float mu, phi, sig;
@@ -167,6 +176,7 @@ struct ResArgs {
     int one_tile;         // 1: block b does tile b only (grid = n_tiles), no counters
     const float* y_ptr;
     float y;
+    int tick_from_ctl;    // 1: the observation of tick ctl->tick - 1 from ctl->y_base
     ObsModel obs;
     int n;
     float* m_log;         // optional per-tile copies for the score's replay

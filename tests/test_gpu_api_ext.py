@@ -104,3 +104,38 @@ def test_smc2_history_accounts_for_forward_particle_ticks():
     assert c["pt_forward"] == want
     assert c["replay_ticks"] == 2 * int(ticks.sum())
     assert c["pt_replay"] == 2 * int(np.dot(sum_n, ticks))
+
+
+@pytest.mark.parametrize("thr", [0.0, 0.5])
+def test_captured_ticks_equal_tick_by_tick(thr):
+    """gpu_bpf_run / gpu_bpf_run_device replay a captured 32-tick graph whose
+    kernels take the tick's inputs from the control block; gpu_bpf_step
+    launches tick by tick.  Both give the same particles and statistics, also
+    across a set_mu (new arguments: a new capture), a second observation
+    array, and a run that mixes the two."""
+    y = _y(150, seed=6)
+    say("captured ticks", thr)
+    a = gpu_bpf.GpuBpf(30000, 0.97, 0.15, -1.0, seed=2, ess_threshold=thr)
+    _, s1 = a.run(y[:70])                 # 2 graph replays + 6 launches
+    a.set_mu(-0.6)
+    yd = gpu_bpf.DeviceArray(y[70:150])
+    a.set_stats_capacity(200)
+    a.run_device(yd.ptr, 40)              # recaptured (mu changed), other array
+    a.step(float(y[110]))                 # one tick launched on its own
+    a.run_device(yd.ptr + 4 * 41, 39)     # below one graph: launched tick by tick
+    st_a = a.stats_tail(150)
+    xa = a.particles()
+    a.close()
+    yd.close()
+    b = gpu_bpf.GpuBpf(30000, 0.97, 0.15, -1.0, seed=2, ess_threshold=thr)
+    for t in range(150):
+        if t == 70:
+            b.set_mu(-0.6)
+        b.step(float(y[t]))
+    b.set_stats_capacity(200)
+    st_b = b.stats_tail(150)
+    xb = b.particles()
+    b.close()
+    np.testing.assert_array_equal(st_a[:70], s1)
+    np.testing.assert_array_equal(st_a, st_b)
+    np.testing.assert_array_equal(xa, xb)
