@@ -423,7 +423,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
         for (int q = 0; q < 4; ++q) {
             const double tot = (double)e[q];
             const double incl = wave_incl_scan(tot, lane);
-            tot4[q] = __shfl(incl, kWave - 1, kWave);
+            tot4[q] = wave_last(incl);
             double excl = incl - tot;
             for (int r = 0; r < q; ++r) excl += tot4[r];
             excl = (q == 0 && lane == 0) ? 0.0 : excl;
@@ -435,9 +435,8 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const double pm = wave_incl_max(cv[q], lane);
-            double prev = __shfl_up(pm, 1, kWave);
-            prev = (lane == 0) ? b0 : prev;
-            pm4[q] = __shfl(pm, kWave - 1, kWave);
+            double prev = wave_prev(pm, b0);
+            pm4[q] = wave_last(pm);
             for (int r = 0; r < q; ++r) prev = fmax(prev, pm4[r]);
             const int jl = 64 * q + lane;
             double c = fmax(cv[q], prev);
@@ -455,8 +454,8 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
         else if (lane == 1) r = (b == nt - 1) ? W : count_le_fast(b1, u, 0, W, n);
         else if (lane == 2) r = (b == 0) ? W : count_le(b0, u, W, n, n, true);
         else if (lane == 3) r = (b == nt - 1) ? n : count_le(b1, u, W, n, n, true);
-        const int ilo = __shfl(r, 0, kWave), ihi = __shfl(r, 1, kWave);
-        const int jlo = __shfl(r, 2, kWave), jhi = __shfl(r, 3, kWave);
+        const int ilo = lane_i32(r, 0), ihi = lane_i32(r, 1);
+        const int jlo = lane_i32(r, 2), jhi = lane_i32(r, 3);
         wave_lds_sync();
         for (int i = ilo + lane; i < ihi; i += kWave) {
             const double t = (double)target_unwrapped(u, i, n);

@@ -9,44 +9,139 @@
 namespace bpf {
 
 // ---------------------------------------------------------------------------
-// Wavefront / block helpers (wave64)
+// Wavefront helpers (wave64) on DPP cross-lane moves -- quad permutations,
+// shifts and mirrors within 16-lane rows, the row broadcasts of lanes 15 and
+// 31 -- and v_readlane, so no step goes through the LDS crossbar.  FP64
+// values move as two 32-bit halves.  Each helper has one fixed association
+// order, so every kernel that uses it forms the same sums bit for bit.  Call
+// them with the whole wave active.
 // ---------------------------------------------------------------------------
 
+namespace dpp {
+constexpr int kQuad1032 = 0xB1;      // quad_perm [1, 0, 3, 2]
+constexpr int kQuad2301 = 0x4E;      // quad_perm [2, 3, 0, 1]
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr3 = 0x113, kRowShr4 = 0x114, kRowShr8 = 0x118;
+constexpr int kWaveShr1 = 0x138;     // lane i takes lane i - 1 across rows
+constexpr int kRowMirror = 0x140;    // lane i of a row takes lane 15 - i
+constexpr int kRowHalfMirror = 0x141;  // lane i of a half row takes lane 7 - i
+constexpr int kRowBcast15 = 0x142;   // rows take lane 15 of the row before
+constexpr int kRowBcast31 = 0x143;   // rows 2 and 3 take lane 31
+}  // namespace dpp
+
+// The value from the lane that CTRL selects; lanes outside the ROWS / BANKS
+// masks, and lanes whose source lies outside their row, receive `id`.
+template <int CTRL, int ROWS = 0xf, int BANKS = 0xf>
+__device__ inline int dpp_i32(int v, int id) {
+    return __builtin_amdgcn_update_dpp(id, v, CTRL, ROWS, BANKS, false);
+}
+
+template <int CTRL, int ROWS = 0xf, int BANKS = 0xf>
+__device__ inline float dpp_f32(float v, float id) {
+    return __int_as_float(dpp_i32<CTRL, ROWS, BANKS>(__float_as_int(v), __float_as_int(id)));
+}
+
+template <int CTRL, int ROWS = 0xf, int BANKS = 0xf>
+__device__ inline double dpp_f64(double v, double id) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned long long ib = (unsigned long long)__double_as_longlong(id);
+    const unsigned lo = (unsigned)dpp_i32<CTRL, ROWS, BANKS>((int)(unsigned)b, (int)(unsigned)ib);
+    const unsigned hi = (unsigned)dpp_i32<CTRL, ROWS, BANKS>((int)(unsigned)(b >> 32), (int)(unsigned)(ib >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Lane L's value in every lane (L a constant).
+__device__ inline double lane_f64(double v, int L) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, L);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), L);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ inline int lane_i32(int v, int L) { return __builtin_amdgcn_readlane(v, L); }
+
+__device__ inline float lane_f32(float v, int L) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), L));
+}
+
+// The last lane's value, and the previous lane's (lane 0: fill).
+__device__ inline double wave_last(double v) { return lane_f64(v, kWave - 1); }
+__device__ inline double wave_prev(double v, double fill) { return dpp_f64<dpp::kWaveShr1>(v, fill); }
+
+// Reductions: quads, half rows, rows, then rows 0 + 1 into row 1 and rows
+// 2 + 3 into row 3, and the two into lane 63, which every lane reads.
 __device__ inline float wave_max(float v) {
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, kWave));
-    return v;
+    constexpr float NI = -__builtin_inff();
+    v = fmaxf(v, dpp_f32<dpp::kQuad1032>(v, NI));
+    v = fmaxf(v, dpp_f32<dpp::kQuad2301>(v, NI));
+    v = fmaxf(v, dpp_f32<dpp::kRowHalfMirror>(v, NI));
+    v = fmaxf(v, dpp_f32<dpp::kRowMirror>(v, NI));
+    v = fmaxf(v, dpp_f32<dpp::kRowBcast15, 0xa>(v, NI));
+    v = fmaxf(v, dpp_f32<dpp::kRowBcast31, 0xc>(v, NI));
+    return lane_f32(v, kWave - 1);
 }
 
 __device__ inline double wave_sum(double v) {
+    v += dpp_f64<dpp::kQuad1032>(v, 0.0);
+    v += dpp_f64<dpp::kQuad2301>(v, 0.0);
+    v += dpp_f64<dpp::kRowHalfMirror>(v, 0.0);
+    v += dpp_f64<dpp::kRowMirror>(v, 0.0);
+    v += dpp_f64<dpp::kRowBcast15, 0xa>(v, 0.0);
+    v += dpp_f64<dpp::kRowBcast31, 0xc>(v, 0.0);
+    return wave_last(v);
+}
+
+// Inclusive scans: each lane adds its three row predecessors, then the
+// partial sums 4 and 8 lanes back (banks 1-3, then banks 2-3), then the
+// totals of the rows before (row broadcasts of lanes 15 and 31).
+__device__ inline double wave_incl_scan(double v, int /*lane*/) {
+    double s = v + dpp_f64<dpp::kRowShr1>(v, 0.0);
+    s += dpp_f64<dpp::kRowShr2>(v, 0.0);
+    s += dpp_f64<dpp::kRowShr3>(v, 0.0);
+    s += dpp_f64<dpp::kRowShr4, 0xf, 0xe>(s, 0.0);
+    s += dpp_f64<dpp::kRowShr8, 0xf, 0xc>(s, 0.0);
+    s += dpp_f64<dpp::kRowBcast15, 0xa>(s, 0.0);
+    s += dpp_f64<dpp::kRowBcast31, 0xc>(s, 0.0);
+    return s;
+}
+
+__device__ inline int wave_incl_scan(int v, int /*lane*/) {
+    int s = v + dpp_i32<dpp::kRowShr1>(v, 0);
+    s += dpp_i32<dpp::kRowShr2>(v, 0);
+    s += dpp_i32<dpp::kRowShr3>(v, 0);
+    s += dpp_i32<dpp::kRowShr4, 0xf, 0xe>(s, 0);
+    s += dpp_i32<dpp::kRowShr8, 0xf, 0xc>(s, 0);
+    s += dpp_i32<dpp::kRowBcast15, 0xa>(s, 0);
+    s += dpp_i32<dpp::kRowBcast31, 0xc>(s, 0);
+    return s;
+}
+
+__device__ inline double wave_incl_max(double v, int /*lane*/) {
+    constexpr double NI = -__builtin_inf();
+    double s = fmax(v, dpp_f64<dpp::kRowShr1>(v, NI));
+    s = fmax(s, dpp_f64<dpp::kRowShr2>(v, NI));
+    s = fmax(s, dpp_f64<dpp::kRowShr3>(v, NI));
+    s = fmax(s, dpp_f64<dpp::kRowShr4, 0xf, 0xe>(s, NI));
+    s = fmax(s, dpp_f64<dpp::kRowShr8, 0xf, 0xc>(s, NI));
+    s = fmax(s, dpp_f64<dpp::kRowBcast15, 0xa>(s, NI));
+    s = fmax(s, dpp_f64<dpp::kRowBcast31, 0xc>(s, NI));
+    return s;
+}
+
+// Lane-shuffle forms with the butterfly (offsets 32, 16, ..., 1) and
+// Hillis-Steele (offsets 1, 2, ..., 32) association orders, for the kernels
+// whose CPU restatement repeats those orders bit for bit (the RBPF inner
+// filter).
+__device__ inline double wave_sum_bfly(double v) {
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
     return v;
 }
 
-__device__ inline int wave_incl_scan(int v, int lane) {
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        int o = __shfl_up(v, off, kWave);
-        v += (lane >= off) ? o : 0;
-    }
-    return v;
-}
-
-__device__ inline double wave_incl_scan(double v, int lane) {
+__device__ inline double wave_incl_scan_hs(double v, int lane) {
 #pragma unroll
     for (int off = 1; off < kWave; off <<= 1) {
         double o = __shfl_up(v, off, kWave);
         v += (lane >= off) ? o : 0.0;
-    }
-    return v;
-}
-
-__device__ inline double wave_incl_max(double v, int lane) {
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        double o = __shfl_up(v, off, kWave);
-        v = (lane >= off) ? fmax(v, o) : v;
     }
     return v;
 }
@@ -258,13 +353,16 @@ __device__ inline int lds_lower_bound(const double* c, int last, double t) {
     return lo;
 }
 
-__device__ inline int wave_incl_max_int(int v, int lane) {
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        int o = __shfl_up(v, off, kWave);
-        v = (lane >= off) ? max(v, o) : v;
-    }
-    return v;
+__device__ inline int wave_incl_max_int(int v, int /*lane*/) {
+    constexpr int NI = -0x7fffffff - 1;
+    int s = max(v, dpp_i32<dpp::kRowShr1>(v, NI));
+    s = max(s, dpp_i32<dpp::kRowShr2>(v, NI));
+    s = max(s, dpp_i32<dpp::kRowShr3>(v, NI));
+    s = max(s, dpp_i32<dpp::kRowShr4, 0xf, 0xe>(s, NI));
+    s = max(s, dpp_i32<dpp::kRowShr8, 0xf, 0xc>(s, NI));
+    s = max(s, dpp_i32<dpp::kRowBcast15, 0xa>(s, NI));
+    s = max(s, dpp_i32<dpp::kRowBcast31, 0xc>(s, NI));
+    return s;
 }
 
 }  // namespace bpf

@@ -797,8 +797,7 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
     const double incl = wave_incl_scan(loc, lane);
     if (lane == kWave - 1) s_a[wid] = incl;
     __syncthreads();
-    double B = __shfl_up(incl, 1, kWave);
-    B = (lane == 0) ? 0.0 : B;
+    double B = wave_prev(incl, 0.0);
     for (int w = 0; w < wid; ++w) B += s_a[w];
     double bt[kFinPer];
     double hi_v = 0.0;
@@ -810,8 +809,7 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
     }
     __syncthreads();
     const double pm = wave_incl_max(hi_v, lane);
-    double pre = __shfl_up(pm, 1, kWave);
-    pre = (lane == 0) ? 0.0 : pre;
+    double pre = wave_prev(pm, 0.0);
     if (lane == kWave - 1) s_a[wid] = pm;
     __syncthreads();
     for (int w = 0; w < wid; ++w) pre = fmax(pre, s_a[w]);
@@ -892,8 +890,7 @@ __device__ inline int tile_cdf(const float (&e)[K], int b, int n, int n_tiles, d
     for (int q = 0; q < K; ++q) L += (double)ev[q];
     // make the in-tile CDF non-decreasing across lanes (prefix max)
     double pm = wave_incl_max(fmin(fma(ft, L, Bb), Bb1), lane);
-    double prev = __shfl_up(pm, 1, kWave);
-    prev = (lane == 0) ? Bb : prev;
+    double prev = wave_prev(pm, Bb);
     __syncthreads();   // s_w reads and s_e (aliasing s_c) reads are done
     if (lane == kWave - 1) s_w[wid] = pm;
     __syncthreads();

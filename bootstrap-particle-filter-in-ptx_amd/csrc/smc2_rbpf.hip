@@ -69,8 +69,8 @@ __device__ inline float blk_max(float v, float* s, int lane, int wid) {
 
 template <int NI>
 __device__ inline void blk_sum2(double& a, double& b, double (*s)[NI / kWave], int lane, int wid) {
-    a = wave_sum(a);
-    b = wave_sum(b);
+    a = wave_sum_bfly(a);
+    b = wave_sum_bfly(b);
     if (lane == 0) {
         s[0][wid] = a;
         s[1][wid] = b;
@@ -89,7 +89,7 @@ __device__ inline void blk_sum2(double& a, double& b, double (*s)[NI / kWave], i
 
 template <int NI>
 __device__ inline double blk_scan(double v, double* s, int lane, int wid, double& total) {
-    const double incl = wave_incl_scan(v, lane);
+    const double incl = wave_incl_scan_hs(v, lane);
     if (lane == kWave - 1) s[wid] = incl;
     __syncthreads();
     double P = 0.0, T = 0.0;
