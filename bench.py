@@ -284,6 +284,42 @@ def extra_workloads(args, gpu_bpf, torch):
     return out
 
 
+def batch_only(args):
+    """The two batched inner-filter workloads alone, as extra_workloads builds
+    them (same seeded particle counts, priors and observations): autotuned
+    block, 32 untimed ticks, then args.batch_ticks timed ticks each.  The
+    command for kernel traces and counters of k_batch_tick."""
+    import gpu_bpf_batch
+    rng = np.random.default_rng(42)
+    rows = {}
+    for name, M, lo, hi, T, nu, score in (
+            ("smc2_cpmmh_m512_t2000", 512, 256, 8192, 2000, 0.0, False),
+            ("smc2_mala_m1024_t5000_student_t", 1024, 512, 16384, 5000, 5.0, True)):
+        mus = sv_data.mu_schedule(T)
+        _, yy = sv_data.sv_series(T, 0.97, 0.15, mus.astype(np.float64), nu_obs=nu, seed=42)
+        yy = yy.astype(np.float32)
+        n = log_uniform_counts(rng, M, lo, hi)
+        mu, phi, sig, seeds = prior_draws(rng, M)
+        b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=nu)
+        blk = b.autotune(yy, 64)
+        b.close()
+        b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=nu, block=blk)
+        if score:
+            b.enable_score()
+        b.run(yy[:32], stats=False)
+        b.synchronize()
+        k = min(args.batch_ticks, T - 32)
+        t0 = time.perf_counter()
+        b.run(yy[32:32 + k], stats=False)
+        b.synchronize()
+        dt = (time.perf_counter() - t0) / k
+        b.close()
+        rows[name] = {"block": blk, "sum_n": int(n.sum()), "ticks": k, "us_per_tick": dt * 1e6,
+                      "particle_ticks_per_s": float(n.sum()) / dt, "score": score}
+        log(name, rows[name])
+    return rows
+
+
 def batch_and_smc2(args, gpu_bpf, gpu_bpf_batch, torch, y, n, mu, phi, sig, seeds, nmin, nmax, nu,
                    kind, moves, ticks_batch):
     """Inner-batch throughput (autotuned), the looped plain filters, the
@@ -468,6 +504,9 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the other configurations")
     ap.add_argument("--rbpf-only", action="store_true",
                     help="run only the RBPF SMC^2 workload and print its JSON line")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the two batched inner-filter workloads and print their JSON line")
+    ap.add_argument("--batch-ticks", type=int, default=256, help="timed ticks of --batch-only")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 5 device-to-device copies of BYTES (the reference kernel "
                          "for scaling HBM traffic counters) and exit")
@@ -481,6 +520,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 5}))
+        return
+    if args.batch_only:
+        print(json.dumps({"inner_batches": batch_only(args)}), flush=True)
         return
     if args.rbpf_only:
         print(json.dumps({"smc2_rbpf_ntheta1024_t1000": rbpf_workload(args)}), flush=True)

@@ -236,10 +236,21 @@ __device__ inline float log_weight(float x, const ObsModel& o) {
     return o.nu > 0.0f ? log_weight_t(x, o) : log_weight_gauss(x, o.ysq);
 }
 
-// Systematic-resampling target of gpu_bpf/gpu_bpf.cu:300-302 (IEEE
-// division), before the wrap at 1.
+// Systematic-resampling target of gpu_bpf/gpu_bpf.cu:300-302, u + i/N in
+// FP32 with the IEEE-rounded quotient, before the wrap at 1.  The quotient is
+// formed without a float division, as float(double(float(i)) * R) with
+// R = 1 / double(float(N)) rounded to double.  For a = float(i) and
+// b = float(N) (24-bit significands) the exact a/b is never a midpoint
+// between adjacent floats and lies more than 2^-50 a/b from every midpoint
+// (a - b m is a nonzero multiple of 2^(e_a - 25)), while the product is
+// within 2^-52 a/b of a/b; both therefore round to the same float, the IEEE
+// quotient.
+__device__ __host__ inline double target_recip(int n) { return 1.0 / (double)(float)n; }
+__device__ __host__ inline float target_at(float u, int i, double rn) {
+    return u + (float)((double)(float)i * rn);
+}
 __device__ __host__ inline float target_unwrapped(float u, int i, int n) {
-    return u + (float)i / (float)n;
+    return target_at(u, i, target_recip(n));
 }
 
 }  // namespace bpf

@@ -974,6 +974,7 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
     constexpr int TP = TILE + TILE / 16;   // padded length
     constexpr int NW = BLOCK / kWave;
     __shared__ float s_x[TP];
+    __shared__ float s_o[FISH ? 1 : TP];   // staged offspring of one chunk
     __shared__ double s_c[TP];
     __shared__ double s_w[NW];
     __shared__ int s_rng[4];
@@ -1099,57 +1100,69 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
         // Offspring.  Output i goes to the first parent j with c_j >= t_i
         // (lower_bound; every owned unwrapped target is <= c_last), and both
         // sequences are non-decreasing, so ancestors are too.
-        // Lane tid owns a contiguous run of ceil(n_out / BLOCK) outputs: it
-        // finds the first one's parent by a binary search over the tile CDF
-        // and each next one by galloping forward from the previous parent
-        // (O(log gap) whatever the weight degeneracy).  The Fisher variant
-        // keeps the merge path below, which moves the three extra vectors.
+        // The outputs are taken in chunks of TILE.  Lane tid owns a
+        // contiguous run of ceil(chunk / BLOCK) of them: it finds the first
+        // one's parent by a binary search over the tile CDF and each next one
+        // by galloping forward from the previous parent (O(log gap) whatever
+        // the weight degeneracy), and stages the offspring in LDS, from where
+        // the block stores the chunk with coalesced writes (lane-strided runs
+        // stored directly cost 24 us more per launch at 51.2M particles).  The
+        // Fisher variant keeps the merge path below, which moves the three
+        // extra vectors.
         const int n_out = ihi - ilo;
         if (n_out > 0) {
             if constexpr (!FISH) {
-                const int per = (n_out + BLOCK - 1) / BLOCK;
-                const int o0 = min(tid * per, n_out), o1 = min(o0 + per, n_out);
-                if (o0 < o1) {
-                    double t = (double)target_unwrapped(u, ilo + o0, n);
-                    int lo = 0, hi = last;
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        if (s_c[pad16(mid)] < t) lo = mid + 1; else hi = mid;
-                    }
-                    int j = lo;
-                    // c_j and c_(j+1) stay in registers: another child of j
-                    // reads no LDS, a child of j + 1 waits on none (the
-                    // refill of c_(j+2) is needed only by the next output)
-                    double cj = s_c[pad16(j)];
-                    double cn = s_c[pad16(min(j + 1, last))];
-                    xd[ilo + o0] = s_x[pad16(j)];
-                    for (int o = o0 + 1; o < o1; ++o) {
-                        t = (double)target_unwrapped(u, ilo + o, n);
-                        if (cj < t) {
-                            // t <= c_last, so j < last here
-                            if (t <= cn) {
-                                ++j;
-                                cj = cn;
-                            } else {
-                                // the parent lies in (j + 1, last]: gallop,
-                                // then search
-                                int g0 = j + 2, g1 = g0, step = 1;
-                                while (g1 < last && s_c[pad16(g1)] < t) {
-                                    g0 = g1 + 1;
-                                    step <<= 1;
-                                    g1 = min(g0 + step - 1, last);
-                                }
-                                while (g0 < g1) {
-                                    const int mid = (g0 + g1) >> 1;
-                                    if (s_c[pad16(mid)] < t) g0 = mid + 1; else g1 = mid;
-                                }
-                                j = g0;
-                                cj = s_c[pad16(j)];
-                            }
-                            cn = s_c[pad16(min(j + 1, last))];
+                const double rn = target_recip(n);
+                for (int c0 = 0; c0 < n_out; c0 += TILE) {
+                    const int cnt = min(TILE, n_out - c0);
+                    const int per = (cnt + BLOCK - 1) / BLOCK;
+                    const int o0 = min(tid * per, cnt), o1 = min(o0 + per, cnt);
+                    const int i0 = ilo + c0;
+                    if (o0 < o1) {
+                        double t = (double)target_at(u, i0 + o0, rn);
+                        int lo = 0, hi = last;
+                        while (lo < hi) {
+                            const int mid = (lo + hi) >> 1;
+                            if (s_c[pad16(mid)] < t) lo = mid + 1; else hi = mid;
                         }
-                        xd[ilo + o] = s_x[pad16(j)];
+                        int j = lo;
+                        // c_j and c_(j+1) stay in registers: another child of
+                        // j reads no LDS, a child of j + 1 waits on none (the
+                        // refill of c_(j+2) is needed only by the next output)
+                        double cj = s_c[pad16(j)];
+                        double cn = s_c[pad16(min(j + 1, last))];
+                        s_o[pad16(o0)] = s_x[pad16(j)];
+                        for (int o = o0 + 1; o < o1; ++o) {
+                            t = (double)target_at(u, i0 + o, rn);
+                            if (cj < t) {
+                                // t <= c_last, so j < last here
+                                if (t <= cn) {
+                                    ++j;
+                                    cj = cn;
+                                } else {
+                                    // the parent lies in (j + 1, last]:
+                                    // gallop, then search
+                                    int g0 = j + 2, g1 = g0, step = 1;
+                                    while (g1 < last && s_c[pad16(g1)] < t) {
+                                        g0 = g1 + 1;
+                                        step <<= 1;
+                                        g1 = min(g0 + step - 1, last);
+                                    }
+                                    while (g0 < g1) {
+                                        const int mid = (g0 + g1) >> 1;
+                                        if (s_c[pad16(mid)] < t) g0 = mid + 1; else g1 = mid;
+                                    }
+                                    j = g0;
+                                    cj = s_c[pad16(j)];
+                                }
+                                cn = s_c[pad16(min(j + 1, last))];
+                            }
+                            s_o[pad16(o)] = s_x[pad16(j)];
+                        }
                     }
+                    __syncthreads();
+                    for (int o = tid; o < cnt; o += BLOCK) xd[i0 + o] = s_o[pad16(o)];
+                    __syncthreads();    // s_o is refilled by the next chunk
                 }
             } else {
                 // merge path over the owned targets and the tile CDF:
