@@ -401,6 +401,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
     const double fg = (M == NEG_INF) ? 0.0 : (double)expf_cw(M - M);
     double* cw = s_c[wid];
     float* xw = s_x[wid];
+    const double rn = target_recip(n);
     for (int b = wid; b < nt; b += NW) {
         const int base = b * kBatchTile;
         const float mb = s_tmax[b];
@@ -449,16 +450,20 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
                 for (int k = 0; k < 3; ++k) s_dx[wid][k][pad16(jl)] = dxb[k][base + jl];
             }
         }
+        // lanes 0, 1: unwrapped targets [0, W) at or below b0, b1; lanes 2,
+        // 3: wrapped targets [W, n) likewise
         int r = 0;
-        if (lane == 0) r = (b == 0) ? 0 : count_le_fast(b0, u, 0, W, n);
-        else if (lane == 1) r = (b == nt - 1) ? W : count_le_fast(b1, u, 0, W, n);
-        else if (lane == 2) r = (b == 0) ? W : count_le(b0, u, W, n, n, true);
-        else if (lane == 3) r = (b == nt - 1) ? n : count_le(b1, u, W, n, n, true);
+        if (lane < 4) {
+            const bool up = lane & 1, wr = lane >= 2;
+            const int lo = wr ? W : 0, hi = wr ? n : W;
+            const bool end = up ? (b == nt - 1) : (b == 0);
+            r = end ? (up ? hi : lo) : count_targets_le(up ? b1 : b0, u, wr ? 1.0f : 0.0f, lo, hi, n, rn);
+        }
         const int ilo = lane_i32(r, 0), ihi = lane_i32(r, 1);
         const int jlo = lane_i32(r, 2), jhi = lane_i32(r, 3);
         wave_lds_sync();
         for (int i = ilo + lane; i < ihi; i += kWave) {
-            const double t = (double)target_unwrapped(u, i, n);
+            const double t = (double)target_at(u, i, rn);
             const int p = pad16(lds_lower_bound(cw, last, t));
             xo[i] = xw[p];
             if (TAN) {
@@ -467,7 +472,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
             }
         }
         for (int i = jlo + lane; i < jhi; i += kWave) {
-            const double t = (double)(target_unwrapped(u, i, n) - 1.0f);
+            const double t = (double)(target_at(u, i, rn) - 1.0f);
             const int p = pad16(lds_lower_bound(cw, last, t));
             xo[i] = xw[p];
             if (TAN) {

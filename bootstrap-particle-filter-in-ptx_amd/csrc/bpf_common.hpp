@@ -294,47 +294,46 @@ __device__ inline void finalize_tick(Ctl& c, int n_particles, float threshold, f
 // Systematic targets t_i = u + i/N (wrapped once at 1 from i = W on).
 // ---------------------------------------------------------------------------
 
-// The target counts are taken by a few threads per tile; kept out of line so
-// that their loops do not raise the register allocation of the kernels that
-// call them.
-__device__ __noinline__ int count_le(double v, float u, int lo, int hi, int n, bool wrapped) {
+// Number of i in [lo, hi) whose resampling target lies at or below v:
+// (double)(t_i - sub) <= v with t_i = target_at(u, i, rn), sub = 0 for the
+// unwrapped targets and 1 for the wrapped ones (i >= W, t_i in [1, 2), so
+// t_i - 1 is exact).  The targets are non-decreasing in i, so the count is
+// where a monotone predicate flips.  It starts from the real-number estimate
+// floor((v + sub - u) N) + 1, which the float rounding of t_i moves by at most
+// about N 2^-23 positions, steps to the exact flip, and bisects [lo, hi)
+// only if that takes more than 16 steps down or 32 in all.  Inline and
+// call-free, so the four range counts of a tile run side by side in four
+// lanes and a lane's per-parent counts interleave.
+__device__ inline int count_targets_le(double v, float u, float sub, int lo, int hi, int n, double rn) {
     if (lo >= hi) return lo;
-    // the ends first: all or none of the range (the usual answer for the
-    // wrapped targets, which all lie below u)
-    {
-        float t = target_unwrapped(u, hi - 1, n);
-        if (wrapped) t -= 1.0f;
-        if ((double)t <= v) return hi;
-        t = target_unwrapped(u, lo, n);
-        if (wrapped) t -= 1.0f;
-        if ((double)t > v) return lo;
-    }
-    while (lo < hi) {
-        int mid = (int)(((unsigned)lo + (unsigned)hi) >> 1);
-        float t = target_unwrapped(u, mid, n);
-        if (wrapped) t -= 1.0f;
-        if ((double)t <= v) lo = mid + 1; else hi = mid;
-    }
-    return lo;
-}
-
-// Same count for unwrapped targets, from the estimate i ~ (v - u) N and an
-// exact correction with the float target rule; falls back to the binary
-// search if the estimate is off by more than a few (it is off by at most
-// about N * 2^-24 positions, so this does not happen below ~2^28 particles).
-__device__ __noinline__ int count_le_fast(double v, float u, int lo, int hi, int n) {
-    if (lo >= hi) return lo;
-    double est = floor((v - (double)u) * (double)n) + 1.0;
+    double est = floor((v + (double)sub - (double)u) * (double)n) + 1.0;
     est = fmin(fmax(est, (double)lo), (double)hi);
     int g = (int)est;
     int steps = 0;
-    while (g > lo && (double)target_unwrapped(u, g - 1, n) > v) {
+    bool ok = true;
+    while (g > lo && (double)(target_at(u, g - 1, rn) - sub) > v) {
         --g;
-        if (++steps > 8) return count_le(v, u, lo, hi, n, false);
+        if (++steps > 16) {
+            ok = false;
+            break;
+        }
     }
-    while (g < hi && (double)target_unwrapped(u, g, n) <= v) {
-        ++g;
-        if (++steps > 16) return count_le(v, u, lo, hi, n, false);
+    if (ok) {
+        while (g < hi && (double)(target_at(u, g, rn) - sub) <= v) {
+            ++g;
+            if (++steps > 32) {
+                ok = false;
+                break;
+            }
+        }
+    }
+    if (!ok) {
+        int a = lo, b = hi;
+        while (a < b) {
+            const int mid = (int)(((unsigned)a + (unsigned)b) >> 1);
+            if ((double)(target_at(u, mid, rn) - sub) <= v) a = mid + 1; else b = mid;
+        }
+        g = a;
     }
     return g;
 }

@@ -906,12 +906,14 @@ __device__ inline int tile_cdf(const float (&e)[K], int b, int n, int n_tiles, d
         s_c[pad16(j)] = c;
     }
     if (tid < 4) {
-        int r;
-        if (tid == 0)      r = (b == 0) ? 0 : count_le_fast(Bb, u, 0, W, n);
-        else if (tid == 1) r = (b == n_tiles - 1) ? W : count_le_fast(Bb1, u, 0, W, n);
-        else if (tid == 2) r = (b == 0) ? W : count_le(Bb, u, W, n, n, true);
-        else               r = (b == n_tiles - 1) ? n : count_le(Bb1, u, W, n, n, true);
-        s_rng[tid] = r;
+        // lanes 0, 1: unwrapped targets [0, W) at or below Bb, Bb1; lanes 2,
+        // 3: wrapped targets [W, n) likewise (the first and last tiles own
+        // their range ends)
+        const bool up = tid & 1, wr = tid >= 2;
+        const int lo = wr ? W : 0, hi = wr ? n : W;
+        const bool end = up ? (b == n_tiles - 1) : (b == 0);
+        s_rng[tid] = end ? (up ? hi : lo)
+                         : count_targets_le(up ? Bb1 : Bb, u, wr ? 1.0f : 0.0f, lo, hi, n, target_recip(n));
     }
     __syncthreads();
     return last;
@@ -965,7 +967,7 @@ __device__ inline int claim_tile(int* ctr, int& r, int& tried, int n_tiles) {
 
 // FISH: the Fisher-identity vectors are gathered with x (staged in LDS).
 template <int BLOCK, int K, bool FISH>
-__global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(FISH ? 1 : 4))) void k_resample(ResArgs a) {
     Ctl* ctl = a.ctl;
     if (!ctl->resample) return;
     constexpr int VEC = K < 4 ? K : 4;
@@ -983,6 +985,7 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
 
     const int tid = threadIdx.x;
     const int n = a.n;
+    const double rn = target_recip(n);   // target quotients i/n as i * rn
     const float* __restrict__ xs = ctl->src ? a.x1 : a.x0;
     float* __restrict__ xd = ctl->dst ? a.x1 : a.x0;
     const float* als[3];
@@ -1112,7 +1115,6 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
         const int n_out = ihi - ilo;
         if (n_out > 0) {
             if constexpr (!FISH) {
-                const double rn = target_recip(n);
                 for (int c0 = 0; c0 < n_out; c0 += TILE) {
                     const int cnt = min(TILE, n_out - c0);
                     const int per = (cnt + BLOCK - 1) / BLOCK;
@@ -1178,12 +1180,12 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
                 int lo = max(0, d0 - P), hi = min(d0, n_out);
                 while (lo < hi) {
                     const int mid = (lo + hi) >> 1;
-                    const double tm = (double)target_unwrapped(u, ilo + mid, n);
+                    const double tm = (double)target_at(u, ilo + mid, rn);
                     if (tm <= s_c[pad16(d0 - mid - 1)]) lo = mid + 1; else hi = mid;
                 }
                 int oa = lo, j = d0 - lo;
                 double cj = (j < P) ? s_c[pad16(j)] : __builtin_inf();
-                double tv = (oa < n_out) ? (double)target_unwrapped(u, ilo + oa, n) : 0.0;
+                double tv = (oa < n_out) ? (double)target_at(u, ilo + oa, rn) : 0.0;
                 for (int d = d0; d < d1; ++d) {
                     if (oa < n_out && tv <= cj) {
                         const int jp = pad16(j);
@@ -1191,7 +1193,7 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
 #pragma unroll
                         for (int c = 0; c < 3; ++c) ald[c][ilo + oa] = s_al[c][jp];
                         ++oa;
-                        tv = (oa < n_out) ? (double)target_unwrapped(u, ilo + oa, n) : 0.0;
+                        tv = (oa < n_out) ? (double)target_at(u, ilo + oa, rn) : 0.0;
                     } else {
                         ++j;
                         cj = (j < P) ? s_c[pad16(j)] : __builtin_inf();
@@ -1201,7 +1203,7 @@ __global__ __launch_bounds__(BLOCK) void k_resample(ResArgs a) {
         }
         // wrapped targets (a few at most)
         for (int i = jlo + tid; i < jhi; i += BLOCK) {
-            double t = (double)(target_unwrapped(u, i, n) - 1.0f);
+            double t = (double)(target_at(u, i, rn) - 1.0f);
             const int j = pad16(lds_lower_bound(s_c, last, t));
             xd[i] = s_x[j];
             if constexpr (FISH) {
@@ -1281,6 +1283,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a) {
     const int ilo = s_rng[0], ihi = s_rng[1], jlo = s_rng[2], jhi = s_rng[3];
 
     // children run ends of every parent
+    const double rn = target_recip(n);
     if (a.xbar_next) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -1288,7 +1291,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a) {
             for (int c = 0; c < VEC; ++c) {
                 const int loc = (v * BLOCK + tid) * VEC + c;
                 const double cj = s_c[pad16(loc)];
-                s_bu[loc] = (loc >= last) ? ihi : count_le_fast(cj, u, ilo, ihi, n);
+                s_bu[loc] = (loc >= last) ? ihi : count_targets_le(cj, u, 0.0f, ilo, ihi, n, rn);
             }
         }
     }
@@ -1311,8 +1314,8 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a) {
                 double cs = 0.0;
                 for (int i = u0; i < u1; ++i) cs += (double)a.xbar_next[i];
                 if (jlo < jhi) {   // wrapped targets (a few, usually tile 0's)
-                    int w0 = (loc == 0) ? jlo : count_le(s_c[pad16(loc - 1)], u, jlo, jhi, n, true);
-                    int w1 = (loc >= last) ? jhi : count_le(s_c[pad16(loc)], u, jlo, jhi, n, true);
+                    int w0 = (loc == 0) ? jlo : count_targets_le(s_c[pad16(loc - 1)], u, 1.0f, jlo, jhi, n, rn);
+                    int w1 = (loc >= last) ? jhi : count_targets_le(s_c[pad16(loc)], u, 1.0f, jlo, jhi, n, rn);
                     for (int i = w0; i < w1; ++i) cs += (double)a.xbar_next[i];
                 }
                 xb += phi * cs;
