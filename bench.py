@@ -301,6 +301,8 @@ def batch_only(args):
         n = log_uniform_counts(rng, M, lo, hi)
         mu, phi, sig, seeds = prior_draws(rng, M)
         b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=nu)
+        if score:
+            b.enable_score()         # the autotuner times the kernel that will run
         blk = b.autotune(yy, 64)
         b.close()
         b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=nu, block=blk)
@@ -327,6 +329,8 @@ def batch_and_smc2(args, gpu_bpf, gpu_bpf_batch, torch, y, n, mu, phi, sig, seed
     score = kind == "mala"
     M = len(n)
     b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=nu)
+    if score:
+        b.enable_score()             # the autotuner times the kernel that will run
     blk = b.autotune(y, 64)
     b.close()
     b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=nu, block=blk)

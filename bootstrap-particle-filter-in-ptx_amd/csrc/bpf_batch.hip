@@ -462,14 +462,52 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
         const int ilo = lane_i32(r, 0), ihi = lane_i32(r, 1);
         const int jlo = lane_i32(r, 2), jhi = lane_i32(r, 3);
         wave_lds_sync();
-        for (int i = ilo + lane; i < ihi; i += kWave) {
-            const double t = (double)target_at(u, i, rn);
-            const int p = pad16(lds_lower_bound(cw, last, t));
-            xo[i] = xw[p];
-            if (TAN) {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) dxo[k][i] = s_dx[wid][k][p];
+        // Unwrapped outputs in chunks of kBatchTile: lane `lane` owns a
+        // contiguous run of at most four, finds the first one's parent by a
+        // binary search over the tile CDF and the next ones by stepping or
+        // galloping forward, and stages the parent indices in the wave's tail
+        // queue (idle until the next tick); the wave then gathers the chunk
+        // with coalesced stores.
+        int* par = reinterpret_cast<int*>(s_q[wid]);
+        for (int c0 = 0; c0 < ihi - ilo; c0 += kBatchTile) {
+            const int i0 = ilo + c0;
+            const int cnt = min(kBatchTile, ihi - i0);
+            const int per = (cnt + kWave - 1) / kWave;
+            const int o0 = min(lane * per, cnt), o1 = min(o0 + per, cnt);
+            if (o0 < o1) {
+                int j = lds_lower_bound(cw, last, (double)target_at(u, i0 + o0, rn));
+                double cj = cw[pad16(j)];
+                par[o0] = j;
+                for (int o = o0 + 1; o < o1; ++o) {
+                    const double t = (double)target_at(u, i0 + o, rn);
+                    if (cj < t) {
+                        // t <= c_last, so j < last here: gallop from j + 1
+                        int g0 = j + 1, g1 = g0, step = 1;
+                        while (g1 < last && cw[pad16(g1)] < t) {
+                            g0 = g1 + 1;
+                            step <<= 1;
+                            g1 = min(g0 + step - 1, last);
+                        }
+                        while (g0 < g1) {
+                            const int mid = (g0 + g1) >> 1;
+                            if (cw[pad16(mid)] < t) g0 = mid + 1; else g1 = mid;
+                        }
+                        j = g0;
+                        cj = cw[pad16(j)];
+                    }
+                    par[o] = j;
+                }
             }
+            wave_lds_sync();
+            for (int o = lane; o < cnt; o += kWave) {
+                const int p = pad16(par[o]);
+                xo[i0 + o] = xw[p];
+                if (TAN) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) dxo[k][i0 + o] = s_dx[wid][k][p];
+                }
+            }
+            wave_lds_sync();      // par is refilled by the next chunk
         }
         for (int i = jlo + lane; i < jhi; i += kWave) {
             const double t = (double)(target_at(u, i, rn) - 1.0f);

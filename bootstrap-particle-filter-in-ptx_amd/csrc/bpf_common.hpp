@@ -338,11 +338,8 @@ __device__ inline int count_targets_le(double v, float u, float sub, int lo, int
     return g;
 }
 
-// Builds the tile's CDF in s_c (TILE doubles) from the lane's weights e[K]
-// (coalesced layout) and returns the tile's output ranges in s_rng:
-// [0,1) unwrapped targets, [2,3) wrapped targets.  Returns the local index of
-// the tile's last valid particle.  s_c doubles as the staging area of e.
-// Contains block barriers; every thread of the block must call it.
+// First j in [0, last] with c_j >= t over a non-decreasing padded CDF
+// (last when none is).
 __device__ inline int lds_lower_bound(const double* c, int last, double t) {
     int lo = 0, hi = last;
     while (lo < hi) {

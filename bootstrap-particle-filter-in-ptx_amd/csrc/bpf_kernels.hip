@@ -832,12 +832,18 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
 // (gpu_bpf/gpu_bpf.cu:297-310, clamp to N-1 as ptx:816-830) on this CDF.
 // ---------------------------------------------------------------------------
 
-// next_out (nullable): thread 0 stores `next` there before the first barrier;
-// pre() runs on every thread right after that barrier (the caller's next
-// tile loads, issued while this tile's CDF is built).
 struct NoPre {
     __device__ void operator()() const {}
 };
+
+// Builds the tile's CDF in s_c (TILE doubles) from the lane's weights e[K]
+// (coalesced layout) and returns the tile's output ranges in s_rng:
+// [0,1) unwrapped targets, [2,3) wrapped targets.  Returns the local index of
+// the tile's last valid particle.  s_c doubles as the staging area of e.
+// next_out (nullable): thread 0 stores `next` there before the first barrier;
+// pre() runs on every thread right after that barrier (the caller's next
+// tile loads, issued while this tile's CDF is built).
+// Contains block barriers; every thread of the block must call it.
 template <int BLOCK, int K, class Pre = NoPre>
 __device__ inline int tile_cdf(const float (&e)[K], int b, int n, int n_tiles, double ft,
                                double Bb, double Bb1, float u, int W, double* s_c,

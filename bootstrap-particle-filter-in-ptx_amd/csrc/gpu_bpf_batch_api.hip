@@ -305,7 +305,9 @@ int gpu_bpf_batch_autotune(GpuBpfBatch* b, const float* y, int ticks) {
     for (int blk : kBlocks) {
         GpuBpfBatch* s = new_batch(b->M, b->n.data(), b->mu.data(), b->phi.data(), b->sig.data(),
                                    b->seed.data(), b->obs.nu, b->threshold);
-        if (create_impl(s)) {
+        // the scratch batch runs the kernel this batch will run: with the
+        // score's tangent or Fisher vectors when the score is enabled
+        if (create_impl(s) || (b->d_score && enable_score(s, b->fisher))) {
             destroy_batch(s);
             return -1;
         }

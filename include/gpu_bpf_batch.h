@@ -36,9 +36,10 @@ void gpu_bpf_batch_destroy(GpuBpfBatch* batch);
  * autotuner at the first run.  Returns 0 or -1. */
 int gpu_bpf_batch_set_block(GpuBpfBatch* batch, int threads);
 int gpu_bpf_batch_get_block(GpuBpfBatch* batch);
-/* Times each block size on a scratch batch of the same filters: two untimed
- * ticks on y[0..1], then ticks - 2 timed ones on the rest of y (ticks >= 3
- * values); keeps the fastest and returns it, or -1. */
+/* Times each block size on a scratch batch of the same filters, with the
+ * same score estimator when this batch's score is enabled (enable it first):
+ * two untimed ticks on y[0..1], then ticks - 2 timed ones on the rest of y
+ * (ticks >= 3 values); keeps the fastest and returns it, or -1. */
 int gpu_bpf_batch_autotune(GpuBpfBatch* batch, const float* y, int ticks);
 
 /* Advance every filter by n_ticks observations (host or device array);

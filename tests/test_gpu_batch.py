@@ -137,6 +137,27 @@ def test_autotune_picks_a_block(ydata):
     b.close()
 
 
+@pytest.mark.parametrize("estimator", ["pathwise", "fisher"])
+def test_autotune_with_score_keeps_the_score(ydata, estimator):
+    """With the score enabled the autotuner times the score kernels; the
+    tuned batch's score and log-likelihoods equal an untuned batch's."""
+    rng = np.random.default_rng(11)
+    n, mu, phi, sig, seeds = filters(24, rng, 256, 4000)
+    say("autotune with score", estimator)
+    b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds)
+    b.enable_score(estimator)
+    blk = b.autotune(ydata, 20)
+    assert blk in (64, 128, 256, 512, 1024) and b.block == blk
+    b.run(ydata[:30], stats=False)
+    ref = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, block=64 if blk != 64 else 128)
+    ref.enable_score(estimator)
+    ref.run(ydata[:30], stats=False)
+    np.testing.assert_array_equal(b.score(), ref.score())
+    np.testing.assert_array_equal(b.loglik(), ref.loglik())
+    b.close()
+    ref.close()
+
+
 def test_batch_errors():
     with pytest.raises(gpu_bpf.GpuBpfError, match="particles"):
         gpu_bpf_batch.GpuBpfBatch([100, 20000], 0.0, 0.9, 0.1, [1, 2])
