@@ -113,6 +113,19 @@ __device__ inline void lane_normals(uint64_t pcg_a, uint64_t pcg_g, long long ba
     }
 }
 
+// Agent-scope (every XCD) relaxed loads and stores: a finalise block hands
+// its group record to the last block within the same launch.  The stores
+// write through the XCD's L2 and the loads bypass it, so no release or
+// acquire fence (a write-back or invalidation of the whole L2) is needed.
+template <class T>
+__device__ inline T ld_agent(const T* p) {
+    return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ inline void st_agent(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ---------------------------------------------------------------------------
 // Group fold (one block of kGroup threads, thread t <-> tile g*kGroup + t):
 // the group's tile partials, written by the previous kernel, rescaled to the
@@ -171,11 +184,11 @@ __device__ inline double group_fold(const Partials& p, int g, int tid, float* s_
             for (int w = 1; w < NW; ++w) acc += s_d[q][w];
             r[q] = acc;
         }
-        p.gm[g] = gmax;
-        p.gs[g] = r[0];
-        p.gsx[g] = r[1];
-        p.gsxx[g] = r[2];
-        p.gsq[g] = r[3];
+        st_agent(p.gm + g, gmax);
+        st_agent(p.gs + g, r[0]);
+        st_agent(p.gsx + g, r[1]);
+        st_agent(p.gsxx + g, r[2]);
+        st_agent(p.gsq + g, r[3]);
     }
     return f;
 }
@@ -623,8 +636,8 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
 
 // ---------------------------------------------------------------------------
 // Finalise: one block per reduction group folds the group's tiles; the block
-// that arrives last (arrival counter; each block publishes its group record
-// with an agent-scope fence first) reduces the group records, takes the
+// that arrives last (arrival counter; each block's agent-scope record stores
+// are acknowledged before it arrives) reduces the group records, takes the
 // tick's decisions and, on resampling ticks, builds the FP64 group prefix.
 // ---------------------------------------------------------------------------
 constexpr int kFinBlock = kGroup;
@@ -660,18 +673,24 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
             double acc = s_l[tid][0];
 #pragma unroll
             for (int w = 1; w < NW; ++w) acc += s_l[tid][w];
-            a.glsum[(size_t)tid * Gg + blockIdx.x] = acc;
+            st_agent(a.glsum + (size_t)tid * Gg + blockIdx.x, acc);
         }
         __syncthreads();
     }
     if (tid == 0) {
-        __threadfence();
+        // the group record (and its learning sums, stored by lanes 1-3 of
+        // this wave) acknowledged before the group counts itself in; the
+        // signal fences keep the compiler from moving stores or the counter
+        // update across the wait
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
         const int old = atomicAdd(a.part.gcnt, 1);
         s_last = (old == (int)gridDim.x - 1);
     }
     __syncthreads();
     if (!s_last) return;
-    __threadfence();
+    // the records are read below at agent scope: no acquire fence
     if (tid == 0) *a.part.gcnt = 0;
 
     // thread tid owns the contiguous groups [tid*C, tid*C + C), C <= kFinPer;
