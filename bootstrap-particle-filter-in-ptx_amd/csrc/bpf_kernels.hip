@@ -1027,8 +1027,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(FISH ? 1 
     ObsModel obs = a.obs;
     {
         // a captured tick runs after the finalise pass advanced ctl->tick
+        // the device observation and the by-value one are selected as values:
+        // a select of the two addresses would copy a.y to scratch
+        const float y_dev = a.y_ptr ? *a.y_ptr : 0.0f;
         const float yv = a.tick_from_ctl ? ctl->y_base[ctl->tick - 1 - ctl->y_tick0]
-                                         : (a.y_ptr ? *a.y_ptr : a.y);
+                                         : (a.y_ptr ? y_dev : a.y);
         obs.ysq = yv * yv;
     }
 
