@@ -223,19 +223,41 @@ __device__ inline float host_pcg_float(unsigned long long* st) {
 }
 
 // W = first i in [0, n] with u + i/n >= 1 (FP32, gpu_bpf/gpu_bpf.cu:300-302);
-// the targets are non-decreasing in i.  Starts from i ~ (1 - u) n and
-// searches a window around it, widened to [0, n] if it does not bracket W.
+// the targets are non-decreasing in i.  Starts from the real-number estimate
+// ceil((1 - u) n), which the float rounding moves by a few positions at most,
+// and steps to the exact boundary; bisects [0, n] only if that takes more
+// than 16 steps down or 32 in all.
 __device__ inline int wrap_start(float u, int n) {
-    long long g = (long long)ceil((1.0 - (double)u) * (double)n);
-    int lo = (int)max(0LL, min((long long)n, g - 512));
-    int hi = (int)max(0LL, min((long long)n, g + 512));
-    if (lo > 0 && target_unwrapped(u, lo - 1, n) >= 1.0f) lo = 0;
-    if (hi < n && target_unwrapped(u, hi, n) < 1.0f) hi = n;
-    while (lo < hi) {
-        int mid = (int)(((unsigned)lo + (unsigned)hi) >> 1);
-        if (target_unwrapped(u, mid, n) >= 1.0f) hi = mid; else lo = mid + 1;
+    const double rn = target_recip(n);
+    const long long est = (long long)ceil((1.0 - (double)u) * (double)n);
+    int g = (int)max(0LL, min((long long)n, est));
+    int steps = 0;
+    bool ok = true;
+    while (g > 0 && target_at(u, g - 1, rn) >= 1.0f) {
+        --g;
+        if (++steps > 16) {
+            ok = false;
+            break;
+        }
     }
-    return lo;
+    if (ok) {
+        while (g < n && target_at(u, g, rn) < 1.0f) {
+            ++g;
+            if (++steps > 32) {
+                ok = false;
+                break;
+            }
+        }
+    }
+    if (!ok) {
+        int lo = 0, hi = n;
+        while (lo < hi) {
+            const int mid = (int)(((unsigned)lo + (unsigned)hi) >> 1);
+            if (target_at(u, mid, rn) >= 1.0f) hi = mid; else lo = mid + 1;
+        }
+        g = lo;
+    }
+    return g;
 }
 
 // The tick's statistics and decisions from the global sums (one thread, on
