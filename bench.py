@@ -644,7 +644,8 @@ def main():
                 "propagate_mean": float(prop_us.mean()),
                 "propagate_p5_p50_p95": [float(v) for v in np.percentile(prop_us, [5, 50, 95])],
                 "propagate_min_max": [float(prop_us.min()), float(prop_us.max())],
-                "finalize_mean": float(fin_ms.mean()) * 1e3,
+                # (small filters finalise inside the propagate launch: no samples)
+                "finalize_mean": float(fin_ms.mean()) * 1e3 if fin_ms.size else None,
                 "resample_on_resample_ticks": (float(np.mean(tick_ms[rs_t, 2])) * 1e3
                                                if rs_t.any() else None),
                 "resample_on_other_ticks": float(np.mean(tick_ms[timed & ~rs, 2])) * 1e3,
