@@ -20,13 +20,6 @@
 
 namespace bpf {
 
-// Orders the wave's own LDS writes before its later LDS reads.
-__device__ inline void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Standard normals z[q] of particles base + 64 q + lane for the draw with
 // jump constants (jm, jg) of stream `seed`.  Tail uniforms (4.85%) are
 // compacted through the wave's LDS queue and evaluated densely.

@@ -49,6 +49,18 @@ __device__ inline double dpp_f64(double v, double id) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// Orders the wave's own LDS writes before its later LDS reads.
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Rank of the calling lane among the lanes of `mask` below it, plus `base`.
+__device__ inline int lanes_below(uint64_t mask, int base) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, base));
+}
+
 // Lane L's value in every lane (L a constant).
 __device__ inline double lane_f64(double v, int L) {
     const unsigned long long b = (unsigned long long)__double_as_longlong(v);

@@ -33,6 +33,19 @@ __device__ inline float pcg_uniform(uint32_t r) {
     return __builtin_amdgcn_fmed3f(u, bits_f32(0x358637BDu), bits_f32(0x3F7FFBCEu));
 }
 
+// The same draw as q = u - 1/2 before the clamp, in two instructions: the
+// 23 bits r >> 9 under the exponent of 1.0 form the float 1 + (r >> 9) 2^-23
+// exactly (one funnel shift), and subtracting 1.5 from it is exact.  The
+// clamp of pcg_uniform moves only uniforms with |u - 1/2| > 0.4999, all in
+// the inverse CDF's tails, so a central draw's q equals pcg_uniform(r) - 1/2
+// bit for bit, and pcg_uniform(r) = uniform_from_half(q) for every r.
+__device__ inline float pcg_half(uint32_t r) {
+    return bits_f32(__builtin_amdgcn_alignbit(0x7Fu, r, 9u)) - 1.5f;
+}
+__device__ inline float uniform_from_half(float q) {
+    return __builtin_amdgcn_fmed3f(q + 0.5f, bits_f32(0x358637BDu), bits_f32(0x3F7FFBCEu));
+}
+
 // exp(x) after S. L. Moshier's Cephes expf: Cody-Waite reduction by ln 2
 // (C1 = 0.693359375, C2 = -2.12194440e-4), degree-5 polynomial, exponent
 // added in the bits.  x < -86 gives 0 (no subnormal results), x > 88 gives
@@ -161,6 +174,9 @@ __device__ inline float icdf_central(float q) {
 __device__ inline bool icdf_is_central(float u) {
     return __builtin_fabsf(u - 0.5f) < bits_f32(0x3EF39581u);
 }
+__device__ inline bool icdf_is_central_half(float q) {
+    return __builtin_fabsf(q) < bits_f32(0x3EF39581u);
+}
 
 // Tails (ptx:185-205): t = sqrt(-2 log p), p = min(u, 1-u), negated above 1/2.
 __device__ inline float icdf_tail(float u) {
@@ -176,7 +192,9 @@ __device__ inline float icdf_tail(float u) {
     den = __builtin_fmaf(t, den, 2.445134137142996e+00f);
     den = __builtin_fmaf(t, den, 3.754408661907416e+00f);
     den = __builtin_fmaf(t, den, 1.0f);
-    float z = num / den;
+    // t in [2.72, 5.26] for the clamped tail uniforms: den in [36, 140] and
+    // |num| = |z| den in [71, 662], inside div_rn_normal's operand range
+    float z = div_rn_normal(num, den);
     return low ? z : -z;
 }
 

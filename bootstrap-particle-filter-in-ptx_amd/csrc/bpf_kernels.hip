@@ -50,66 +50,56 @@ __global__ __launch_bounds__(kBlock) void k_init(InitArgs a) {
 // ---------------------------------------------------------------------------
 // The K standard normals of this lane for one draw index: PCG32 state by
 // jump-ahead (state = pcg_a + inc_i * pcg_g, inc_i = 2i + 1; the next particle
-// of the lane adds 2 pcg_g), uniform, Acklam ICDF.  Central draws are
-// evaluated in place; tail uniforms (4.85%) go through an LDS queue that the
-// whole block evaluates densely, so no wave runs the long tail branch for a
-// single lane.  Item (v, c) of lane tid is particle base + (v*BLOCK+tid)*VEC+c.
-// Contains two block barriers; every thread of the block must call it.
+// of the lane adds 2 pcg_g, the lane's next VEC group 2 BLOCK VEC pcg_g),
+// uniform, Acklam ICDF.  Every item is evaluated as central; the tail items
+// (4.85% of draws) are compacted through the wave's own LDS queue, in item
+// order, by ballot rank, evaluated densely by the wave and read back over
+// the central value, so no wave runs the long tail branch for a single lane.
+// Item (v, c) of lane tid is particle base + (v*BLOCK+tid)*VEC+c.  s_q holds
+// BLOCK*K floats (kWave*K per wave).  No block barrier; the whole wave must
+// call it.
 // ---------------------------------------------------------------------------
 
 template <int BLOCK, int K>
 __device__ inline void lane_normals(uint64_t pcg_a, uint64_t pcg_g, long long base, int tid,
-                                    float* s_q, int* s_cnt, float (&z)[K]) {
+                                    float* s_q, float (&z)[K]) {
     constexpr int VEC = K < 4 ? K : 4;
     constexpr int NV = K / VEC;
-    constexpr int NW = BLOCK / kWave;
     const int lane = tid & (kWave - 1);
-    const int wid = tid / kWave;
-    bool tail[K];
-    int nt = 0;
+    // the wave's queue, as a wave-uniform offset so that each queue address
+    // is one shift-add of the lane's rank onto a scalar
+    float* q_w = s_q + __builtin_amdgcn_readfirstlane((tid / kWave) * (kWave * K));
     const uint64_t g2 = pcg_g * 2u;
+    const uint64_t gv = pcg_g * (uint64_t)(2 * BLOCK * VEC);
+    uint64_t sv = pcg_a + (uint64_t)(2 * (base + (long long)tid * VEC) + 1) * pcg_g;
+    bool tail[K];
+    uint64_t bal[K];                   // the wave's tail lanes of item k
+    int tot = 0;                       // tail items queued so far by the wave
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-        long long i0 = base + (long long)(v * BLOCK + tid) * VEC;
-        uint64_t s = pcg_a + (uint64_t)(2 * i0 + 1) * pcg_g;
+        uint64_t s = sv;
+        sv += gv;
 #pragma unroll
         for (int c = 0; c < VEC; ++c) {
             const int k = v * VEC + c;
-            float u = pcg_uniform(pcg_output(s));
+            const float q = pcg_half(pcg_output(s));
             s += g2;
-            bool cen = icdf_is_central(u);
-            float zc = icdf_central(u - 0.5f);
-            z[k] = cen ? zc : u;    // tail items carry u until the queue pass
-            tail[k] = !cen;
-            nt += cen ? 0 : 1;
+            tail[k] = !icdf_is_central_half(q);
+            z[k] = icdf_central(q);     // replaced below for tail items
+            bal[k] = __ballot(tail[k]);
+            if (tail[k]) q_w[tot + lanes_below(bal[k], 0)] = q;
+            tot += __popcll(bal[k]);
         }
     }
-    // block-wide exclusive scan of the per-lane tail counts
-    int incl = wave_incl_scan(nt, lane);
-    if (lane == kWave - 1) s_cnt[wid] = incl;
-    __syncthreads();
-    int wbase = 0, total = 0;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-        int c = s_cnt[w];
-        wbase += (w < wid) ? c : 0;
-        total += c;
-    }
-    const int first = wbase + incl - nt;
-    int pos = first;
+    if (tot == 0) return;
+    wave_lds_sync();
+    for (int e = lane; e < tot; e += kWave) q_w[e] = icdf_tail(uniform_from_half(q_w[e]));
+    wave_lds_sync();
+    int pos = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        if (tail[k]) s_q[pos] = z[k];
-        pos += tail[k] ? 1 : 0;
-    }
-    __syncthreads();
-    for (int e = tid; e < total; e += BLOCK) s_q[e] = icdf_tail(s_q[e]);
-    __syncthreads();
-    pos = first;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        if (tail[k]) z[k] = s_q[pos];
-        pos += tail[k] ? 1 : 0;
+        if (tail[k]) z[k] = q_w[pos + lanes_below(bal[k], 0)];
+        pos += __popcll(bal[k]);
     }
 }
 
@@ -367,7 +357,6 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
     // ones (non-temporal stores alone: 156 us, loads alone: 152 us)
     constexpr bool NTL = true, NTS = true;
     __shared__ float s_q[TILE];
-    __shared__ int s_cnt[NW];
     __shared__ float s_m[NW];
     __shared__ double s_d[FISH ? 7 : (LEARN ? 8 : 4)][NW];
 
@@ -462,7 +451,7 @@ __global__ __launch_bounds__(kBlock) void k_propagate(PropArgs a) {
 
     if (!APF1 && do_prop) {
         float z[K];
-        lane_normals<kBlock, K>(pcg_a, pcg_g, base, tid, s_q, s_cnt, z);
+        lane_normals<kBlock, K>(pcg_a, pcg_g, base, tid, s_q, z);
         if constexpr (FISH) {
 #pragma unroll
             for (int k = 0; k < K; ++k) {
@@ -1263,7 +1252,6 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a) {
     constexpr int TILE = BLOCK * K;
     constexpr int NW = BLOCK / kWave;
     __shared__ int s_bu[TILE];     // children run end (unwrapped targets) per parent
-    __shared__ int s_cnt[NW];
     __shared__ double s_c[TILE + TILE / 16];
     __shared__ double s_w[NW];
     __shared__ int s_rng[4];
@@ -1284,7 +1272,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a) {
         load_vec<VEC>(a.pre + off, &pre[v * VEC]);
     }
     // draw t of the streams (draw 0, the initial one, at tick 0)
-    lane_normals<BLOCK, K>(a.pcg_a, a.pcg_g, base, tid, s_q, s_cnt, z);
+    lane_normals<BLOCK, K>(a.pcg_a, a.pcg_g, base, tid, s_q, z);
 #pragma unroll
     for (int k = 0; k < K; ++k) x[k] = (a.tick > 0) ? transition(pre[k], a.mu, a.phi, a.sig, z[k]) : pre[k];
     ObsModel obs = a.obs;
