@@ -313,8 +313,9 @@ def batch_only(args):
         if score:
             b.enable_score()         # the autotuner times the kernel that will run
         blk = b.autotune(yy, 64)
+        tile = b.tile
         b.close()
-        b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=nu, block=blk)
+        b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=nu, block=blk, tile=tile)
         if score:
             b.enable_score()
         b.run(yy[:32], stats=False)
@@ -325,7 +326,7 @@ def batch_only(args):
         b.synchronize()
         dt = (time.perf_counter() - t0) / k
         b.close()
-        rows[name] = {"block": blk, "sum_n": int(n.sum()), "ticks": k, "us_per_tick": dt * 1e6,
+        rows[name] = {"block": blk, "tile": tile, "sum_n": int(n.sum()), "ticks": k, "us_per_tick": dt * 1e6,
                       "particle_ticks_per_s": float(n.sum()) / dt, "score": score}
         log(name, rows[name])
     return rows
@@ -341,8 +342,9 @@ def batch_and_smc2(args, gpu_bpf, gpu_bpf_batch, torch, y, n, mu, phi, sig, seed
     if score:
         b.enable_score()             # the autotuner times the kernel that will run
     blk = b.autotune(y, 64)
+    tile = b.tile
     b.close()
-    b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=nu, block=blk)
+    b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=nu, block=blk, tile=tile)
     if score:
         b.enable_score()
     b.run(y[:32], stats=False)
@@ -367,11 +369,11 @@ def batch_and_smc2(args, gpu_bpf, gpu_bpf_batch, torch, y, n, mu, phi, sig, seed
         row["looped_plain_streams_particle_ticks_per_s"] = strm
         row["batch_over_looped_streams"] = batch_pt / strm
         row["looped_streams_sample"] = f"all {M} filters, one stream each, 200 ticks"
-    row.update({"block": blk, "sum_n": int(n.sum())})
+    row.update({"block": blk, "tile": tile, "sum_n": int(n.sum())})
     T = len(y)
     t0 = time.perf_counter()
     s = gpu_bpf_batch.GpuSmc2(M, nmin, nmax, nu_obs=nu, seed=42, kind=kind, moves=moves,
-                              mala_step=0.5, capacity=T, block=blk)
+                              mala_step=0.5, capacity=T, block=blk, tile=tile)
     s.run(y)
     wall = time.perf_counter() - t0
     c = s.counters()
@@ -391,12 +393,12 @@ def batch_and_smc2(args, gpu_bpf, gpu_bpf_batch, torch, y, n, mu, phi, sig, seed
         pick = np.argsort(n)[np.linspace(0, M - 1, 8).astype(int)]
         errs = []
         bs = gpu_bpf_batch.GpuBpfBatch(n[pick], mu[pick], phi[pick], sig[pick], seeds[pick], nu_obs=nu,
-                                       block=blk)
+                                       block=blk, tile=tile)
         bs.run(y[:500], stats=False)
         ll = bs.loglik()
         bs.close()
         for q, m in enumerate(pick):
-            errs.append(cpu_ll_error(y[:500], ll[q], int(n[m]), 256, args.cpu_threads,
+            errs.append(cpu_ll_error(y[:500], ll[q], int(n[m]), tile, args.cpu_threads,
                                      rho=float(phi[m]), sigma_z=float(sig[m]), mu=float(mu[m]),
                                      nu_obs=nu, ess_threshold=0.0, seed=int(seeds[m])))
         row["ll_rel_err_vs_fp64_oracle_max"] = float(max(errs))

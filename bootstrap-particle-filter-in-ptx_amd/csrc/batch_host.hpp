@@ -11,7 +11,8 @@
 
 namespace bpf {
 hipError_t launch_batch_init(const BatchInitArgs& a, hipStream_t st);
-hipError_t launch_batch_tick(const BatchArgs& a, int n_blocks, int block, hipStream_t st);
+hipError_t launch_batch_tick(const BatchArgs& a, int n_blocks, int block, int tile, hipStream_t st);
+bool batch_tick_fits(int block, int tile, bool tangents);
 hipError_t launch_outer_check(const double* logw, const Ctl* ctl, int M, float threshold, int* halt,
                               double* ess_out, hipStream_t st);
 hipError_t launch_smc2_gather(const GatherArgs& a, hipStream_t st);
@@ -19,7 +20,7 @@ hipError_t launch_smc2_accept(const AcceptArgs& a, int n_slots, hipStream_t st);
 }  // namespace bpf
 
 struct GpuBpfBatch {
-    int M = 0, cap = 0, block = 0;
+    int M = 0, cap = 0, block = 0, tile = 0;   // tile: particles per wave tile (0 = 256)
     float threshold = 0.0f;
     bpf::ObsModel obs{};
     hipStream_t stream = nullptr;
@@ -44,7 +45,7 @@ struct GpuBpfBatch {
     int fisher = 0;                    // score estimator: 0 pathwise, 1 Fisher identity
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    int graph_block = 0;
+    int graph_block = 0, graph_tile = 0;
 };
 
 

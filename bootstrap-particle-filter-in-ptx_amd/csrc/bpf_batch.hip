@@ -20,69 +20,80 @@
 
 namespace bpf {
 
-// Standard normals z[q] of particles base + 64 q + lane for the draw with
-// jump constants (jm, jg) of stream `seed`.  Tail uniforms (4.85%) are
-// compacted through the wave's LDS queue and evaluated densely.
+// Item p = q KB + c of a lane is tile particle (64 q + lane) KB + c: the
+// single filter's tile of 256 KB particles at KB particles per lane, with
+// 64-particle lane group q playing that filter's wave q.
+template <int KB>
+__device__ inline int tile_item(int q, int c, int lane) {
+    return (64 * q + lane) * KB + c;
+}
+
+// Standard normals of the lane's 4 KB particles (tile at `base`) for the
+// draw with jump constants (pa, pg).  Every item is evaluated as central;
+// tail uniforms (4.85%) are compacted through the wave's LDS queue s_q
+// (256 KB floats) by ballot rank, evaluated densely and read back over the
+// central value.
+template <int KB>
 __device__ inline void wave_normals(uint64_t pa, uint64_t pg, int base, int lane, float* s_q,
-                                    float (&z)[4]) {
-    bool tail[4];
+                                    float (&z)[4 * KB]) {
+    bool tail[4 * KB];
+    uint64_t bal[4 * KB];
+    int tot = 0;
+    const uint64_t g2 = pg * 2u;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const int i = base + 64 * q + lane;
-        const uint64_t st = pa + (uint64_t)(2 * i + 1) * pg;
-        const float u = pcg_uniform(pcg_output(st));
-        const bool cen = icdf_is_central(u);
-        z[q] = cen ? icdf_central(u - 0.5f) : u;
-        tail[q] = !cen;
+        uint64_t st = pa + (uint64_t)(2 * (base + tile_item<KB>(q, 0, lane)) + 1) * pg;
+#pragma unroll
+        for (int c = 0; c < KB; ++c) {
+            const int p = q * KB + c;
+            const float h = pcg_half(pcg_output(st));
+            st += g2;
+            tail[p] = !icdf_is_central_half(h);
+            z[p] = icdf_central(h);
+            bal[p] = __ballot(tail[p]);
+            if (tail[p]) s_q[tot + lanes_below(bal[p], 0)] = h;
+            tot += __popcll(bal[p]);
+        }
     }
-    int pos[4];
-    int total = 0;
-    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    if (tot == 0) return;
+    wave_lds_sync();
+    for (int e = lane; e < tot; e += kWave) s_q[e] = icdf_tail(uniform_from_half(s_q[e]));
+    wave_lds_sync();
+    int pos = 0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const uint64_t bal = __ballot(tail[q]);
-        pos[q] = total + __popcll(bal & below);
-        total += __popcll(bal);
+    for (int p = 0; p < 4 * KB; ++p) {
+        if (tail[p]) z[p] = s_q[pos + lanes_below(bal[p], 0)];
+        pos += __popcll(bal[p]);
     }
-    if (total == 0) return;
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-        if (tail[q]) s_q[pos[q]] = z[q];
-    wave_lds_sync();
-    for (int e = lane; e < total; e += kWave) s_q[e] = icdf_tail(s_q[e]);
-    wave_lds_sync();
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-        if (tail[q]) z[q] = s_q[pos[q]];
     wave_lds_sync();
 }
 
 // The normals of one tick of slot m under the batch's noise source.
+template <int KB>
 __device__ inline void batch_normals(const BatchArgs& a, int m, uint64_t pa, uint64_t jm,
                                      uint64_t jg, int base, int lane, int zc, size_t zoff,
-                                     float* s_q, float (&z)[4]) {
+                                     float* s_q, float (&z)[4 * KB]) {
     if (a.noise == kNoiseStored || a.noise == kNoisePropose) {
         const float* zs = a.zpool[zc] + zoff + base;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) z[q] = zs[64 * q + lane];
+        for (int q = 0; q < 4; ++q) load_vec<KB>(zs + tile_item<KB>(q, 0, lane), &z[q * KB]);
         if (a.noise == kNoisePropose) {
-            float xi[4];
+            float xi[4 * KB];
             const uint64_t pp = (uint64_t)(int64_t)a.prop_seed[m] * jm;
-            wave_normals(pp, jg, base, lane, s_q, xi);
+            wave_normals<KB>(pp, jg, base, lane, s_q, xi);
             float* zd = a.zpool[1 - zc] + zoff + base;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                z[q] = __builtin_fmaf(a.cn_s, xi[q], a.cn_rho * z[q]);
-                zd[64 * q + lane] = z[q];
-            }
+            for (int p = 0; p < 4 * KB; ++p) z[p] = __builtin_fmaf(a.cn_s, xi[p], a.cn_rho * z[p]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) store_vec<KB>(zd + tile_item<KB>(q, 0, lane), &z[q * KB]);
         }
         return;
     }
-    wave_normals(pa, jg, base, lane, s_q, z);
+    wave_normals<KB>(pa, jg, base, lane, s_q, z);
     if (a.noise == kNoisePcgStore) {
         float* zd = a.zpool[zc] + zoff + base;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) zd[64 * q + lane] = z[q];
+        for (int q = 0; q < 4; ++q) store_vec<KB>(zd + tile_item<KB>(q, 0, lane), &z[q * KB]);
     }
 }
 
@@ -157,17 +168,19 @@ __device__ inline float dlw_dx(float x, const ObsModel& o) {
 // ancestry (the observation density does not depend on theta), and the
 // filter's score is the Fisher-identity estimate sum_i w_ti alpha_ti of the
 // latest tick.
-template <int BLOCK, bool STUDENT, bool TAN, bool FISHER>
+template <int BLOCK, int KB, bool STUDENT, bool TAN, bool FISHER>
 __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
     constexpr int NW = BLOCK / kWave;
-    constexpr int TP = kBatchTile + kBatchTile / 16;       // padded tile
+    constexpr int TB = kBatchTile * KB;                   // particles per tile
+    constexpr int P = 4 * KB;                             // particles per lane per tile
+    constexpr int TP = TB + TB / 16;                      // padded tile
     constexpr float NEG_INF = -__builtin_inff();
-    __shared__ float s_q[NW][kBatchTile];     // tail queue of each wave
+    __shared__ float s_q[NW][TB];             // tail queue of each wave
     __shared__ double s_c[NW][TP];            // in-tile CDF of each wave
     __shared__ float s_x[NW][TP];             // tile particles of each wave
     __shared__ float s_tmax[kBatchMaxTiles];
     __shared__ double s_ts[TAN ? 7 : 4][kBatchMaxTiles];
-    __shared__ float s_dx[TAN ? NW : 1][TAN ? 3 : 1][TP];   // tile tangents of each wave
+    __shared__ float s_dx[TAN ? NW : 1][TAN ? 3 : 1][TAN ? TP : 1];   // tile tangents of each wave
     __shared__ double s_tf[kBatchMaxTiles];
     __shared__ double s_tp[kBatchMaxTiles];
     __shared__ int s_flag, s_W;
@@ -179,7 +192,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const int m = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
     const int n = a.n[m];
-    const int nt = (n + kBatchTile - 1) / kBatchTile;
+    const int nt = (n + TB - 1) / TB;
     Ctl* cp = a.ctl + m;
     const int cur = cp->cur, carry = cp->carry;
     const float moff = carry ? cp->m_prev : 0.0f;
@@ -210,56 +223,67 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
     const size_t zoff = ((size_t)tick * a.n_slots + m) * a.cap;
 
     // Propagate, log-weights (carry as the single filter), tile maxima and
-    // the tile sums of the single filter's propagate pass: one wave sum per
-    // 64-particle group, the four groups added in order.
+    // the tile sums of the single filter's propagate pass: the lane's KB
+    // particles of group q summed in order, one wave sum per group, the four
+    // groups added in order.
     for (int b = wid; b < nt; b += NW) {
-        const int base = b * kBatchTile;
-        float x[4], l[4];
+        const int base = b * TB;
+        float x[P], l[P];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int j = base + 64 * q + lane;
-            x[q] = xb[j];
-            l[q] = carry ? lb[j] : 0.0f;
+            load_vec<KB>(xb + base + tile_item<KB>(q, 0, lane), &x[q * KB]);
+            if (carry) load_vec<KB>(lb + base + tile_item<KB>(q, 0, lane), &l[q * KB]);
+            else {
+#pragma unroll
+                for (int c = 0; c < KB; ++c) l[q * KB + c] = 0.0f;
+            }
         }
-        float d[TAN ? 3 : 1][4];
+        float d[TAN ? 3 : 1][TAN ? P : 1];
         if (TAN) {
 #pragma unroll
             for (int k = 0; k < 3; ++k)
 #pragma unroll
-                for (int q = 0; q < 4; ++q) d[k][q] = dxb[k][base + 64 * q + lane];
+                for (int q = 0; q < 4; ++q) load_vec<KB>(dxb[k] + base + tile_item<KB>(q, 0, lane), &d[k][q * KB]);
         }
         if (tick > 0) {      // tick 0 weights the stationary draw
-            float z[4];
-            batch_normals(a, m, pa, jm, jg, base, lane, zc, zoff, s_q[wid], z);
+            float z[P];
+            batch_normals<KB>(a, m, pa, jm, jg, base, lane, zc, zoff, s_q[wid], z);
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                if (TAN && FISHER) {   // grad log N(x'; mu + phi (x - mu), sigma^2)
+                    const float zs = z[p] / sig;
+                    d[0][p] = __builtin_fmaf(zs, 1.0f - phi, d[0][p]);
+                    d[1][p] = __builtin_fmaf(zs, x[p] - mu, d[1][p]);
+                    d[2][p] += __builtin_fmaf(z[p], z[p], -1.0f) / sig;
+                } else if (TAN) {    // x' = mu + phi (x - mu) + sigma z
+                    d[0][p] = __builtin_fmaf(phi, d[0][p], 1.0f - phi);
+                    d[1][p] = __builtin_fmaf(phi, d[1][p], x[p] - mu);
+                    d[2][p] = __builtin_fmaf(phi, d[2][p], z[p]);
+                }
+                x[p] = transition(x[p], mu, phi, sig, z[p]);
+            }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                if (TAN && FISHER) {   // grad log N(x'; mu + phi (x - mu), sigma^2)
-                    const float zs = z[q] / sig;
-                    d[0][q] = __builtin_fmaf(zs, 1.0f - phi, d[0][q]);
-                    d[1][q] = __builtin_fmaf(zs, x[q] - mu, d[1][q]);
-                    d[2][q] += __builtin_fmaf(z[q], z[q], -1.0f) / sig;
+                store_vec<KB>(xb + base + tile_item<KB>(q, 0, lane), &x[q * KB]);
+                if (TAN) {
 #pragma unroll
-                    for (int k = 0; k < 3; ++k) dxb[k][base + 64 * q + lane] = d[k][q];
-                } else if (TAN) {    // x' = mu + phi (x - mu) + sigma z
-                    d[0][q] = __builtin_fmaf(phi, d[0][q], 1.0f - phi);
-                    d[1][q] = __builtin_fmaf(phi, d[1][q], x[q] - mu);
-                    d[2][q] = __builtin_fmaf(phi, d[2][q], z[q]);
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) dxb[k][base + 64 * q + lane] = d[k][q];
+                    for (int k = 0; k < 3; ++k) store_vec<KB>(dxb[k] + base + tile_item<KB>(q, 0, lane), &d[k][q * KB]);
                 }
-                x[q] = transition(x[q], mu, phi, sig, z[q]);
-                xb[base + 64 * q + lane] = x[q];
             }
         }
         float mloc = NEG_INF;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int j = base + 64 * q + lane;
-            const float lw = STUDENT ? log_weight_t(x[q], obs) : log_weight_gauss(x[q], obs.ysq);
-            const float lk = lw + (l[q] - moff);
-            l[q] = (j < n) ? lk : NEG_INF;
-            mloc = fmaxf(mloc, l[q]);
-            lb[j] = l[q];
+#pragma unroll
+            for (int c = 0; c < KB; ++c) {
+                const int p = q * KB + c;
+                const int j = base + tile_item<KB>(q, c, lane);
+                const float lw = STUDENT ? log_weight_t(x[p], obs) : log_weight_gauss(x[p], obs.ysq);
+                const float lk = lw + (l[p] - moff);
+                l[p] = (j < n) ? lk : NEG_INF;
+                mloc = fmaxf(mloc, l[p]);
+            }
+            store_vec<KB>(lb + base + tile_item<KB>(q, 0, lane), &l[q * KB]);
         }
         const float mt = wave_max(mloc);
         constexpr int NS = TAN ? 7 : 4;
@@ -270,23 +294,32 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
         if (mt != NEG_INF) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int j = base + 64 * q + lane;
-                const double de = (double)expf_cw_nonpos(l[q] - mt);
-                const double dx = (j < n) ? (double)x[q] : 0.0;
-                const double t = de * dx;
-                const double ws = wave_sum(0.0 + de);
-                const double wx = wave_sum(0.0 + t);
-                const double wxx = wave_sum(fma(t, dx, 0.0));
-                const double wq = wave_sum(fma(de, de, 0.0));
+                double S = 0.0, SX = 0.0, SXX = 0.0, Q = 0.0;
+#pragma unroll
+                for (int c = 0; c < KB; ++c) {
+                    const int p = q * KB + c;
+                    const int j = base + tile_item<KB>(q, c, lane);
+                    const double de = (double)expf_cw_nonpos(l[p] - mt);
+                    const double dx = (j < n) ? (double)x[p] : 0.0;
+                    const double t = de * dx;
+                    S += de;
+                    SX += t;
+                    SXX = fma(t, dx, SXX);
+                    Q = fma(de, de, Q);
+                    if (TAN) {
+                        const double eg = (j < n) ? (FISHER ? de : de * (double)dlw_dx<STUDENT>(x[p], obs)) : 0.0;
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) ls[k] = fma(eg, (double)d[k][p], ls[k]);
+                    }
+                }
+                const double ws = wave_sum(S);
+                const double wx = wave_sum(SX);
+                const double wxx = wave_sum(SXX);
+                const double wq = wave_sum(Q);
                 if (q == 0) {
                     acc[0] = ws; acc[1] = wx; acc[2] = wxx; acc[3] = wq;
                 } else {
                     acc[0] += ws; acc[1] += wx; acc[2] += wxx; acc[3] += wq;
-                }
-                if (TAN) {
-                    const double eg = (j < n) ? (FISHER ? de : de * (double)dlw_dx<STUDENT>(x[q], obs)) : 0.0;
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) ls[k] = fma(eg, (double)d[k][q], ls[k]);
                 }
             }
             if (TAN) {
@@ -305,7 +338,6 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
         }
     }
     __syncthreads();
-
     // Fold of the filter's tiles (its one reduction group), statistics,
     // log-likelihood and decision -- the single filter's finalise pass with
     // one group: its empty waves and lanes contribute exact zeros.
@@ -384,8 +416,8 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
     if (!s_flag) return;
 
     // Systematic resampling into the other buffer.  Tile b owns the CDF
-    // interval (b0, b1]; its CDF is the single filter's tile CDF (one
-    // particle per lane, 64-particle group q as wave q), and every target
+    // interval (b0, b1]; its CDF is the single filter's tile CDF (the lane's
+    // KB particles of group q in order, group q as wave q), and every target
     // the tile owns takes the first particle whose CDF value reaches it.
     const double rS = 1.0 / s_S;
     const float u = s_u;
@@ -396,51 +428,65 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
     float* xw = s_x[wid];
     const double rn = target_recip(n);
     for (int b = wid; b < nt; b += NW) {
-        const int base = b * kBatchTile;
+        const int base = b * TB;
         const float mb = s_tmax[b];
         const double pex = (b == 0) ? 0.0 : s_tp[b - 1];
         const double pin = s_tp[b];
         const double ft = (s_tf[b] * fg) * rS;
         const double b0 = fmin(fma(fg * pex, rS, 0.0), 1.0);
         const double b1 = (b == kGroup - 1 || b == nt - 1) ? 1.0 : fmin(fma(fg * pin, rS, 0.0), 1.0);
-        const int last = (b == nt - 1) ? (n - 1 - base) : (kBatchTile - 1);
-        float xv[4], e[4];
+        const int last = (b == nt - 1) ? (n - 1 - base) : (TB - 1);
+        float xv[P], e[P];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int j = base + 64 * q + lane;
-            xv[q] = xb[j];
-            const float lk = lb[j];
-            e[q] = (mb == NEG_INF || j >= n) ? 0.0f : expf_cw(lk - mb);
+            float lk[KB];
+            load_vec<KB>(xb + base + tile_item<KB>(q, 0, lane), &xv[q * KB]);
+            load_vec<KB>(lb + base + tile_item<KB>(q, 0, lane), lk);
+#pragma unroll
+            for (int c = 0; c < KB; ++c) {
+                const int j = base + tile_item<KB>(q, c, lane);
+                e[q * KB + c] = (mb == NEG_INF || j >= n) ? 0.0f : expf_cw(lk[c] - mb);
+            }
         }
-        double cv[4], tot4[4];
+        double excl[4], tot4[4], cl[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const double tot = (double)e[q];
+            double tot = 0.0;
+#pragma unroll
+            for (int c = 0; c < KB; ++c) tot += (double)e[q * KB + c];
             const double incl = wave_incl_scan(tot, lane);
             tot4[q] = wave_last(incl);
-            double excl = incl - tot;
-            for (int r = 0; r < q; ++r) excl += tot4[r];
-            excl = (q == 0 && lane == 0) ? 0.0 : excl;
-            double L = excl;
-            L += (double)e[q];
-            cv[q] = fmin(fma(ft, L, b0), b1);
+            double ex = incl - tot;
+            for (int r = 0; r < q; ++r) ex += tot4[r];
+            ex = (q == 0 && lane == 0) ? 0.0 : ex;
+            excl[q] = ex;
+            double L = ex;                                 // the lane's last CDF value
+#pragma unroll
+            for (int c = 0; c < KB; ++c) L += (double)e[q * KB + c];
+            cl[q] = fmin(fma(ft, L, b0), b1);
         }
         double pm4[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const double pm = wave_incl_max(cv[q], lane);
+            const double pm = wave_incl_max(cl[q], lane);
             double prev = wave_prev(pm, b0);
             pm4[q] = wave_last(pm);
             for (int r = 0; r < q; ++r) prev = fmax(prev, pm4[r]);
-            const int jl = 64 * q + lane;
-            double c = fmax(cv[q], prev);
-            c = (jl == last) ? b1 : c;
-            c = (jl > last) ? __builtin_inf() : c;
-            cw[pad16(jl)] = c;
-            xw[pad16(jl)] = xv[q];
-            if (TAN) {
+            double L = excl[q];
 #pragma unroll
-                for (int k = 0; k < 3; ++k) s_dx[wid][k][pad16(jl)] = dxb[k][base + jl];
+            for (int c = 0; c < KB; ++c) {
+                const int p = q * KB + c;
+                const int jl = tile_item<KB>(q, c, lane);
+                L += (double)e[p];
+                double cv = fmax(fmin(fma(ft, L, b0), b1), prev);
+                cv = (jl == last) ? b1 : cv;
+                cv = (jl > last) ? __builtin_inf() : cv;
+                cw[pad16(jl)] = cv;
+                xw[pad16(jl)] = xv[p];
+                if (TAN) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) s_dx[wid][k][pad16(jl)] = dxb[k][base + jl];
+                }
             }
         }
         // lanes 0, 1: unwrapped targets [0, W) at or below b0, b1; lanes 2,
@@ -455,16 +501,16 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
         const int ilo = lane_i32(r, 0), ihi = lane_i32(r, 1);
         const int jlo = lane_i32(r, 2), jhi = lane_i32(r, 3);
         wave_lds_sync();
-        // Unwrapped outputs in chunks of kBatchTile: lane `lane` owns a
-        // contiguous run of at most four, finds the first one's parent by a
+        // Unwrapped outputs in chunks of TB: lane `lane` owns a
+        // contiguous run of at most 4 KB, finds the first one's parent by a
         // binary search over the tile CDF and the next ones by stepping or
         // galloping forward, and stages the parent indices in the wave's tail
         // queue (idle until the next tick); the wave then gathers the chunk
         // with coalesced stores.
         int* par = reinterpret_cast<int*>(s_q[wid]);
-        for (int c0 = 0; c0 < ihi - ilo; c0 += kBatchTile) {
+        for (int c0 = 0; c0 < ihi - ilo; c0 += TB) {
             const int i0 = ilo + c0;
-            const int cnt = min(kBatchTile, ihi - i0);
+            const int cnt = min(TB, ihi - i0);
             const int per = (cnt + kWave - 1) / kWave;
             const int o0 = min(lane * per, cnt), o1 = min(o0 + per, cnt);
             if (o0 < o1) {
@@ -667,32 +713,62 @@ hipError_t launch_batch_init(const BatchInitArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <int BLOCK, bool STUDENT>
-static void launch_batch_tick_s(const BatchArgs& a, int n_blocks, hipStream_t st) {
-    if (!a.score)
-        hipLaunchKernelGGL((k_batch_tick<BLOCK, STUDENT, false, false>), dim3(n_blocks), dim3(BLOCK), 0, st, a);
-    else if (a.fisher)
-        hipLaunchKernelGGL((k_batch_tick<BLOCK, STUDENT, true, true>), dim3(n_blocks), dim3(BLOCK), 0, st, a);
-    else
-        hipLaunchKernelGGL((k_batch_tick<BLOCK, STUDENT, true, false>), dim3(n_blocks), dim3(BLOCK), 0, st, a);
+// Static LDS of k_batch_tick<BLOCK, KB, *, TAN, *> (the arrays above).
+constexpr size_t batch_tick_lds(int block, int kb, bool tan) {
+    const size_t nw = (size_t)block / kWave, tb = (size_t)kBatchTile * kb, tp = tb + tb / 16;
+    return nw * tb * 4 + nw * tp * 8 + nw * tp * 4 + kBatchMaxTiles * 4 + (tan ? 7 : 4) * kBatchMaxTiles * 8 +
+           (tan ? nw * 3 * tp * 4 : 4) + 2 * kBatchMaxTiles * 8 + 64;
+}
+constexpr size_t kLdsPerWorkgroup = 160 * 1024;
+
+template <int BLOCK, int KB, bool STUDENT, bool TAN, bool FISHER>
+static hipError_t launch_batch_tick_v(const BatchArgs& a, int n_blocks, hipStream_t st) {
+    if constexpr (batch_tick_lds(BLOCK, KB, TAN) <= kLdsPerWorkgroup) {
+        hipLaunchKernelGGL((k_batch_tick<BLOCK, KB, STUDENT, TAN, FISHER>), dim3(n_blocks), dim3(BLOCK), 0, st, a);
+        return hipGetLastError();
+    } else {
+        return hipErrorInvalidConfiguration;    // the tile's LDS does not fit this many waves
+    }
+}
+
+template <int BLOCK, int KB>
+static hipError_t launch_batch_tick_k(const BatchArgs& a, int n_blocks, hipStream_t st) {
+    const bool student = a.obs.nu > 0.0f;
+    if (!a.score) {
+        return student ? launch_batch_tick_v<BLOCK, KB, true, false, false>(a, n_blocks, st)
+                       : launch_batch_tick_v<BLOCK, KB, false, false, false>(a, n_blocks, st);
+    }
+    if (a.fisher) {
+        return student ? launch_batch_tick_v<BLOCK, KB, true, true, true>(a, n_blocks, st)
+                       : launch_batch_tick_v<BLOCK, KB, false, true, true>(a, n_blocks, st);
+    }
+    return student ? launch_batch_tick_v<BLOCK, KB, true, true, false>(a, n_blocks, st)
+                   : launch_batch_tick_v<BLOCK, KB, false, true, false>(a, n_blocks, st);
 }
 
 template <int BLOCK>
-static hipError_t launch_batch_tick_b(const BatchArgs& a, int n_blocks, bool student, hipStream_t st) {
-    if (student) launch_batch_tick_s<BLOCK, true>(a, n_blocks, st);
-    else launch_batch_tick_s<BLOCK, false>(a, n_blocks, st);
-    return hipGetLastError();
+static hipError_t launch_batch_tick_b(const BatchArgs& a, int n_blocks, int tile, hipStream_t st) {
+    switch (tile) {
+        case kBatchTile: return launch_batch_tick_k<BLOCK, 1>(a, n_blocks, st);
+        case 2 * kBatchTile: return launch_batch_tick_k<BLOCK, 2>(a, n_blocks, st);
+        case 4 * kBatchTile: return launch_batch_tick_k<BLOCK, 4>(a, n_blocks, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
-// block: 64, 128, 256, 512 or 1024 threads per inner filter
-hipError_t launch_batch_tick(const BatchArgs& a, int n_blocks, int block, hipStream_t st) {
-    const bool student = a.obs.nu > 0.0f;
+bool batch_tick_fits(int block, int tile, bool tan) {
+    return tile % kBatchTile == 0 && batch_tick_lds(block, tile / kBatchTile, tan) <= kLdsPerWorkgroup;
+}
+
+// block: 64, 128, 256, 512 or 1024 threads per inner filter; tile: 256, 512
+// or 1024 particles (each wave takes whole tiles)
+hipError_t launch_batch_tick(const BatchArgs& a, int n_blocks, int block, int tile, hipStream_t st) {
     switch (block) {
-        case 64: return launch_batch_tick_b<64>(a, n_blocks, student, st);
-        case 128: return launch_batch_tick_b<128>(a, n_blocks, student, st);
-        case 256: return launch_batch_tick_b<256>(a, n_blocks, student, st);
-        case 512: return launch_batch_tick_b<512>(a, n_blocks, student, st);
-        case 1024: return launch_batch_tick_b<1024>(a, n_blocks, student, st);
+        case 64: return launch_batch_tick_b<64>(a, n_blocks, tile, st);
+        case 128: return launch_batch_tick_b<128>(a, n_blocks, tile, st);
+        case 256: return launch_batch_tick_b<256>(a, n_blocks, tile, st);
+        case 512: return launch_batch_tick_b<512>(a, n_blocks, tile, st);
+        case 1024: return launch_batch_tick_b<1024>(a, n_blocks, tile, st);
         default: return hipErrorInvalidValue;
     }
 }

@@ -226,12 +226,15 @@ struct FisherArgs {
 };
 
 // ---------------------------------------------------------------------------
-// Batched inner filters: one workgroup runs one filter's whole tick.  Filters
-// hold at most kBatchMaxN particles, i.e. at most 64 tiles of 256 -- the
-// single filter's own tiles at these sizes (one particle per lane), so each
-// inner filter is one reduction group and reproduces the single filter.
+// Batched inner filters: one workgroup runs one filter's whole tick.  A tile
+// holds 256 KB particles (KB = 1, 2 or 4: the single filter's tile at KB
+// particles per lane), filters hold at most kBatchMaxN particles, i.e. at
+// most 64 tiles of the smallest size, so each inner filter is one reduction
+// group and reproduces the single filter with the same tile.  Slots hold a
+// multiple of the largest tile, so the tile can change between runs.
 // ---------------------------------------------------------------------------
 constexpr int kBatchTile = 256;
+constexpr int kBatchTileMax = 4 * kBatchTile;
 constexpr int kBatchMaxTiles = 64;
 constexpr int kBatchMaxN = kBatchTile * kBatchMaxTiles;
 

@@ -27,6 +27,7 @@ namespace {
 
 constexpr int kGraphTicks = 32;        // launches per captured graph
 constexpr int kBlocks[5] = {64, 128, 256, 512, 1024};
+constexpr int kTiles[3] = {kBatchTile, 2 * kBatchTile, 4 * kBatchTile};
 
 double c_obs_student(double nu) {
     return lgamma((nu + 1.0) / 2.0) - lgamma(nu / 2.0) - 0.5 * log(nu * 3.14159265358979323846);
@@ -197,7 +198,8 @@ static int capture_graph(GpuBpfBatch* b) {
     const BatchArgs a = tick_args(b);
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
-    for (int k = 0; k < kGraphTicks && e == hipSuccess; ++k) e = launch_batch_tick(a, b->M, b->block, b->stream);
+    for (int k = 0; k < kGraphTicks && e == hipSuccess; ++k)
+        e = launch_batch_tick(a, b->M, b->block, b->tile, b->stream);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(b->stream, &g);
     if (e != hipSuccess || e2 != hipSuccess) {
@@ -208,12 +210,19 @@ static int capture_graph(GpuBpfBatch* b) {
     b->graph = g;
     HIP_TRY(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
     b->graph_block = b->block;
+    b->graph_tile = b->tile;
     return 0;
 }
 
 int run_ticks(GpuBpfBatch* b, int n_ticks) {
     if (b->block == 0) b->block = 256;
-    if (n_ticks >= kGraphTicks && (!b->graph_exec || b->graph_block != b->block)) {
+    if (b->tile == 0) b->tile = kBatchTile;
+    if (!batch_tick_fits(b->block, b->tile, b->d_score != nullptr)) {
+        set_err("batch: %d threads with tiles of %d particles need more LDS than one CU holds", b->block,
+                b->tile);
+        return -1;
+    }
+    if (n_ticks >= kGraphTicks && (!b->graph_exec || b->graph_block != b->block || b->graph_tile != b->tile)) {
         if (capture_graph(b)) return -1;
     }
     int done = 0;
@@ -222,7 +231,7 @@ int run_ticks(GpuBpfBatch* b, int n_ticks) {
         done += kGraphTicks;
     }
     const BatchArgs a = tick_args(b);
-    for (; done < n_ticks; ++done) HIP_TRY(launch_batch_tick(a, b->M, b->block, b->stream));
+    for (; done < n_ticks; ++done) HIP_TRY(launch_batch_tick(a, b->M, b->block, b->tile, b->stream));
     b->timestep += n_ticks;
     return 0;
 }
@@ -238,7 +247,7 @@ GpuBpfBatch* new_batch(int M, const int* n, const float* mu, const float* phi, c
     b->seed.assign(seeds, seeds + M);
     b->threshold = thr;
     const int nmax = *std::max_element(b->n.begin(), b->n.end());
-    b->cap = (nmax + kBatchTile - 1) / kBatchTile * kBatchTile;
+    b->cap = (nmax + kBatchTileMax - 1) / kBatchTileMax * kBatchTileMax;
     b->order.resize(M);
     std::iota(b->order.begin(), b->order.end(), 0);
     std::stable_sort(b->order.begin(), b->order.end(), [&](int i, int j) { return b->n[i] > b->n[j]; });
@@ -294,6 +303,21 @@ int gpu_bpf_batch_set_block(GpuBpfBatch* b, int threads) {
 
 int gpu_bpf_batch_get_block(GpuBpfBatch* b) { return b ? b->block : -1; }
 
+int gpu_bpf_batch_set_tile(GpuBpfBatch* b, int particles) {
+    if (!b) {
+        set_err("gpu_bpf_batch_set_tile: null batch");
+        return -1;
+    }
+    if (particles != 0 && particles != kTiles[0] && particles != kTiles[1] && particles != kTiles[2]) {
+        set_err("gpu_bpf_batch_set_tile: %d is not 0, 256, 512 or 1024", particles);
+        return -1;
+    }
+    b->tile = particles;
+    return 0;
+}
+
+int gpu_bpf_batch_get_tile(GpuBpfBatch* b) { return b ? (b->tile ? b->tile : kBatchTile) : -1; }
+
 int gpu_bpf_batch_autotune(GpuBpfBatch* b, const float* y, int ticks) {
     // two untimed ticks, then ticks - 2 timed ones: y holds `ticks` values
     if (!b || !y || ticks < 3) {
@@ -301,39 +325,46 @@ int gpu_bpf_batch_autotune(GpuBpfBatch* b, const float* y, int ticks) {
         return -1;
     }
     float best_ms = 0.0f;
-    int best = 0;
-    for (int blk : kBlocks) {
-        GpuBpfBatch* s = new_batch(b->M, b->n.data(), b->mu.data(), b->phi.data(), b->sig.data(),
-                                   b->seed.data(), b->obs.nu, b->threshold);
-        // the scratch batch runs the kernel this batch will run: with the
-        // score's tangent or Fisher vectors when the score is enabled
-        if (create_impl(s) || (b->d_score && enable_score(s, b->fisher))) {
+    int best = 0, best_tile = 0;
+    for (int tile : kTiles) {
+        for (int blk : kBlocks) {
+            // the scratch batch runs the kernel this batch will run: with the
+            // score's tangent or Fisher vectors when the score is enabled;
+            // (block, tile) pairs whose LDS does not fit one CU are skipped
+            if (!batch_tick_fits(blk, tile, b->d_score != nullptr)) continue;
+            GpuBpfBatch* s = new_batch(b->M, b->n.data(), b->mu.data(), b->phi.data(), b->sig.data(),
+                                       b->seed.data(), b->obs.nu, b->threshold);
+            if (create_impl(s) || (b->d_score && enable_score(s, b->fisher))) {
+                destroy_batch(s);
+                return -1;
+            }
+            s->block = blk;
+            s->tile = tile;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            float ms = 0.0f;
+            int rc = -1;
+            if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
+                append_y(s, y, ticks, hipMemcpyHostToDevice) == 0 && run_ticks(s, 2) == 0 &&
+                hipEventRecord(e0, s->stream) == hipSuccess && run_ticks(s, ticks - 2) == 0 &&
+                hipEventRecord(e1, s->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+                hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+                rc = 0;
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
             destroy_batch(s);
-            return -1;
-        }
-        s->block = blk;
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        float ms = 0.0f;
-        int rc = -1;
-        if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
-            append_y(s, y, ticks, hipMemcpyHostToDevice) == 0 && run_ticks(s, 2) == 0 &&
-            hipEventRecord(e0, s->stream) == hipSuccess && run_ticks(s, ticks - 2) == 0 &&
-            hipEventRecord(e1, s->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
-            hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
-            rc = 0;
-        if (e0) (void)hipEventDestroy(e0);
-        if (e1) (void)hipEventDestroy(e1);
-        destroy_batch(s);
-        if (rc) {
-            set_err("gpu_bpf_batch_autotune: timing run failed");
-            return -1;
-        }
-        if (best == 0 || ms < best_ms) {
-            best_ms = ms;
-            best = blk;
+            if (rc) {
+                set_err("gpu_bpf_batch_autotune: timing run failed");
+                return -1;
+            }
+            if (best == 0 || ms < best_ms) {
+                best_ms = ms;
+                best = blk;
+                best_tile = tile;
+            }
         }
     }
     b->block = best;
+    b->tile = best_tile;
     return best;
 }
 

@@ -63,7 +63,7 @@ struct TickGraph {
     hipGraph_t g = nullptr;
     hipGraphExec_t e = nullptr;
     const float* y = nullptr;         // history pointer the capture used
-    int block = 0;
+    int block = 0, tile = 0;
     void reset() {
         if (e) (void)hipGraphExecDestroy(e);
         if (g) (void)hipGraphDestroy(g);
@@ -167,12 +167,18 @@ static int enqueue_run(GpuSmc2* s, TickGraph& tg, const BatchArgs& a, bool check
                        hipStream_t st) {
     GpuBpfBatch* b = check ? s->main : s->rep;
     if (b->block == 0) b->block = 256;
-    if (n_ticks >= kRunTicks && (!tg.e || tg.y != a.y || tg.block != b->block)) {
+    if (b->tile == 0) b->tile = kBatchTile;
+    if (!batch_tick_fits(b->block, b->tile, a.score != nullptr)) {
+        set_err("gpu_smc2: %d threads with tiles of %d particles need more LDS than one CU holds", b->block,
+                b->tile);
+        return -1;
+    }
+    if (n_ticks >= kRunTicks && (!tg.e || tg.y != a.y || tg.block != b->block || tg.tile != b->tile)) {
         tg.reset();
         HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         hipError_t e = hipSuccess;
         for (int k = 0; k < kRunTicks && e == hipSuccess; ++k) {
-            e = launch_batch_tick(a, s->M, b->block, st);
+            e = launch_batch_tick(a, s->M, b->block, b->tile, st);
             if (e == hipSuccess && check)
                 e = launch_outer_check(s->d_logw, s->main->d_ctl, s->M, s->outer_thr, s->d_halt, s->d_ess, st);
         }
@@ -187,6 +193,7 @@ static int enqueue_run(GpuSmc2* s, TickGraph& tg, const BatchArgs& a, bool check
         HIP_TRY(hipGraphInstantiate(&tg.e, g, nullptr, nullptr, 0));
         tg.y = a.y;
         tg.block = b->block;
+        tg.tile = b->tile;
     }
     int done = 0;
     while (n_ticks - done >= kRunTicks) {
@@ -194,7 +201,7 @@ static int enqueue_run(GpuSmc2* s, TickGraph& tg, const BatchArgs& a, bool check
         done += kRunTicks;
     }
     for (; done < n_ticks; ++done) {
-        HIP_TRY(launch_batch_tick(a, s->M, b->block, st));
+        HIP_TRY(launch_batch_tick(a, s->M, b->block, b->tile, st));
         if (check) HIP_TRY(launch_outer_check(s->d_logw, s->main->d_ctl, s->M, s->outer_thr, s->d_halt, s->d_ess, st));
     }
     return 0;
@@ -1000,6 +1007,15 @@ int gpu_smc2_set_block(GpuSmc2* s, int threads) {
         return -1;
     }
     if (gpu_bpf_batch_set_block(s->main, threads) || gpu_bpf_batch_set_block(s->rep, threads)) return -1;
+    return 0;
+}
+
+int gpu_smc2_set_tile(GpuSmc2* s, int particles) {
+    if (!s || !s->ready) {
+        set_err("gpu_smc2_set_tile: not initialised");
+        return -1;
+    }
+    if (gpu_bpf_batch_set_tile(s->main, particles) || gpu_bpf_batch_set_tile(s->rep, particles)) return -1;
     return 0;
 }
 

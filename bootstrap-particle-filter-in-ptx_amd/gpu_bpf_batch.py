@@ -28,6 +28,8 @@ def _lib():
             "gpu_bpf_batch_destroy": (None, [P]),
             "gpu_bpf_batch_set_block": (I, [P, I]),
             "gpu_bpf_batch_get_block": (I, [P]),
+            "gpu_bpf_batch_set_tile": (I, [P, I]),
+            "gpu_bpf_batch_get_tile": (I, [P]),
             "gpu_bpf_batch_autotune": (I, [P, P, I]),
             "gpu_bpf_batch_run": (I, [P, P, I]),
             "gpu_bpf_batch_run_device": (I, [P, P, I]),
@@ -55,7 +57,8 @@ def _ptr(a):
 class GpuBpfBatch:
     """M inner filters; per-filter arrays n, mu, phi, sigma_z, seeds."""
 
-    def __init__(self, n, mu, phi, sigma_z, seeds, nu_obs=0.0, ess_threshold=0.0, block=None):
+    def __init__(self, n, mu, phi, sigma_z, seeds, nu_obs=0.0, ess_threshold=0.0, block=None,
+                 tile=None):
         self.L = _lib()
         self.n = np.ascontiguousarray(n, np.int32)
         self.M = int(self.n.size)
@@ -70,6 +73,8 @@ class GpuBpfBatch:
             raise gpu_bpf.GpuBpfError(gpu_bpf.last_error())
         if block is not None:
             self._check(self.L.gpu_bpf_batch_set_block(self.h, int(block)))
+        if tile is not None:
+            self._check(self.L.gpu_bpf_batch_set_tile(self.h, int(tile)))
 
     def _check(self, rc):
         if rc < 0:
@@ -87,7 +92,17 @@ class GpuBpfBatch:
     def block(self):
         return self.L.gpu_bpf_batch_get_block(self.h)
 
+    @property
+    def tile(self):
+        """Particles per tile (256, 512 or 1024)."""
+        return self.L.gpu_bpf_batch_get_tile(self.h)
+
+    def set_tile(self, particles):
+        self._check(self.L.gpu_bpf_batch_set_tile(self.h, int(particles)))
+
     def autotune(self, y, ticks=64):
+        """Times every (block, tile) pair that fits; keeps the fastest and
+        returns its block (its tile is self.tile)."""
         y = np.ascontiguousarray(y[:ticks], np.float32)
         return self._check(self.L.gpu_bpf_batch_autotune(self.h, _ptr(y), y.size))
 
@@ -166,6 +181,7 @@ def _smc2_lib():
             "gpu_smc2_get_loglik": (I, [P, P]),
             "gpu_smc2_get_counters": (I, [P, P]),
             "gpu_smc2_set_block": (I, [P, I]),
+            "gpu_smc2_set_tile": (I, [P, I]),
             "gpu_smc2_set_mala_step": (I, [P, F]),
             "gpu_smc2_selftest_mcmc": (I, [I, I, I, I, P]),
             "gpu_smc2_get_history": (I, [P, P, P, I]),
@@ -189,7 +205,7 @@ class GpuSmc2:
 
     def __init__(self, n_theta, n_min, n_max, nu_obs=0.0, seed=42, moves=5, cpmmh_rho=0.99,
                  outer_threshold=0.5, capacity=2048, prior=None, theta_scale=1.0, u_step=0.05,
-                 block=None, kind="cpmmh", mala_step=0.5):
+                 block=None, kind="cpmmh", mala_step=0.5, tile=None):
         self.L = _smc2_lib()
         self.M = int(n_theta)
         self.h = self.L.gpu_smc2_create(self.M, int(n_min), int(n_max), float(nu_obs), int(seed))
@@ -209,6 +225,8 @@ class GpuSmc2:
         self._check(self.L.gpu_smc2_init_from_prior(self.h))
         if block is not None:
             self._check(self.L.gpu_smc2_set_block(self.h, int(block)))
+        if tile is not None:
+            self._check(self.L.gpu_smc2_set_tile(self.h, int(tile)))
 
     def _check(self, rc):
         if rc < 0:

@@ -36,10 +36,18 @@ void gpu_bpf_batch_destroy(GpuBpfBatch* batch);
  * autotuner at the first run.  Returns 0 or -1. */
 int gpu_bpf_batch_set_block(GpuBpfBatch* batch, int threads);
 int gpu_bpf_batch_get_block(GpuBpfBatch* batch);
-/* Times each block size on a scratch batch of the same filters, with the
- * same score estimator when this batch's score is enabled (enable it first):
- * two untimed ticks on y[0..1], then ticks - 2 timed ones on the rest of y
- * (ticks >= 3 values); keeps the fastest and returns it, or -1. */
+/* Particles per tile (256, 512 or 1024; 0 = 256): each wave of an inner
+ * filter's workgroup takes whole tiles, 4 tile/256 particles per lane, and
+ * each inner filter is the plain filter with the same tile
+ * (gpu_bpf_set_items_per_thread(tile / 256)).  Returns 0 or -1. */
+int gpu_bpf_batch_set_tile(GpuBpfBatch* batch, int particles);
+int gpu_bpf_batch_get_tile(GpuBpfBatch* batch);
+/* Times every (block, tile) pair whose LDS fits one CU on a scratch batch of
+ * the same filters, with the same score estimator when this batch's score
+ * is enabled (enable it first): two untimed ticks on y[0..1], then
+ * ticks - 2 timed ones on the rest of y (ticks >= 3 values); keeps the
+ * fastest pair and returns its block (gpu_bpf_batch_get_tile gives its
+ * tile), or -1. */
 int gpu_bpf_batch_autotune(GpuBpfBatch* batch, const float* y, int ticks);
 
 /* Advance every filter by n_ticks observations (host or device array);
@@ -141,6 +149,8 @@ int gpu_smc2_get_history(GpuSmc2* smc2, int* ticks, long long* sum_n, int max_en
  * and are never chosen; returns -1 when none is finite. */
 int gpu_smc2_systematic_ancestors(const double* log_w, int n, double u, int* anc);
 int gpu_smc2_set_block(GpuSmc2* smc2, int threads);
+/* Particles per tile of the inner filters (as gpu_bpf_batch_set_tile). */
+int gpu_smc2_set_tile(GpuSmc2* smc2, int particles);
 /* Verification aid (host only, no GPU): the rejuvenation moves' proposal and
  * acceptance code on a closed-form 3-d Gaussian target; kind as above;
  * fault 1 drops the MALA proposal correction.  out: mean[3], cov[9],

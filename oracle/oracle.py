@@ -90,7 +90,8 @@ class OrcSmc2Cfg(ctypes.Structure):
                 ("seed", ctypes.c_int), ("kind", ctypes.c_int), ("moves", ctypes.c_int),
                 ("nu_obs", ctypes.c_float), ("cn_rho", ctypes.c_float), ("u_step", ctypes.c_float),
                 ("prop_scale", ctypes.c_float), ("mala_h", ctypes.c_float),
-                ("outer_thr", ctypes.c_float), ("lo", ctypes.c_float * 3), ("hi", ctypes.c_float * 3)]
+                ("outer_thr", ctypes.c_float), ("lo", ctypes.c_float * 3), ("hi", ctypes.c_float * 3),
+                ("tile", ctypes.c_int)]
 
 
 class OrcSmc2Count(ctypes.Structure):
@@ -604,9 +605,9 @@ def rb_half_roundtrip(x):
 
 def run_sv_smc2(y, n_theta, n_min, n_max, seed=42, kind="cpmmh", moves=5, nu_obs=0.0, cn_rho=0.99,
                 u_step=0.05, theta_scale=1.0, mala_step=0.5, outer_threshold=0.5,
-                prior=((-2.0, 0.90, 0.05), (0.0, 0.995, 0.40)), fault=0):
+                prior=((-2.0, 0.90, 0.05), (0.0, 0.995, 0.40)), fault=0, tile=256):
     """SMC^2 over batched SV bootstrap filters (smc2_oracle.c), with the
-    arguments of gpu_bpf_batch.GpuSmc2.  Returns a dict: theta [M, 3], n [M],
+    arguments of gpu_bpf_batch.GpuSmc2 (tile: particles per inner tile).  Returns a dict: theta [M, 3], n [M],
     logw [M], ll [M], score [M, 3], ess [T], history (ticks, sum_n), counters."""
     y = np.ascontiguousarray(y, np.float32)
     T, M = len(y), int(n_theta)
@@ -618,6 +619,7 @@ def run_sv_smc2(y, n_theta, n_min, n_max, seed=42, kind="cpmmh", moves=5, nu_obs
     c.prop_scale, c.mala_h, c.outer_thr = float(theta_scale), float(mala_step), float(outer_threshold)
     for k in range(3):
         c.lo[k], c.hi[k] = float(prior[0][k]), float(prior[1][k])
+    c.tile = int(tile)
     th = np.zeros((M, 3), np.float32)
     n = np.zeros(M, np.int32)
     lw = np.zeros(M, np.float64)

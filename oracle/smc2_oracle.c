@@ -55,6 +55,7 @@
 #define S2_MULT 6364136223846793005ULL
 
 static int g_sfault = 0;
+static int g_tile = S2_TILE;   /* tile of the run in progress (orc_sv_smc2_run) */
 
 /* Plant a mistake (tests only): 0 none, 1 drop the MALA proposal-density
  * correction, 2 Crank-Nicolson with 1 - rho^2 in place of sqrt(1 - rho^2),
@@ -210,7 +211,7 @@ static double slot_tick(Slot* s, float y, const Obs* o, const Noise* nz, Work* w
         }
     }
     for (int i = 0; i < n; i++) wk->l[i] = orc_lw(s->x[i], ysq, o->nu, o->c_obs, o->nph);
-    const float M = orc_tiled_weights(wk->l, n, S2_TILE, wk->tmax, wk->tf, wk->w);
+    const float M = orc_tiled_weights(wk->l, n, g_tile, wk->tmax, wk->tf, wk->w);
     double S = 0.0, Q = 0.0;
     for (int i = 0; i < n; i++) {
         S += wk->w[i];
@@ -374,6 +375,7 @@ typedef struct {
     int moves;
     float nu_obs, cn_rho, u_step, prop_scale, mala_h, outer_thr;
     float lo[3], hi[3];
+    int tile;            /* particles per tile of the inner filters (0: 256) */
 } OrcSmc2Cfg;
 
 typedef struct {
@@ -413,6 +415,8 @@ int orc_sv_smc2_run(const OrcSmc2Cfg* c, const float* y, int T, float* theta_out
                     int* hist_tick, long long* hist_sum_n, int hist_cap, OrcSmc2Count* cnt) {
     const int M = c->n_theta;
     if (M <= 0 || c->n_min <= 0 || c->n_max < c->n_min || c->n_max > S2_MAXN || T < 0) return -1;
+    if (c->tile != 0 && c->tile != 256 && c->tile != 512 && c->tile != 1024) return -1;
+    g_tile = c->tile ? c->tile : S2_TILE;
     const int mala = c->kind == 1;
     const int cap = (c->n_max + S2_TILE - 1) / S2_TILE * S2_TILE;
     const int store = !mala;

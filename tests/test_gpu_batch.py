@@ -97,6 +97,53 @@ def test_block_sizes_give_identical_results(ydata, block):
     ref.close()
 
 
+@pytest.mark.parametrize("tile", [512, 1024])
+@pytest.mark.parametrize("thr", [0.0, 0.5])
+def test_tiles_equal_the_plain_filter_with_the_same_tile(ydata, tile, thr):
+    """A batch with tiles of 256 K particles is, filter by filter, the plain
+    filter at K particles per lane (the same tile)."""
+    rng = np.random.default_rng(13)
+    n, mu, phi, sig, seeds = filters(8, rng)
+    n[:4] = [256, tile + 1, 3 * tile - 5, 16384]   # partial, padded, ragged and largest filters
+    T = 45
+    y = ydata[:T]
+    say("tile", tile, thr, n.tolist())
+    b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, ess_threshold=thr,
+                                  block=256 if tile < 1024 else 128, tile=tile)
+    assert b.tile == tile
+    st = b.run(y)
+    counts = b.resample_counts()
+    for m in range(len(n)):
+        f = gpu_bpf.GpuBpf(int(n[m]), float(phi[m]), float(sig[m]), float(mu[m]), seed=int(seeds[m]),
+                           ess_threshold=thr, items_per_thread=tile // 256)
+        _, stp = f.run(y)
+        assert f.layout()[0] == tile
+        assert_same(st[:, m], stp)
+        np.testing.assert_array_equal(b.particles(m), f.particles())
+        assert counts[m] == f.resample_count()
+        f.close()
+    b.close()
+
+
+@pytest.mark.parametrize("block,tile", [(64, 512), (1024, 512), (64, 1024), (512, 1024)])
+def test_blocks_at_one_tile_give_identical_results(ydata, block, tile):
+    rng = np.random.default_rng(15)
+    n, mu, phi, sig, seeds = filters(7, rng)
+    y = ydata[:40]
+    say("block", block, "tile", tile)
+    ref = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, ess_threshold=0.5, block=256 if tile < 1024 else 128,
+                                    tile=tile)
+    sr = ref.run(y)
+    b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, ess_threshold=0.5, block=block, tile=tile)
+    sb = b.run(y)
+    for k in STAT_FIELDS + ("resampled",):
+        np.testing.assert_array_equal(sb[k], sr[k], err_msg=k)
+    for m in range(len(n)):
+        np.testing.assert_array_equal(b.particles(m), ref.particles(m))
+    b.close()
+    ref.close()
+
+
 def test_student_t_batch(ydata):
     rng = np.random.default_rng(7)
     n, mu, phi, sig, seeds = filters(5, rng, 300, 5000)
@@ -132,8 +179,16 @@ def test_autotune_picks_a_block(ydata):
     b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds)
     blk = b.autotune(ydata, 40)
     assert blk in (64, 128, 256, 512, 1024) and b.block == blk
+    assert b.tile in (256, 512, 1024)
+    say("autotuned", blk, b.tile)
     b.run(ydata[:10], stats=False)
     assert b.timestep() == 10
+    ll = b.loglik()
+    # the tuned pair is the same filters: equal to a batch set to that tile
+    ref = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, block=64 if blk != 64 else 128, tile=b.tile)
+    ref.run(ydata[:10], stats=False)
+    np.testing.assert_array_equal(ll, ref.loglik())
+    ref.close()
     b.close()
 
 
@@ -149,7 +204,7 @@ def test_autotune_with_score_keeps_the_score(ydata, estimator):
     blk = b.autotune(ydata, 20)
     assert blk in (64, 128, 256, 512, 1024) and b.block == blk
     b.run(ydata[:30], stats=False)
-    ref = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, block=64 if blk != 64 else 128)
+    ref = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, block=64 if blk != 64 else 128, tile=b.tile)
     ref.enable_score(estimator)
     ref.run(ydata[:30], stats=False)
     np.testing.assert_array_equal(b.score(), ref.score())
@@ -163,6 +218,13 @@ def test_batch_errors():
         gpu_bpf_batch.GpuBpfBatch([100, 20000], 0.0, 0.9, 0.1, [1, 2])
     with pytest.raises(gpu_bpf.GpuBpfError, match="set_block"):
         gpu_bpf_batch.GpuBpfBatch([100], 0.0, 0.9, 0.1, [1], block=96)
+    with pytest.raises(gpu_bpf.GpuBpfError, match="set_tile"):
+        gpu_bpf_batch.GpuBpfBatch([100], 0.0, 0.9, 0.1, [1], tile=300)
+    # a (block, tile) pair whose LDS does not fit one CU is refused at the run
+    b = gpu_bpf_batch.GpuBpfBatch([100], 0.0, 0.9, 0.1, [1], block=1024, tile=1024)
+    with pytest.raises(gpu_bpf.GpuBpfError):
+        b.run(np.zeros(3, np.float32), stats=False)
+    b.close()
 
 
 def test_batch_score_matches_adjoint_and_oracle(orc, ydata):

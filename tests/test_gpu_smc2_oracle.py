@@ -25,6 +25,11 @@ CASES = {
                                                 nu_obs=5.0)),
     "mala_gauss_m64": (64, 256, 2048, 100, dict(kind="mala", moves=2, outer_threshold=0.7, seed=11,
                                                 mala_step=0.3)),
+    # inner tiles of 512 and 1024 particles (the oracle's tiled weights with the same tile)
+    "cpmmh_tile512": (24, 128, 3000, 120, dict(kind="cpmmh", moves=3, outer_threshold=0.8, seed=13,
+                                               tile=512)),
+    "mala_student_t_tile1024": (16, 300, 4000, 100, dict(kind="mala", moves=2, outer_threshold=0.8,
+                                                         seed=17, nu_obs=5.0, tile=1024)),
 }
 
 
@@ -38,7 +43,8 @@ def test_smc2_equals_oracle(orc, name):
     nu = kw.get("nu_obs", 0.0)
     y = sv_data.sv_series(T, 0.97, 0.15, -1.0, nu_obs=nu, seed=21)[1].astype(np.float32)
     say("smc2 vs oracle", name)
-    g = gpu_bpf_batch.GpuSmc2(M, nmin, nmax, capacity=T, **kw)
+    tile = kw.get("tile", 256)
+    g = gpu_bpf_batch.GpuSmc2(M, nmin, nmax, capacity=T, block=256 if tile < 1024 else 128, **kw)
     g.run(y)
     th, lw, n = g.theta()
     ll = g.loglik()
