@@ -47,8 +47,8 @@ __device__ inline float uniform_from_half(float q) {
 }
 
 // exp(x) after S. L. Moshier's Cephes expf: Cody-Waite reduction by ln 2
-// (C1 = 0.693359375, C2 = -2.12194440e-4), degree-5 polynomial, exponent
-// added in the bits.  x < -86 gives 0 (no subnormal results), x > 88 gives
+// (C1 = 0.693359375, C2 = -2.12194440e-4), Cephes' degree-5 polynomial P,
+// exponent added in the bits.  x < -86 gives 0 (no subnormal results), x > 88 gives
 // +inf, NaN propagates.  The two selects at the end decide every x outside
 // [-86, 88], so the reduction runs unclamped: within the range the integer
 // part z = rint(x log2 e) is found by adding and subtracting 1.5 * 2^23
@@ -56,9 +56,12 @@ __device__ inline float uniform_from_half(float q) {
 // of the sum, and 2^z is applied by ldexp, exact for these normal results;
 // outside the range the discarded lanes compute finite or NaN values
 // without any undefined conversion, and a NaN x stays NaN through ldexp.
-__device__ inline float expf_cw(float x) {
+// The reduction's product and shift are one fused multiply-add (z is the
+// integer nearest the exact x log2 e), and the polynomial closes in Horner
+// form, 1 + r (1 + r P(r)).
+__device__ inline float expf_cw_core(float x, float& zs) {
     constexpr float kShift = 12582912.0f;        // 1.5 * 2^23
-    float zs = x * 1.44269504088896341f + kShift;
+    zs = __builtin_fmaf(x, 1.44269504088896341f, kShift);
     float z = zs - kShift;
     float r = __builtin_fmaf(z, -0.693359375f, x);
     r = __builtin_fmaf(z, 2.12194440e-4f, r);
@@ -68,10 +71,14 @@ __device__ inline float expf_cw(float x) {
     p = __builtin_fmaf(p, r, 4.1665795894e-2f);
     p = __builtin_fmaf(p, r, 1.6666665459e-1f);
     p = __builtin_fmaf(p, r, 5.0000001201e-1f);
-    float rr = r * r;
-    p = __builtin_fmaf(p, rr, r);
-    p = p + 1.0f;
-    float res = __builtin_amdgcn_ldexpf(p, (int)(f32_bits(zs) - 0x4B400000u));
+    p = __builtin_fmaf(p, r, 1.0f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    return __builtin_amdgcn_ldexpf(p, (int)(f32_bits(zs) - 0x4B400000u));
+}
+
+__device__ inline float expf_cw(float x) {
+    float zs;
+    float res = expf_cw_core(x, zs);
     res = (x > 88.0f) ? __builtin_inff() : res;
     res = (x < -86.0f) ? 0.0f : res;       // NaN fails both tests and stays NaN
     return res;
@@ -81,21 +88,8 @@ __device__ inline float expf_cw(float x) {
 // taken over the same values, or -inf, or NaN): the overflow select of
 // expf_cw can never fire, so the results are those of expf_cw.
 __device__ inline float expf_cw_nonpos(float x) {
-    constexpr float kShift = 12582912.0f;
-    float zs = x * 1.44269504088896341f + kShift;
-    float z = zs - kShift;
-    float r = __builtin_fmaf(z, -0.693359375f, x);
-    r = __builtin_fmaf(z, 2.12194440e-4f, r);
-    float p = 1.9875691500e-4f;
-    p = __builtin_fmaf(p, r, 1.3981999507e-3f);
-    p = __builtin_fmaf(p, r, 8.3334519073e-3f);
-    p = __builtin_fmaf(p, r, 4.1665795894e-2f);
-    p = __builtin_fmaf(p, r, 1.6666665459e-1f);
-    p = __builtin_fmaf(p, r, 5.0000001201e-1f);
-    float rr = r * r;
-    p = __builtin_fmaf(p, rr, r);
-    p = p + 1.0f;
-    float res = __builtin_amdgcn_ldexpf(p, (int)(f32_bits(zs) - 0x4B400000u));
+    float zs;
+    const float res = expf_cw_core(x, zs);
     return (x < -86.0f) ? 0.0f : res;
 }
 
@@ -152,6 +146,20 @@ __device__ inline float div_rn_normal(float n, float d) {
     return __builtin_fmaf(rem, y, q);
 }
 
+// n / d for the central inverse-CDF rational: the reciprocal refined once
+// and one Markstein correction.  It is the correctly rounded quotient for
+// every central draw the generator can produce (all 2^23 uniforms are
+// checked against IEEE division on the device); div_rn_normal adds the
+// second correction that makes it so for any operands in its range.
+__device__ inline float div_central(float n, float d) {
+    float y = __builtin_amdgcn_rcpf(d);
+    float e = __builtin_fmaf(-d, y, 1.0f);
+    y = __builtin_fmaf(e, y, y);
+    float q = n * y;
+    float rem = __builtin_fmaf(-d, q, n);
+    return __builtin_fmaf(rem, y, q);
+}
+
 // Acklam (2003) inverse normal CDF with the FP32 coefficients of ptx:172-203.
 // Central region |u - 1/2| < 0.47575 (0x3EF39581, ptx:169): rational in
 // r = q^2.  Valid for every q; callers select it only for central lanes.
@@ -168,7 +176,7 @@ __device__ inline float icdf_central(float q) {
     den = __builtin_fmaf(r, den, 66.80131188771972f);
     den = __builtin_fmaf(r, den, -13.28068155288572f);
     den = __builtin_fmaf(r, den, 1.0f);
-    return div_rn_normal(num, den);
+    return div_central(num, den);
 }
 
 __device__ inline bool icdf_is_central(float u) {

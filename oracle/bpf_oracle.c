@@ -102,12 +102,16 @@ float orc_uniform(uint32_t r) {
 /* ========================================================================= */
 
 /* exp(x) after S. L. Moshier's Cephes expf (Cody-Waite reduction with
- * C1 = 0.693359375, C2 = -2.12194440e-4, degree-5 polynomial).  Inputs below
- * -86 return 0 (no subnormal results), above 88 return +inf. */
+ * C1 = 0.693359375, C2 = -2.12194440e-4, Cephes' degree-5 polynomial P).
+ * z is the integer nearest the exact product x log2 e (one fused
+ * multiply-add onto 1.5 * 2^23), and the polynomial closes in Horner form,
+ * 1 + r (1 + r P(r)).  Inputs below -86 return 0 (no subnormal results),
+ * above 88 return +inf. */
 float orc_expf(float x) {
     if (!(x >= -86.0f)) return (x != x) ? x : 0.0f;
     if (x > 88.0f) return INFINITY;
-    float z = rintf(x * 1.44269504088896341f);
+    const float shift = 12582912.0f;
+    float z = fmaf(x, 1.44269504088896341f, shift) - shift;
     float r = fmaf(z, -0.693359375f, x);
     r = fmaf(z, 2.12194440e-4f, r);
     float p = 1.9875691500e-4f;
@@ -116,9 +120,8 @@ float orc_expf(float x) {
     p = fmaf(p, r, 4.1665795894e-2f);
     p = fmaf(p, r, 1.6666665459e-1f);
     p = fmaf(p, r, 5.0000001201e-1f);
-    float rr = r * r;
-    p = fmaf(p, rr, r);
-    p = p + 1.0f;
+    p = fmaf(p, r, 1.0f);
+    p = fmaf(p, r, 1.0f);
     int n = (int)z;
     return f32_bits(u32_of(p) + ((uint32_t)n << 23));
 }
