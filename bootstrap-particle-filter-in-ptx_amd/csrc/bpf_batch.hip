@@ -423,7 +423,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
     const float u = s_u;
     const int W = s_W;
     const float M = s_M;
-    const double fg = (M == NEG_INF) ? 0.0 : (double)expf_cw(M - M);
+    const double fg = (M == NEG_INF) ? 0.0 : (double)expf_cw_nonpos(M - M);
     double* cw = s_c[wid];
     float* xw = s_x[wid];
     const double rn = target_recip(n);
@@ -445,7 +445,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
 #pragma unroll
             for (int c = 0; c < KB; ++c) {
                 const int j = base + tile_item<KB>(q, c, lane);
-                e[q * KB + c] = (mb == NEG_INF || j >= n) ? 0.0f : expf_cw(lk[c] - mb);
+                e[q * KB + c] = (mb == NEG_INF || j >= n) ? 0.0f : expf_cw_nonpos(lk[c] - mb);
             }
         }
         double excl[4], tot4[4], cl[4];

@@ -948,7 +948,7 @@ __device__ inline TileScale tile_scale(const Partials& p, const double* gptil, f
     const int g = b / kGroup;
     const int l = b - g * kGroup;
     const float gm = p.gm[g];
-    const double fg = (gm == NEG_INF) ? 0.0 : (double)expf_cw(gm - M);
+    const double fg = (gm == NEG_INF) ? 0.0 : (double)expf_cw_nonpos(gm - M);
     const double rS = 1.0 / S;
     const double g0 = gptil[g], g1 = gptil[g + 1];
     const double pex = (l == 0) ? 0.0 : p.tp[b - 1];
@@ -1089,7 +1089,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(FISH ? 1 
                 const int k = v * VEC + c;
                 const int loc = (v * BLOCK + tid) * VEC + c;
                 bool valid = base + loc < n;
-                e[k] = (dead || !valid) ? 0.0f : expf_cw(lv[k] - mb);
+                e[k] = (dead || !valid) ? 0.0f : expf_cw_nonpos(lv[k] - mb);
                 s_x[pad16(loc)] = xv[k];
             }
         }
@@ -1391,7 +1391,7 @@ __global__ __launch_bounds__(kBlock) void k_fisher_score(FisherArgs a) {
     double acc[3] = {0.0, 0.0, 0.0};
     for (int b = tid; b < G; b += kBlock) {
         const float gm = a.part.gm[b / kGroup];
-        const double fg = (gm == NEG_INF) ? 0.0 : (double)expf_cw(gm - M);
+        const double fg = (gm == NEG_INF) ? 0.0 : (double)expf_cw_nonpos(gm - M);
         const double f = a.part.tf[b] * fg;
 #pragma unroll
         for (int c = 0; c < 3; ++c) acc[c] = fma(f, a.fsum[(size_t)c * G + b], acc[c]);
