@@ -125,9 +125,10 @@ __device__ inline void st_agent(T* p, T v) {
 // the group's sums.
 // ---------------------------------------------------------------------------
 
-// Returns the calling thread's tile scale f (0 past the last tile).
+// Returns the calling thread's tile scale f (0 past the last tile); thread 0
+// also receives the group record it stores (rec_m, rec_s).
 __device__ inline double group_fold(const Partials& p, int g, int tid, float* s_f, double* s_a,
-                                    double (*s_d)[kGroup / kWave]) {
+                                    double (*s_d)[kGroup / kWave], float& rec_m, double (&rec_s)[4]) {
     constexpr int NW = kGroup / kWave;
     const int lane = tid & (kWave - 1), wid = tid / kWave;
     const int t = g * kGroup + tid;
@@ -179,6 +180,9 @@ __device__ inline double group_fold(const Partials& p, int g, int tid, float* s_
         st_agent(p.gsx + g, r[1]);
         st_agent(p.gsxx + g, r[2]);
         st_agent(p.gsq + g, r[3]);
+        rec_m = gmax;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) rec_s[q] = r[q];
     }
     return f;
 }
@@ -641,12 +645,20 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
     __shared__ double s_S;
     __shared__ int s_last;
     __shared__ double s_l[4][NW];
+    __shared__ double s_glq[4];        // this block's group learning sums
 
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const int Gg = a.part.n_groups;
     constexpr float NEG_INF = -__builtin_inff();
 
-    const double ft = group_fold(a.part, blockIdx.x, tid, s_f, s_a, s_d);
+    // A filter of one reduction group (up to 256 tiles) is folded by this one
+    // block: it is its own last block, and thread 0 holds the group record,
+    // so the arrival counter and the record's round trip through memory are
+    // skipped (the record is still stored for the resample pass).
+    const bool single = gridDim.x == 1;
+    float rec_m = NEG_INF;
+    double rec_s[4] = {0.0, 0.0, 0.0, 0.0};
+    const double ft = group_fold(a.part, blockIdx.x, tid, s_f, s_a, s_d, rec_m, rec_s);
     if (a.lsum) {
         // the group's online-learning sums: tile sums scaled by the tile scales
         const int t = blockIdx.x * kGroup + tid;
@@ -663,24 +675,27 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
 #pragma unroll
             for (int w = 1; w < NW; ++w) acc += s_l[tid][w];
             st_agent(a.glsum + (size_t)tid * Gg + blockIdx.x, acc);
+            s_glq[tid] = acc;
         }
         __syncthreads();
     }
-    if (tid == 0) {
-        // the group record (and its learning sums, stored by lanes 1-3 of
-        // this wave) acknowledged before the group counts itself in; the
-        // signal fences keep the compiler from moving stores or the counter
-        // update across the wait
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        __builtin_amdgcn_s_waitcnt(0);
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        const int old = atomicAdd(a.part.gcnt, 1);
-        s_last = (old == (int)gridDim.x - 1);
+    if (!single) {
+        if (tid == 0) {
+            // the group record (and its learning sums, stored by lanes 1-3 of
+            // this wave) acknowledged before the group counts itself in; the
+            // signal fences keep the compiler from moving stores or the
+            // counter update across the wait
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __builtin_amdgcn_s_waitcnt(0);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            const int old = atomicAdd(a.part.gcnt, 1);
+            s_last = (old == (int)gridDim.x - 1);
+        }
+        __syncthreads();
+        if (!s_last) return;
+        // the records are read below at agent scope: no acquire fence
+        if (tid == 0) *a.part.gcnt = 0;
     }
-    __syncthreads();
-    if (!s_last) return;
-    // the records are read below at agent scope: no acquire fence
-    if (tid == 0) *a.part.gcnt = 0;
 
     // thread tid owns the contiguous groups [tid*C, tid*C + C), C <= kFinPer;
     // their records are read once
@@ -693,11 +708,19 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
     for (int j = 0; j < kFinPer; ++j) {
         const int g = g0 + j;
         const bool in = j < C && g < Gg;
-        gm[j] = in ? __hip_atomic_load(&a.part.gm[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : NEG_INF;
-        gs[j] = in ? __hip_atomic_load(&a.part.gs[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
-        gsx[j] = in ? __hip_atomic_load(&a.part.gsx[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
-        gsxx[j] = in ? __hip_atomic_load(&a.part.gsxx[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
-        gsq[j] = in ? __hip_atomic_load(&a.part.gsq[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+        if (single) {             // group 0 is thread 0's (C = 1)
+            gm[j] = in ? rec_m : NEG_INF;
+            gs[j] = in ? rec_s[0] : 0.0;
+            gsx[j] = in ? rec_s[1] : 0.0;
+            gsxx[j] = in ? rec_s[2] : 0.0;
+            gsq[j] = in ? rec_s[3] : 0.0;
+        } else {
+            gm[j] = in ? __hip_atomic_load(&a.part.gm[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : NEG_INF;
+            gs[j] = in ? __hip_atomic_load(&a.part.gs[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+            gsx[j] = in ? __hip_atomic_load(&a.part.gsx[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+            gsxx[j] = in ? __hip_atomic_load(&a.part.gsxx[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+            gsq[j] = in ? __hip_atomic_load(&a.part.gsq[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+        }
         m = fmaxf(m, gm[j]);
     }
     Ctl c;
@@ -742,8 +765,9 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
                 const double f = (gm[j] == NEG_INF) ? 0.0 : (double)expf_cw_nonpos(gm[j] - M);
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    Lq[q] = fma(f, __hip_atomic_load(&a.glsum[(size_t)q * Gg + g], __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT), Lq[q]);
+                    Lq[q] = fma(f, single ? s_glq[q]
+                                          : __hip_atomic_load(&a.glsum[(size_t)q * Gg + g], __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT), Lq[q]);
             }
         }
 #pragma unroll
