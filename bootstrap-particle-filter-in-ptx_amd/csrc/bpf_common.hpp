@@ -49,6 +49,28 @@ __device__ inline double dpp_f64(double v, double id) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// The same moves with nothing preset in the destination: lanes that the
+// masks disable keep whatever the register held (callers read none of
+// them), and with ZERO a lane whose source lies outside its row receives
+// +0.0 / 0.  They save the move of the identity that the forms above need.
+template <int CTRL, int ROWS = 0xf, int BANKS = 0xf, bool ZERO = false>
+__device__ inline int dpp_raw_i32(int v) {
+    return __builtin_amdgcn_mov_dpp(v, CTRL, ROWS, BANKS, ZERO);
+}
+
+template <int CTRL, int ROWS = 0xf, int BANKS = 0xf, bool ZERO = false>
+__device__ inline float dpp_raw_f32(float v) {
+    return __int_as_float(dpp_raw_i32<CTRL, ROWS, BANKS, ZERO>(__float_as_int(v)));
+}
+
+template <int CTRL, int ROWS = 0xf, int BANKS = 0xf, bool ZERO = false>
+__device__ inline double dpp_raw_f64(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)dpp_raw_i32<CTRL, ROWS, BANKS, ZERO>((int)(unsigned)b);
+    const unsigned hi = (unsigned)dpp_raw_i32<CTRL, ROWS, BANKS, ZERO>((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Orders the wave's own LDS writes before its later LDS reads.
 __device__ inline void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -80,62 +102,67 @@ __device__ inline double wave_last(double v) { return lane_f64(v, kWave - 1); }
 __device__ inline double wave_prev(double v, double fill) { return dpp_f64<dpp::kWaveShr1>(v, fill); }
 
 // Reductions: quads, half rows, rows, then rows 0 + 1 into row 1 and rows
-// 2 + 3 into row 3, and the two into lane 63, which every lane reads.
+// 2 + 3 into row 3, and the two into lane 63, which every lane reads.  The
+// quad and mirror moves give every lane a source; after the two row
+// broadcasts only rows 1 and 3 hold defined values, and only lane 63 is read.
 __device__ inline float wave_max(float v) {
-    constexpr float NI = -__builtin_inff();
-    v = fmaxf(v, dpp_f32<dpp::kQuad1032>(v, NI));
-    v = fmaxf(v, dpp_f32<dpp::kQuad2301>(v, NI));
-    v = fmaxf(v, dpp_f32<dpp::kRowHalfMirror>(v, NI));
-    v = fmaxf(v, dpp_f32<dpp::kRowMirror>(v, NI));
-    v = fmaxf(v, dpp_f32<dpp::kRowBcast15, 0xa>(v, NI));
-    v = fmaxf(v, dpp_f32<dpp::kRowBcast31, 0xc>(v, NI));
+    v = fmaxf(v, dpp_raw_f32<dpp::kQuad1032>(v));
+    v = fmaxf(v, dpp_raw_f32<dpp::kQuad2301>(v));
+    v = fmaxf(v, dpp_raw_f32<dpp::kRowHalfMirror>(v));
+    v = fmaxf(v, dpp_raw_f32<dpp::kRowMirror>(v));
+    v = fmaxf(v, dpp_raw_f32<dpp::kRowBcast15, 0xa>(v));
+    v = fmaxf(v, dpp_raw_f32<dpp::kRowBcast31, 0xc>(v));
     return lane_f32(v, kWave - 1);
 }
 
 __device__ inline double wave_sum(double v) {
-    v += dpp_f64<dpp::kQuad1032>(v, 0.0);
-    v += dpp_f64<dpp::kQuad2301>(v, 0.0);
-    v += dpp_f64<dpp::kRowHalfMirror>(v, 0.0);
-    v += dpp_f64<dpp::kRowMirror>(v, 0.0);
-    v += dpp_f64<dpp::kRowBcast15, 0xa>(v, 0.0);
-    v += dpp_f64<dpp::kRowBcast31, 0xc>(v, 0.0);
+    v += dpp_raw_f64<dpp::kQuad1032>(v);
+    v += dpp_raw_f64<dpp::kQuad2301>(v);
+    v += dpp_raw_f64<dpp::kRowHalfMirror>(v);
+    v += dpp_raw_f64<dpp::kRowMirror>(v);
+    v += dpp_raw_f64<dpp::kRowBcast15, 0xa>(v);
+    v += dpp_raw_f64<dpp::kRowBcast31, 0xc>(v);
     return wave_last(v);
 }
 
 // Inclusive scans: each lane adds its three row predecessors, then the
-// partial sums 4 and 8 lanes back (banks 1-3, then banks 2-3), then the
-// totals of the rows before (row broadcasts of lanes 15 and 31).
+// partial sums 4 and 8 lanes back, then the totals of the rows before (row
+// broadcasts of lanes 15 and 31).  A lane with no predecessor that far back
+// in its row adds +0 (the row shifts' zero fill); rows 0 and 2, and rows 0
+// and 1, add +0 in the two broadcasts.
 __device__ inline double wave_incl_scan(double v, int /*lane*/) {
-    double s = v + dpp_f64<dpp::kRowShr1>(v, 0.0);
-    s += dpp_f64<dpp::kRowShr2>(v, 0.0);
-    s += dpp_f64<dpp::kRowShr3>(v, 0.0);
-    s += dpp_f64<dpp::kRowShr4, 0xf, 0xe>(s, 0.0);
-    s += dpp_f64<dpp::kRowShr8, 0xf, 0xc>(s, 0.0);
+    double s = v + dpp_raw_f64<dpp::kRowShr1, 0xf, 0xf, true>(v);
+    s += dpp_raw_f64<dpp::kRowShr2, 0xf, 0xf, true>(v);
+    s += dpp_raw_f64<dpp::kRowShr3, 0xf, 0xf, true>(v);
+    s += dpp_raw_f64<dpp::kRowShr4, 0xf, 0xf, true>(s);
+    s += dpp_raw_f64<dpp::kRowShr8, 0xf, 0xf, true>(s);
     s += dpp_f64<dpp::kRowBcast15, 0xa>(s, 0.0);
     s += dpp_f64<dpp::kRowBcast31, 0xc>(s, 0.0);
     return s;
 }
 
 __device__ inline int wave_incl_scan(int v, int /*lane*/) {
-    int s = v + dpp_i32<dpp::kRowShr1>(v, 0);
-    s += dpp_i32<dpp::kRowShr2>(v, 0);
-    s += dpp_i32<dpp::kRowShr3>(v, 0);
-    s += dpp_i32<dpp::kRowShr4, 0xf, 0xe>(s, 0);
-    s += dpp_i32<dpp::kRowShr8, 0xf, 0xc>(s, 0);
+    int s = v + dpp_raw_i32<dpp::kRowShr1, 0xf, 0xf, true>(v);
+    s += dpp_raw_i32<dpp::kRowShr2, 0xf, 0xf, true>(v);
+    s += dpp_raw_i32<dpp::kRowShr3, 0xf, 0xf, true>(v);
+    s += dpp_raw_i32<dpp::kRowShr4, 0xf, 0xf, true>(s);
+    s += dpp_raw_i32<dpp::kRowShr8, 0xf, 0xf, true>(s);
     s += dpp_i32<dpp::kRowBcast15, 0xa>(s, 0);
     s += dpp_i32<dpp::kRowBcast31, 0xc>(s, 0);
     return s;
 }
 
+// Inclusive prefix maximum of non-negative values (CDF values and prefix
+// sums of weights), in the scan's order; +0 stands in for a missing
+// predecessor, which leaves any value >= +0 unchanged.
 __device__ inline double wave_incl_max(double v, int /*lane*/) {
-    constexpr double NI = -__builtin_inf();
-    double s = fmax(v, dpp_f64<dpp::kRowShr1>(v, NI));
-    s = fmax(s, dpp_f64<dpp::kRowShr2>(v, NI));
-    s = fmax(s, dpp_f64<dpp::kRowShr3>(v, NI));
-    s = fmax(s, dpp_f64<dpp::kRowShr4, 0xf, 0xe>(s, NI));
-    s = fmax(s, dpp_f64<dpp::kRowShr8, 0xf, 0xc>(s, NI));
-    s = fmax(s, dpp_f64<dpp::kRowBcast15, 0xa>(s, NI));
-    s = fmax(s, dpp_f64<dpp::kRowBcast31, 0xc>(s, NI));
+    double s = fmax(v, dpp_raw_f64<dpp::kRowShr1, 0xf, 0xf, true>(v));
+    s = fmax(s, dpp_raw_f64<dpp::kRowShr2, 0xf, 0xf, true>(v));
+    s = fmax(s, dpp_raw_f64<dpp::kRowShr3, 0xf, 0xf, true>(v));
+    s = fmax(s, dpp_raw_f64<dpp::kRowShr4, 0xf, 0xf, true>(s));
+    s = fmax(s, dpp_raw_f64<dpp::kRowShr8, 0xf, 0xf, true>(s));
+    s = fmax(s, dpp_f64<dpp::kRowBcast15, 0xa>(s, 0.0));
+    s = fmax(s, dpp_f64<dpp::kRowBcast31, 0xc>(s, 0.0));
     return s;
 }
 
