@@ -309,13 +309,13 @@ def batch_only(args):
         yy = yy.astype(np.float32)
         n = log_uniform_counts(rng, M, lo, hi)
         mu, phi, sig, seeds = prior_draws(rng, M)
-        b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=nu)
+        b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=nu, stats=False)
         if score:
             b.enable_score()         # the autotuner times the kernel that will run
         blk = b.autotune(yy, 64)
         tile = b.tile
         b.close()
-        b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=nu, block=blk, tile=tile)
+        b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=nu, block=blk, tile=tile, stats=False)
         if score:
             b.enable_score()
         b.run(yy[:32], stats=False)
@@ -338,22 +338,29 @@ def batch_and_smc2(args, gpu_bpf, gpu_bpf_batch, torch, y, n, mu, phi, sig, seed
     SMC^2 run, and the inner filters' log-likelihood error on a sample."""
     score = kind == "mala"
     M = len(n)
-    b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=nu)
+    # the inner filters as SMC^2 runs them: no per-tick mean, variance or ESS
+    b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=nu, stats=False)
     if score:
         b.enable_score()             # the autotuner times the kernel that will run
     blk = b.autotune(y, 64)
     tile = b.tile
     b.close()
-    b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=nu, block=blk, tile=tile)
-    if score:
-        b.enable_score()
-    b.run(y[:32], stats=False)
-    b.synchronize()
-    t0 = time.perf_counter()
-    b.run(y[32:32 + ticks_batch], stats=False)
-    b.synchronize()
-    batch_pt = float(n.sum()) * ticks_batch / (time.perf_counter() - t0)
-    b.close()
+
+    def batch_rate(stats):
+        b = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=nu, block=blk, tile=tile, stats=stats)
+        if score:
+            b.enable_score()
+        b.run(y[:32], stats=False)
+        b.synchronize()
+        t0 = time.perf_counter()
+        b.run(y[32:32 + ticks_batch], stats=False)
+        b.synchronize()
+        rate = float(n.sum()) * ticks_batch / (time.perf_counter() - t0)
+        b.close()
+        return rate
+
+    batch_pt = batch_rate(False)
+    batch_stats_pt = batch_rate(True)
     k = 16 if score else 64
     loop_pt = looped_plain(gpu_bpf, n[:k], mu, phi, sig, seeds, y, 100 if score else 40, nu=nu,
                            score=score)
@@ -363,6 +370,8 @@ def batch_and_smc2(args, gpu_bpf, gpu_bpf_batch, torch, y, n, mu, phi, sig, seed
         ("looped_plain_score_particle_ticks_per_s" if score else
          "looped_plain_particle_ticks_per_s"): loop_pt,
         "batch_over_looped": batch_pt / loop_pt,
+        ("inner_batch_with_score_and_stats_particle_ticks_per_s" if score else
+         "inner_batch_with_stats_particle_ticks_per_s"): batch_stats_pt,
     }
     if not score:
         strm = looped_plain_streams(gpu_bpf, n[:M], mu, phi, sig, seeds, y, 200, nu=nu)

@@ -37,6 +37,8 @@ struct GpuBpfBatch {
     float* d_l = nullptr;
     BpfTickStats* d_ring = nullptr;
     int ring_cap = 1024;
+    int stats = 1;                     // per-tick statistics recorded in the ring
+    int stats_from = 0;                // first tick of the current recording span
     float* d_y = nullptr;              // observation history, indexed by tick
     int y_cap = 0;
     int timestep = 0;
@@ -45,7 +47,7 @@ struct GpuBpfBatch {
     int fisher = 0;                    // score estimator: 0 pathwise, 1 Fisher identity
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    int graph_block = 0, graph_tile = 0;
+    int graph_block = 0, graph_tile = 0, graph_stats = 1;
 };
 
 

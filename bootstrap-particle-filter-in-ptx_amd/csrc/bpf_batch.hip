@@ -16,6 +16,8 @@
 // Crank-Nicolson step around the store (correlated pseudo-marginal moves).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "bpf_common.hpp"
 
 namespace bpf {
@@ -221,6 +223,11 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
     }
     const int zc = (a.noise >= kNoisePcgStore) ? a.zcur[m] : 0;
     const size_t zoff = ((size_t)tick * a.n_slots + m) * a.cap;
+    // The mean and variance are formed only for the statistics ring, and the
+    // sum of squared weights only for it or an ESS threshold; the
+    // log-likelihood, the decision and the CDF need the weight sum alone.
+    const bool moments = a.ring != nullptr;
+    const bool squares = moments || a.threshold > 0.0f;
 
     // Propagate, log-weights (carry as the single filter), tile maxima and
     // the tile sums of the single filter's propagate pass: the lane's KB
@@ -292,36 +299,48 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
         for (int k = 0; k < NS; ++k) acc[k] = 0.0;
         double ls[3] = {0.0, 0.0, 0.0};    // lane sums of the score terms
         if (mt != NEG_INF) {
+            // MOMENTS: the sums of the mean and variance; SQUARES: the sum of
+            // squared weights (ESS).  Sums that are not needed stay 0.
+            auto tile_sums = [&](auto moments_c, auto squares_c) {
+                constexpr bool MOMENTS = decltype(moments_c)::value;
+                constexpr bool SQUARES = decltype(squares_c)::value;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                double S = 0.0, SX = 0.0, SXX = 0.0, Q = 0.0;
+                for (int q = 0; q < 4; ++q) {
+                    double S = 0.0, SX = 0.0, SXX = 0.0, Q = 0.0;
 #pragma unroll
-                for (int c = 0; c < KB; ++c) {
-                    const int p = q * KB + c;
-                    const int j = base + tile_item<KB>(q, c, lane);
-                    const double de = (double)expf_cw_nonpos(l[p] - mt);
-                    const double dx = (j < n) ? (double)x[p] : 0.0;
-                    const double t = de * dx;
-                    S += de;
-                    SX += t;
-                    SXX = fma(t, dx, SXX);
-                    Q = fma(de, de, Q);
-                    if (TAN) {
-                        const double eg = (j < n) ? (FISHER ? de : de * (double)dlw_dx<STUDENT>(x[p], obs)) : 0.0;
+                    for (int c = 0; c < KB; ++c) {
+                        const int p = q * KB + c;
+                        const int j = base + tile_item<KB>(q, c, lane);
+                        const double de = (double)expf_cw_nonpos(l[p] - mt);
+                        S += de;
+                        if (MOMENTS) {
+                            const double dx = (j < n) ? (double)x[p] : 0.0;
+                            const double t = de * dx;
+                            SX += t;
+                            SXX = fma(t, dx, SXX);
+                        }
+                        if (SQUARES) Q = fma(de, de, Q);
+                        if (TAN) {
+                            const double eg =
+                                (j < n) ? (FISHER ? de : de * (double)dlw_dx<STUDENT>(x[p], obs)) : 0.0;
 #pragma unroll
-                        for (int k = 0; k < 3; ++k) ls[k] = fma(eg, (double)d[k][p], ls[k]);
+                            for (int k = 0; k < 3; ++k) ls[k] = fma(eg, (double)d[k][p], ls[k]);
+                        }
+                    }
+                    const double ws = wave_sum(S);
+                    const double wx = MOMENTS ? wave_sum(SX) : 0.0;
+                    const double wxx = MOMENTS ? wave_sum(SXX) : 0.0;
+                    const double wq = SQUARES ? wave_sum(Q) : 0.0;
+                    if (q == 0) {
+                        acc[0] = ws; acc[1] = wx; acc[2] = wxx; acc[3] = wq;
+                    } else {
+                        acc[0] += ws; acc[1] += wx; acc[2] += wxx; acc[3] += wq;
                     }
                 }
-                const double ws = wave_sum(S);
-                const double wx = wave_sum(SX);
-                const double wxx = wave_sum(SXX);
-                const double wq = wave_sum(Q);
-                if (q == 0) {
-                    acc[0] = ws; acc[1] = wx; acc[2] = wxx; acc[3] = wq;
-                } else {
-                    acc[0] += ws; acc[1] += wx; acc[2] += wxx; acc[3] += wq;
-                }
-            }
+            };
+            if (moments) tile_sums(std::true_type(), std::true_type());
+            else if (squares) tile_sums(std::false_type(), std::true_type());
+            else tile_sums(std::false_type(), std::false_type());
             if (TAN) {
 #pragma unroll
                 for (int k = 0; k < 3; ++k) acc[4 + k] = wave_sum(ls[k]);
@@ -357,9 +376,9 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
             s_tp[lane] = 0.0 + pin;
         }
         vs = wave_sum(vs);
-        vx = wave_sum(vx);
-        vxx = wave_sum(vxx);
-        vq = wave_sum(vq);
+        vx = moments ? wave_sum(vx) : 0.0;
+        vxx = moments ? wave_sum(vxx) : 0.0;
+        vq = squares ? wave_sum(vq) : 0.0;
         if (TAN) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {

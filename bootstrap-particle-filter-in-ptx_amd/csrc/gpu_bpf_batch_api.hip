@@ -72,7 +72,7 @@ BatchArgs tick_args(const GpuBpfBatch* b) {
     a.x0 = b->d_x[0];
     a.x1 = b->d_x[1];
     a.lw = b->d_l;
-    a.ring = b->d_ring;
+    a.ring = b->stats ? b->d_ring : nullptr;   // no ring: no mean, variance or ESS
     a.ring_cap = b->ring_cap;
     a.cap = b->cap;
     a.n_slots = b->M;
@@ -211,6 +211,7 @@ static int capture_graph(GpuBpfBatch* b) {
     HIP_TRY(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
     b->graph_block = b->block;
     b->graph_tile = b->tile;
+    b->graph_stats = b->stats;
     return 0;
 }
 
@@ -222,7 +223,8 @@ int run_ticks(GpuBpfBatch* b, int n_ticks) {
                 b->tile);
         return -1;
     }
-    if (n_ticks >= kGraphTicks && (!b->graph_exec || b->graph_block != b->block || b->graph_tile != b->tile)) {
+    if (n_ticks >= kGraphTicks && (!b->graph_exec || b->graph_block != b->block || b->graph_tile != b->tile ||
+                                   b->graph_stats != b->stats)) {
         if (capture_graph(b)) return -1;
     }
     int done = 0;
@@ -318,6 +320,16 @@ int gpu_bpf_batch_set_tile(GpuBpfBatch* b, int particles) {
 
 int gpu_bpf_batch_get_tile(GpuBpfBatch* b) { return b ? (b->tile ? b->tile : kBatchTile) : -1; }
 
+int gpu_bpf_batch_set_stats(GpuBpfBatch* b, int on) {
+    if (!b || (on != 0 && on != 1)) {
+        set_err("gpu_bpf_batch_set_stats: a batch and 0 or 1");
+        return -1;
+    }
+    if (on && !b->stats) b->stats_from = b->timestep;
+    b->stats = on;
+    return 0;
+}
+
 int gpu_bpf_batch_autotune(GpuBpfBatch* b, const float* y, int ticks) {
     // two untimed ticks, then ticks - 2 timed ones: y holds `ticks` values
     if (!b || !y || ticks < 3) {
@@ -340,6 +352,7 @@ int gpu_bpf_batch_autotune(GpuBpfBatch* b, const float* y, int ticks) {
             }
             s->block = blk;
             s->tile = tile;
+            s->stats = b->stats;
             hipEvent_t e0 = nullptr, e1 = nullptr;
             float ms = 0.0f;
             int rc = -1;
@@ -404,9 +417,10 @@ int gpu_bpf_batch_copy_stats(GpuBpfBatch* b, BpfTickStats* out, int n_ticks) {
         set_err("gpu_bpf_batch_copy_stats: bad arguments");
         return -1;
     }
-    if (n_ticks > b->timestep || n_ticks > b->ring_cap) {
-        set_err("gpu_bpf_batch_copy_stats: %d ticks requested, %d available", n_ticks,
-                std::min(b->timestep, b->ring_cap));
+    const int avail = b->stats ? std::min(b->timestep - b->stats_from, b->ring_cap) : 0;
+    if (n_ticks > avail) {
+        set_err("gpu_bpf_batch_copy_stats: %d ticks requested, %d available%s", n_ticks, avail,
+                b->stats ? "" : " (statistics are off)");
         return -1;
     }
     HIP_TRY(hipStreamSynchronize(b->stream));

@@ -755,6 +755,10 @@ int gpu_smc2_init_from_prior(GpuSmc2* s) {
     s->main->n = n;
     s->rep = new_batch(M, ncap.data(), mu.data(), phi.data(), sig.data(), seeds.data(), s->nu_obs, 0.0f);
     s->rep->n = n;
+    // the outer layer reads the inner filters' log-likelihoods only: no
+    // per-tick mean, variance or ESS
+    s->main->stats = 0;
+    s->rep->stats = 0;
     if (create_impl(s->main) || create_impl(s->rep)) return -1;
     if (stores_noise(s)) {
         if (s->t_cap <= 0) s->t_cap = 2048;

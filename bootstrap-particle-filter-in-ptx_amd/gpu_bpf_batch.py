@@ -29,6 +29,7 @@ def _lib():
             "gpu_bpf_batch_set_block": (I, [P, I]),
             "gpu_bpf_batch_get_block": (I, [P]),
             "gpu_bpf_batch_set_tile": (I, [P, I]),
+            "gpu_bpf_batch_set_stats": (I, [P, I]),
             "gpu_bpf_batch_get_tile": (I, [P]),
             "gpu_bpf_batch_autotune": (I, [P, P, I]),
             "gpu_bpf_batch_run": (I, [P, P, I]),
@@ -58,7 +59,7 @@ class GpuBpfBatch:
     """M inner filters; per-filter arrays n, mu, phi, sigma_z, seeds."""
 
     def __init__(self, n, mu, phi, sigma_z, seeds, nu_obs=0.0, ess_threshold=0.0, block=None,
-                 tile=None):
+                 tile=None, stats=True):
         self.L = _lib()
         self.n = np.ascontiguousarray(n, np.int32)
         self.M = int(self.n.size)
@@ -75,6 +76,8 @@ class GpuBpfBatch:
             self._check(self.L.gpu_bpf_batch_set_block(self.h, int(block)))
         if tile is not None:
             self._check(self.L.gpu_bpf_batch_set_tile(self.h, int(tile)))
+        if not stats:
+            self.set_stats(False)
 
     def _check(self, rc):
         if rc < 0:
@@ -99,6 +102,11 @@ class GpuBpfBatch:
 
     def set_tile(self, particles):
         self._check(self.L.gpu_bpf_batch_set_tile(self.h, int(particles)))
+
+    def set_stats(self, on):
+        """Per-tick statistics (mean, variance, ESS, the ring) on or off; off,
+        the ticks form only the weight sums the filters need."""
+        self._check(self.L.gpu_bpf_batch_set_stats(self.h, 1 if on else 0))
 
     def autotune(self, y, ticks=64):
         """Times every (block, tile) pair that fits; keeps the fastest and

@@ -42,6 +42,14 @@ int gpu_bpf_batch_get_block(GpuBpfBatch* batch);
  * (gpu_bpf_set_items_per_thread(tile / 256)).  Returns 0 or -1. */
 int gpu_bpf_batch_set_tile(GpuBpfBatch* batch, int particles);
 int gpu_bpf_batch_get_tile(GpuBpfBatch* batch);
+/* Per-tick statistics on (1, the default) or off (0).  Off, the ticks form
+ * only what the filters need -- the weight sum for the log-likelihood and the
+ * CDF, and the sum of squared weights when an ESS threshold is set -- and
+ * record nothing in the ring; log-likelihoods, resampling, particles and
+ * scores are unchanged bit for bit.  Statistics are available again for the
+ * ticks run after they are switched back on.  No counterpart in the
+ * reference; SMC^2 runs its inner filters with them off.  Returns 0 or -1. */
+int gpu_bpf_batch_set_stats(GpuBpfBatch* batch, int on);
 /* Times every (block, tile) pair whose LDS fits one CU on a scratch batch of
  * the same filters, with the same score estimator when this batch's score
  * is enabled (enable it first): two untimed ticks on y[0..1], then

@@ -213,6 +213,47 @@ def test_autotune_with_score_keeps_the_score(ydata, estimator):
     ref.close()
 
 
+@pytest.mark.parametrize("case", ["gauss", "ess_threshold", "student_t_score", "fisher_tile512"])
+def test_statistics_off_leaves_the_filters_unchanged(ydata, case):
+    """With the per-tick statistics off, the ticks skip the mean, variance and
+    (without an ESS threshold) squared-weight sums; log-likelihoods, resample
+    counts, particles and scores equal a batch with them on, bit for bit, over
+    a captured 32-tick graph and single launches.  Statistics switched back on
+    cover the ticks run since, and equal the other batch's."""
+    rng = np.random.default_rng(21)
+    n, mu, phi, sig, seeds = filters(12, rng, 256, 6000)
+    nu = 5.0 if case == "student_t_score" else 0.0
+    thr = 0.5 if case == "ess_threshold" else 0.0
+    tile = 512 if case == "fisher_tile512" else None
+    say("statistics off", case)
+    on = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=nu, ess_threshold=thr, block=512, tile=tile)
+    off = gpu_bpf_batch.GpuBpfBatch(n, mu, phi, sig, seeds, nu_obs=nu, ess_threshold=thr, block=512, tile=tile,
+                                    stats=False)
+    if case in ("student_t_score", "fisher_tile512"):
+        est = "fisher" if case == "fisher_tile512" else "pathwise"
+        on.enable_score(est)
+        off.enable_score(est)
+    on.run(ydata[:40], stats=False)
+    off.run(ydata[:40], stats=False)
+    np.testing.assert_array_equal(off.loglik(), on.loglik())
+    np.testing.assert_array_equal(off.resample_counts(), on.resample_counts())
+    for m in range(len(n)):
+        np.testing.assert_array_equal(off.particles(m), on.particles(m))
+    if case in ("student_t_score", "fisher_tile512"):
+        np.testing.assert_array_equal(off.score(), on.score())
+    with pytest.raises(gpu_bpf.GpuBpfError, match="statistics are off"):
+        off.stats(1)
+    off.set_stats(True)
+    st_on = on.run(ydata[40:45])
+    st_off = off.run(ydata[40:45])
+    for k in st_on.dtype.names:
+        np.testing.assert_array_equal(st_off[k], st_on[k], err_msg=k)
+    with pytest.raises(gpu_bpf.GpuBpfError, match="available"):
+        off.stats(6)
+    on.close()
+    off.close()
+
+
 def test_batch_errors():
     with pytest.raises(gpu_bpf.GpuBpfError, match="particles"):
         gpu_bpf_batch.GpuBpfBatch([100, 20000], 0.0, 0.9, 0.1, [1, 2])
