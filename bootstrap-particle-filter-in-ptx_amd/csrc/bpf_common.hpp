@@ -310,12 +310,25 @@ __device__ inline void finalize_tick(Ctl& c, int n_particles, float threshold, f
     BpfTickStats r;
     r.tick = c.tick;
     const bool ok = isfinite(S) && S > 0.0 && isfinite(Md);
+    const double ess = (S * S) / Q;
+    const int resample = ok && (threshold <= 0.0f || (ess / n) < (double)threshold);
+    // log(S): the increment of a tick whose weights carried in, and the next
+    // tick's when they carry on (kept in the control block, so each tick
+    // takes one log of S rather than logs of S and of the previous S)
+    const double log_s = (carry || !resample) ? log(S) : 0.0;
     if (ok) {
         r.mean = SX / S;
         r.var = SXX / S - r.mean * r.mean;
-        r.ess = (S * S) / Q;
-        r.ll_ref = c.off + Md + log(fmax(S / n, (double)bits_f32(0x0D6BF94Eu)));
-        r.ll_incr = carry ? (Md + log(S)) - log(c.s_prev) : Md + log(S / n);
+        r.ess = ess;
+        // one log of S / n serves both log-likelihoods
+        const double sn = S / n;
+        const double lsn = log(sn);
+        const double fl = (double)bits_f32(0x0D6BF94Eu);
+        double lref;
+        if (sn >= fl) lref = lsn;
+        else lref = log(fl);
+        r.ll_ref = c.off + Md + lref;
+        r.ll_incr = carry ? (Md + log_s) - c.log_s_prev : Md + lsn;
     } else {
         const double qn = __builtin_nan("");
         r.mean = r.var = r.ess = r.ll_ref = r.ll_incr = qn;
@@ -324,7 +337,6 @@ __device__ inline void finalize_tick(Ctl& c, int n_particles, float threshold, f
     r.ll_cum = c.ll_cum;
     r.h_mean = (float)r.mean;
     r.log_lik = (float)r.ll_ref;
-    const int resample = ok && (threshold <= 0.0f || (r.ess / n) < (double)threshold);
     r.resampled = resample;
     if (resample) {
         const float ur = (u_raw_in >= 0.0f) ? u_raw_in : host_pcg_float(&c.host_rng);
@@ -342,6 +354,7 @@ __device__ inline void finalize_tick(Ctl& c, int n_particles, float threshold, f
         c.carry = 1;
         c.off += Md;
         c.s_prev = S;
+        c.log_s_prev = log_s;
     }
     c.m_prev = M;
     c.M = M;

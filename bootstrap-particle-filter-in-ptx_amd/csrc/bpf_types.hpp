@@ -40,6 +40,7 @@ struct Ctl {
     float u_raw;         // uniform behind u (u = u_raw / N)
     double S;            // sum exp(l - M) of the latest tick
     double s_prev;       // sum exp(l - m_prev) of the carried weights
+    double log_s_prev;   // log(s_prev) (0 = log 1 at the start)
     double off;          // raw log-weight offset (reference log_lik semantics)
     double ll_cum;       // FP64 cumulative log-likelihood
     unsigned long long host_rng;   // host-side PCG32 stream of the reference
