@@ -5,11 +5,11 @@
 // tick of slot order[g] (propagate, weights, FP64 statistics and
 // log-likelihood, the resampling decision, FP64 CDF and systematic
 // resampling), so filters of uneven size need no grid-wide step.  Each wave
-// takes whole 256-particle tiles, four particles per lane.  The arithmetic
-// restates the single filter's kernels at one particle per lane -- 64-particle
-// group q of a tile plays the single filter's wave q -- so an inner filter
-// with the same parameters, particle count and seed produces the single
-// filter's particles, ancestors and statistics.
+// takes whole tiles of 256 KB particles (KB = 1, 2 or 4), 4 KB per lane.  The
+// arithmetic restates the single filter's kernels at KB particles per lane --
+// lane group q (64 lanes x KB particles) of a tile plays the single filter's
+// wave q -- so an inner filter with the same parameters, particle count, seed
+// and tile produces the single filter's particles, ancestors and statistics.
 //
 // Noise: the PCG32 streams of the single filter (per slot seed), optionally
 // stored for later replays, or read back from the store, or proposed by a
