@@ -138,7 +138,7 @@ int create_impl(GpuBpfBatch* b) {
     HIP_TRY(hipMalloc(&b->d_x[0], slots * sizeof(float)));
     HIP_TRY(hipMalloc(&b->d_x[1], slots * sizeof(float)));
     HIP_TRY(hipMalloc(&b->d_l, slots * sizeof(float)));
-    HIP_TRY(hipMalloc(&b->d_ring, (size_t)b->ring_cap * M * sizeof(BpfTickStats)));
+    if (b->stats) HIP_TRY(hipMalloc(&b->d_ring, (size_t)b->ring_cap * M * sizeof(BpfTickStats)));
     HIP_TRY(hipMemsetAsync(b->d_x[1], 0, slots * sizeof(float), b->stream));
     HIP_TRY(hipMemsetAsync(b->d_l, 0, slots * sizeof(float), b->stream));
     HIP_TRY(hipMemcpyAsync(b->d_order, b->order.data(), M * sizeof(int), hipMemcpyHostToDevice, b->stream));
@@ -325,6 +325,7 @@ int gpu_bpf_batch_set_stats(GpuBpfBatch* b, int on) {
         set_err("gpu_bpf_batch_set_stats: a batch and 0 or 1");
         return -1;
     }
+    if (on && !b->d_ring) HIP_TRY(hipMalloc(&b->d_ring, (size_t)b->ring_cap * b->M * sizeof(BpfTickStats)));
     if (on && !b->stats) b->stats_from = b->timestep;
     b->stats = on;
     return 0;

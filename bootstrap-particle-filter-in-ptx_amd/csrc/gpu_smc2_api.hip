@@ -756,7 +756,7 @@ int gpu_smc2_init_from_prior(GpuSmc2* s) {
     s->rep = new_batch(M, ncap.data(), mu.data(), phi.data(), sig.data(), seeds.data(), s->nu_obs, 0.0f);
     s->rep->n = n;
     // the outer layer reads the inner filters' log-likelihoods only: no
-    // per-tick mean, variance or ESS
+    // per-tick mean, variance or ESS, and no statistics ring
     s->main->stats = 0;
     s->rep->stats = 0;
     if (create_impl(s->main) || create_impl(s->rep)) return -1;
