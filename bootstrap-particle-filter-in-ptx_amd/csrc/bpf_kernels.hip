@@ -193,8 +193,8 @@ __device__ inline double group_fold(const Partials& p, int g, int tid, float* s_
 // groups at (v*kBlock + tid)*VEC, so every load and store instruction of a
 // wave covers one contiguous 64*VEC*4-byte run.
 // ICDF tails (|u-1/2| >= 0.47575, 4.85% of draws) would make nearly every
-// wave execute the ~60-instruction tail branch; instead tail uniforms are
-// compacted into an LDS queue and evaluated densely by the whole block.
+// wave execute the ~60-instruction tail branch; instead each wave compacts its
+// tail uniforms into its own LDS queue and evaluates them densely.
 // ---------------------------------------------------------------------------
 
 // Log-weights of the lane's K particles (Gaussian or Student-t chosen once
@@ -678,6 +678,28 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
             s_glq[tid] = acc;
         }
         __syncthreads();
+    }
+    if (single && !a.lsum) {
+        // One group: the global fold below reduces to the group record itself
+        // (its scale exp(gm - M) is exactly 1, every other term +0, the group
+        // prefix is {0, 1}), so thread 0 applies it directly; adding +0 keeps
+        // the fold's one effect on the record, -0 becoming +0.
+        if (tid == 0) {
+            Ctl c = *a.ctl;
+            finalize_tick(c, a.n, a.threshold, rec_m, rec_s[0] + 0.0, rec_s[1] + 0.0, rec_s[2] + 0.0,
+                          rec_s[3] + 0.0);
+            *a.ctl = c;
+            a.ring[c.last.tick % a.ring_cap] = c.last;
+            if (c.resample) {
+                if (a.u_log) *a.u_log = c.u;
+                if (a.w_log) *a.w_log = c.wrap_start;
+                a.gptil[0] = 0.0;
+                a.gptil[1] = 1.0;
+#pragma unroll
+                for (int r = 0; r < kClaimRanges; ++r) a.tile_ctr[r * kClaimStride] = 0;
+            }
+        }
+        return;
     }
     if (!single) {
         if (tid == 0) {
