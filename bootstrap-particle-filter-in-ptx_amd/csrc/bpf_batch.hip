@@ -189,6 +189,9 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
     __shared__ float s_u, s_M;
     __shared__ double s_S;
     __shared__ double s_sc[3];
+    __shared__ Ctl s_ctl;                 // the fold's inputs, loaded by wave 0 at the start
+    __shared__ double s_fin[4];           // outer log-weight, accumulated score
+    __shared__ float s_uin;               // stored resampling uniform
 
     if (a.halt && *a.halt) return;       // the outer layer stopped the run of ticks
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
@@ -228,6 +231,24 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
     // log-likelihood, the decision and the CDF need the weight sum alone.
     const bool moments = a.ring != nullptr;
     const bool squares = moments || a.threshold > 0.0f;
+
+    // The fold between the two phases reads the slot's control block, outer
+    // log-weight, accumulated score and stored resampling uniform.  Wave 0
+    // loads them now, one word per lane, and leaves them in LDS at the end of
+    // the propagate phase, so that the fold waits on no global load.
+    constexpr int kCtlW = (int)(sizeof(Ctl) / 4);
+    static_assert(sizeof(Ctl) % 8 == 0 && kCtlW + 9 <= kWave, "fold inputs: one word per lane of a wave");
+    const size_t ui = (size_t)tick * a.n_slots + m;
+    const bool stored_u = a.noise == kNoiseStored || a.noise == kNoisePropose;
+    uint32_t fold_word = 0;
+    if (wid == 0) {
+        const uint32_t* src = nullptr;
+        if (lane < kCtlW) src = reinterpret_cast<const uint32_t*>(cp) + lane;
+        else if (lane < kCtlW + 2) src = a.outer_logw ? reinterpret_cast<const uint32_t*>(a.outer_logw + m) + (lane - kCtlW) : nullptr;
+        else if (lane < kCtlW + 8) src = (TAN && !FISHER) ? reinterpret_cast<const uint32_t*>(a.score + 3 * m) + (lane - kCtlW - 2) : nullptr;
+        else if (lane == kCtlW + 8) src = stored_u ? reinterpret_cast<const uint32_t*>(a.upool[zc] + ui) : nullptr;
+        if (src) fold_word = *src;
+    }
 
     // Propagate, log-weights (carry as the single filter), tile maxima and
     // the tile sums of the single filter's propagate pass: the lane's KB
@@ -356,6 +377,13 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
             for (int k = 0; k < NS; ++k) s_ts[k][b] = acc[k];
         }
     }
+    if (wid == 0) {
+        uint32_t* dst = nullptr;
+        if (lane < kCtlW) dst = reinterpret_cast<uint32_t*>(&s_ctl) + lane;
+        else if (lane < kCtlW + 8) dst = reinterpret_cast<uint32_t*>(s_fin) + (lane - kCtlW);
+        else if (lane == kCtlW + 8) dst = reinterpret_cast<uint32_t*>(&s_uin);
+        if (dst) *dst = fold_word;
+    }
     __syncthreads();
     // Fold of the filter's tiles (its one reduction group), statistics,
     // log-likelihood and decision -- the single filter's finalise pass with
@@ -397,15 +425,14 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
             const double SX = (((((((0.0 + fg * gsx) + 0.0) + 0.0) + 0.0) + 0.0) + 0.0) + 0.0) + 0.0;
             const double SXX = (((((((0.0 + fg * gsxx) + 0.0) + 0.0) + 0.0) + 0.0) + 0.0) + 0.0) + 0.0;
             const double Q = (((((((0.0 + (fg * fg) * gsq) + 0.0) + 0.0) + 0.0) + 0.0) + 0.0) + 0.0) + 0.0;
-            Ctl c = *cp;
-            const size_t ui = (size_t)tick * a.n_slots + m;
+            Ctl c = s_ctl;
             float ur_in = -1.0f;
             if (a.noise == kNoiseStored) {
-                ur_in = a.upool[zc][ui];
+                ur_in = s_uin;
             } else if (a.noise == kNoisePropose) {
                 const uint64_t pp = (uint64_t)(int64_t)a.prop_seed[m] * jm;
                 const float v = pcg_uniform(pcg_output(pp + (uint64_t)(2 * kBatchMaxN + 1) * jg));
-                float w = a.upool[zc][ui] + a.u_step * (2.0f * v - 1.0f);
+                float w = s_uin + a.u_step * (2.0f * v - 1.0f);
                 w = w - floorf(w);
                 ur_in = (w < 1.0f) ? w : 0.0f;
             }
@@ -414,7 +441,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     if (FISHER) a.score[3 * m + k] = s_sc[k] / S;
-                    else a.score[3 * m + k] += s_sc[k] / S;
+                    else a.score[3 * m + k] = s_fin[1 + k] + s_sc[k] / S;
                 }
             }
             if (c.resample) {
@@ -423,7 +450,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
             }
             *cp = c;
             if (a.ring) a.ring[(size_t)(c.last.tick % a.ring_cap) * a.n_slots + m] = c.last;
-            if (a.outer_logw) a.outer_logw[m] += c.last.ll_incr;
+            if (a.outer_logw) a.outer_logw[m] = s_fin[0] + c.last.ll_incr;
             s_flag = c.resample;
             s_S = c.S;
             s_u = c.u;
@@ -736,7 +763,7 @@ hipError_t launch_batch_init(const BatchInitArgs& a, hipStream_t st) {
 constexpr size_t batch_tick_lds(int block, int kb, bool tan) {
     const size_t nw = (size_t)block / kWave, tb = (size_t)kBatchTile * kb, tp = tb + tb / 16;
     return nw * tb * 4 + nw * tp * 8 + nw * tp * 4 + kBatchMaxTiles * 4 + (tan ? 7 : 4) * kBatchMaxTiles * 8 +
-           (tan ? nw * 3 * tp * 4 : 4) + 2 * kBatchMaxTiles * 8 + 64;
+           (tan ? nw * 3 * tp * 4 : 4) + 2 * kBatchMaxTiles * 8 + 64 + sizeof(Ctl) + 5 * 8;
 }
 constexpr size_t kLdsPerWorkgroup = 160 * 1024;
 
