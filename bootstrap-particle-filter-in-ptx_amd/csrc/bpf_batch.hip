@@ -254,9 +254,12 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
     // the tile sums of the single filter's propagate pass: the lane's KB
     // particles of group q summed in order, one wave sum per group, the four
     // groups added in order.
+    // x and l outlive the loop: without tangents, the wave's last tile is
+    // resampled first, from these registers, in the second phase (with them,
+    // the held registers would cost more than the reload they save)
+    float x[P], l[P];
     for (int b = wid; b < nt; b += NW) {
         const int base = b * TB;
-        float x[P], l[P];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             load_vec<KB>(xb + base + tile_item<KB>(q, 0, lane), &x[q * KB]);
@@ -473,7 +476,13 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
     double* cw = s_c[wid];
     float* xw = s_x[wid];
     const double rn = target_recip(n);
-    for (int b = wid; b < nt; b += NW) {
+    // the wave's tiles in reverse order, so that the first is the one whose
+    // states and log-weights the first phase left in registers; a tile's
+    // outputs do not depend on the order
+    const int wt = (wid < nt) ? (nt - 1 - wid) / NW + 1 : 0;
+    for (int i = wt - 1; i >= 0; --i) {
+        const int b = wid + i * NW;
+        const bool held = !TAN && i == wt - 1;
         const int base = b * TB;
         const float mb = s_tmax[b];
         const double pex = (b == 0) ? 0.0 : s_tp[b - 1];
@@ -486,8 +495,16 @@ __global__ __launch_bounds__(BLOCK) void k_batch_tick(BatchArgs a) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             float lk[KB];
-            load_vec<KB>(xb + base + tile_item<KB>(q, 0, lane), &xv[q * KB]);
-            load_vec<KB>(lb + base + tile_item<KB>(q, 0, lane), lk);
+            if (held) {
+#pragma unroll
+                for (int c = 0; c < KB; ++c) {
+                    xv[q * KB + c] = x[q * KB + c];
+                    lk[c] = l[q * KB + c];
+                }
+            } else {
+                load_vec<KB>(xb + base + tile_item<KB>(q, 0, lane), &xv[q * KB]);
+                load_vec<KB>(lb + base + tile_item<KB>(q, 0, lane), lk);
+            }
 #pragma unroll
             for (int c = 0; c < KB; ++c) {
                 const int j = base + tile_item<KB>(q, c, lane);
