@@ -265,6 +265,9 @@ def test_batch_errors():
     b = gpu_bpf_batch.GpuBpfBatch([100], 0.0, 0.9, 0.1, [1], block=1024, tile=1024)
     with pytest.raises(gpu_bpf.GpuBpfError):
         b.run(np.zeros(3, np.float32), stats=False)
+    # the statistics switch takes 0 or 1
+    with pytest.raises(gpu_bpf.GpuBpfError, match="set_stats"):
+        b._check(b.L.gpu_bpf_batch_set_stats(b.h, 2))
     b.close()
 
 
