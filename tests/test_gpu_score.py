@@ -59,6 +59,23 @@ def test_checkpoint_spacing_is_exact(ydata):
         np.testing.assert_array_equal(gpu_score(n, y, every=every)[1], ref)
 
 
+def test_overlapped_sweeps_are_exact(ydata):
+    """At 2^20 particles the replay of each segment overlaps the reverse sweep
+    of the segment after it (two streams, two buffer sets); the score equals
+    the one-segment run's bit for bit, and a second backward call repeats it."""
+    y = ydata[:96]
+    n = 1 << 20
+    say("overlap", n)
+    ref = gpu_score(n, y, every=96)[1]
+    f = gpu_bpf.GpuBpf(n, RHO, SIG, MU, seed=42)
+    f.score_forward(y, 8)
+    g1 = f.score_backward(True)
+    g2 = f.score_backward(True)
+    f.close()
+    np.testing.assert_array_equal(g1, ref)
+    np.testing.assert_array_equal(g2, ref)
+
+
 @pytest.mark.parametrize("x0", [True, False])
 @pytest.mark.parametrize("items", [1, 4, 16])
 def test_score_matches_oracle(orc, ydata, x0, items):
