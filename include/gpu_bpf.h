@@ -254,12 +254,14 @@ int gpu_bpf_restart(GpuBpfState* state);
  * score_forward runs n_ticks ticks from a fresh filter (it sets the ESS
  * threshold to 0), keeps the particle states every checkpoint_every ticks
  * and returns LL (FP64).  score_backward replays each segment from its
- * checkpoint and sweeps it in reverse (the replay of a segment runs beside
- * the sweep of the one after it, on a second stream); score[3] =
+ * checkpoint and sweeps it in reverse (segments replay on three streams
+ * ahead of the sweep, which runs on a fourth); score[3] =
  * dLL/d(mu, phi, sigma_z), the same bit for bit whatever checkpoint_every.
  * x0_reparam = 1 also differentiates the stationary initial draw
  * x_0 = mu + sigma_z / sqrt(1 - phi^2) z_0.  Memory: (T / checkpoint_every
- * + 2 * checkpoint_every + 4) * N floats. */
+ * + 3 * checkpoint_every + 8) * N floats (three segments replay at once); a
+ * later score_forward of the same n_ticks and checkpoint_every (after
+ * gpu_bpf_restart) keeps these buffers. */
 int gpu_bpf_score_forward(GpuBpfState* state, const float* y, int n_ticks,
                           int checkpoint_every, double* loglik);
 int gpu_bpf_score_backward(GpuBpfState* state, int x0_reparam, double* score);

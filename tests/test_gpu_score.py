@@ -60,20 +60,26 @@ def test_checkpoint_spacing_is_exact(ydata):
 
 
 def test_overlapped_sweeps_are_exact(ydata):
-    """At 2^20 particles the replay of each segment overlaps the reverse sweep
-    of the segment after it (two streams, two buffer sets); the score equals
-    the one-segment run's bit for bit, and a second backward call repeats it."""
+    """At 2^20 particles segments replay on three streams at once, ahead of
+    the reverse sweeps on a fourth; the score equals the one-segment run's bit
+    for bit, a second backward call repeats it, and so does a second forward
+    pass of the same shape after a restart (which keeps the buffers)."""
     y = ydata[:96]
     n = 1 << 20
     say("overlap", n)
     ref = gpu_score(n, y, every=96)[1]
     f = gpu_bpf.GpuBpf(n, RHO, SIG, MU, seed=42)
-    f.score_forward(y, 8)
+    ll1 = f.score_forward(y, 8)
     g1 = f.score_backward(True)
     g2 = f.score_backward(True)
+    f.restart()
+    ll2 = f.score_forward(y, 8)
+    g3 = f.score_backward(True)
     f.close()
     np.testing.assert_array_equal(g1, ref)
     np.testing.assert_array_equal(g2, ref)
+    np.testing.assert_array_equal(g3, ref)
+    assert ll1 == ll2
 
 
 @pytest.mark.parametrize("x0", [True, False])
