@@ -190,43 +190,8 @@ def looped_plain_streams(gpu_bpf, n, mu, phi, sig, seeds, y, ticks, nu=0.0):
     return float(np.sum(n[:len(fs)])) * ticks / dt
 
 
-def extra_workloads(args, gpu_bpf, torch):
-    import gpu_bpf_batch
-    out = {}
-    thr = args.cpu_threads
-    # always-resample N = 65,536, T = 1,000 (the reference's CPU-runnable case)
-    _, y0 = sv_data.sv_series(1000, 0.97, 0.15, -1.0, seed=42)
-    y0 = y0.astype(np.float32)
-    f = gpu_bpf.GpuBpf(65536, 0.97, 0.15, -1.0, seed=42, ess_threshold=0.0)
-    tile, _ = f.layout()
-    t0 = time.perf_counter()
-    _, st = f.run(y0)
-    cold = time.perf_counter() - t0
-    # steady state: the same run again on the same instance after
-    # gpu_bpf_restart (its captured tick graph is reused)
-    f.restart()
-    t0 = time.perf_counter()
-    _, st2 = f.run(y0)
-    wall = time.perf_counter() - t0
-    f.close()
-    if float(st2["ll_cum"][-1]) != float(st["ll_cum"][-1]):
-        raise RuntimeError("C0 rerun after restart differs from the first run")
-    row = {
-        "particle_ticks_per_s": 65536 * 1000 / wall,
-        "ms_per_tick": wall,
-        "first_run_ms_per_tick": cold,
-        "ll_fp64": float(st["ll_cum"][-1]),
-    }
-    if not args.no_cpu:
-        import oracle
-        row["ll_rel_err_vs_fp64_oracle"] = cpu_ll_error(y0, st["ll_cum"][-1], 65536, tile, thr,
-                                                        ess_threshold=0.0)
-        t0 = time.perf_counter()
-        oracle.run_filter(y0[:200], 65536, ess_threshold=0.0, tile=tile, threads=thr)
-        row["cpu_oracle_particle_ticks_per_s"] = 65536 * 200 / (time.perf_counter() - t0)
-        row["cpu_threads"] = thr
-    out["plain_n65536_t1000"] = row
-    log("C0", row)
+def adjoint_workload(args, gpu_bpf, thr):
+    """C2: the differentiable filter's forward pass and pathwise score."""
     # differentiable filter: N = 2^20, T = 2,000, checkpoint every 64 ticks;
     # theta from the host stream of seed 42 (uniform prior box)
     u = host_uniforms(42, 3)
@@ -271,7 +236,47 @@ def extra_workloads(args, gpu_bpf, torch):
                                                         rho=th[1], sigma_z=th[2], mu=th[0],
                                                         ess_threshold=0.0)
         row["ll_err_sample"] = "N=2^20, first 200 ticks"
-    out["adjoint_n1M_t2000"] = row
+    return row
+
+
+def extra_workloads(args, gpu_bpf, torch):
+    import gpu_bpf_batch
+    out = {}
+    thr = args.cpu_threads
+    # always-resample N = 65,536, T = 1,000 (the reference's CPU-runnable case)
+    _, y0 = sv_data.sv_series(1000, 0.97, 0.15, -1.0, seed=42)
+    y0 = y0.astype(np.float32)
+    f = gpu_bpf.GpuBpf(65536, 0.97, 0.15, -1.0, seed=42, ess_threshold=0.0)
+    tile, _ = f.layout()
+    t0 = time.perf_counter()
+    _, st = f.run(y0)
+    cold = time.perf_counter() - t0
+    # steady state: the same run again on the same instance after
+    # gpu_bpf_restart (its captured tick graph is reused)
+    f.restart()
+    t0 = time.perf_counter()
+    _, st2 = f.run(y0)
+    wall = time.perf_counter() - t0
+    f.close()
+    if float(st2["ll_cum"][-1]) != float(st["ll_cum"][-1]):
+        raise RuntimeError("C0 rerun after restart differs from the first run")
+    row = {
+        "particle_ticks_per_s": 65536 * 1000 / wall,
+        "ms_per_tick": wall,
+        "first_run_ms_per_tick": cold,
+        "ll_fp64": float(st["ll_cum"][-1]),
+    }
+    if not args.no_cpu:
+        import oracle
+        row["ll_rel_err_vs_fp64_oracle"] = cpu_ll_error(y0, st["ll_cum"][-1], 65536, tile, thr,
+                                                        ess_threshold=0.0)
+        t0 = time.perf_counter()
+        oracle.run_filter(y0[:200], 65536, ess_threshold=0.0, tile=tile, threads=thr)
+        row["cpu_oracle_particle_ticks_per_s"] = 65536 * 200 / (time.perf_counter() - t0)
+        row["cpu_threads"] = thr
+    out["plain_n65536_t1000"] = row
+    log("C0", row)
+    out["adjoint_n1M_t2000"] = adjoint_workload(args, gpu_bpf, thr)
     log("C2", row)
     # batched SMC^2: CPMMH, M = 512, N log-uniform in [256, 8192], T = 2,000
     T3 = 2000
@@ -540,6 +545,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the other configurations")
     ap.add_argument("--rbpf-only", action="store_true",
                     help="run only the RBPF SMC^2 workload and print its JSON line")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the differentiable filter's forward pass and score (profiling)")
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the two batched inner-filter workloads and print their JSON line")
     ap.add_argument("--batch-ticks", type=int, default=256, help="timed ticks of --batch-only")
@@ -556,6 +563,11 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 5}))
+        return
+    if args.adjoint_only:
+        import gpu_bpf
+        print(json.dumps({"adjoint_n1M_t2000": adjoint_workload(args, gpu_bpf, args.cpu_threads)}),
+              flush=True)
         return
     if args.batch_only:
         print(json.dumps({"inner_batches": batch_only(args)}), flush=True)
