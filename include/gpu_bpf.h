@@ -229,7 +229,9 @@ int gpu_bpf_copy_particles(GpuBpfState* state, float* x, float* log_w);
 int gpu_bpf_get_layout(GpuBpfState* state, int* tile, int* n_tiles);
 
 /* Particles per lane of the per-tile kernels (1, 2, 4, 8 or 16); only before
- * the first step.  0 restores the automatic choice. */
+ * the first step.  0 restores the automatic choice.  A change of tiling drops
+ * a score pass run before it (gpu_bpf_score_backward then asks for a new
+ * gpu_bpf_score_forward). */
 int gpu_bpf_set_items_per_thread(GpuBpfState* state, int items);
 
 /* Wait for all work enqueued on the state's stream. */

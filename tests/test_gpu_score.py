@@ -82,6 +82,25 @@ def test_overlapped_sweeps_are_exact(ydata):
     assert ll1 == ll2
 
 
+def test_retiling_drops_the_score_pass(ydata):
+    """A new tiling after a restart (gpu_bpf_set_items_per_thread) drops the
+    forward pass laid out for the old tiles: the backward call then asks for
+    a new forward pass, which gives the new tiling's score."""
+    y = ydata[:40]
+    n = 20_000
+    say("retile", n)
+    f = gpu_bpf.GpuBpf(n, RHO, SIG, MU, seed=42, items_per_thread=2)
+    f.score_forward(y, 8)
+    f.restart()
+    f._check(f.L.gpu_bpf_set_items_per_thread(f.h, 4))
+    with pytest.raises(gpu_bpf.GpuBpfError, match="score_forward first"):
+        f.score_backward(True)
+    f.score_forward(y, 8)
+    g = f.score_backward(True)
+    f.close()
+    np.testing.assert_array_equal(g, gpu_score(n, y, every=8, items=4)[1])
+
+
 @pytest.mark.parametrize("x0", [True, False])
 @pytest.mark.parametrize("items", [1, 4, 16])
 def test_score_matches_oracle(orc, ydata, x0, items):
